@@ -1,0 +1,188 @@
+/*
+ * mccfr.h — C ABI of the MI355X MCCFR trainer for multiplayer no-limit hold'em.
+ *
+ * Every entry point is plain C: integers, pointers and sizes; no C++ or HIP
+ * types cross the boundary.  Each call returns 0 (MCCFR_OK) on success or a
+ * negative MCCFR_E* code; mccfr_last_error() gives the text.  The library
+ * never exits the process.  A context owns one HIP stream and all device
+ * buffers it allocates; a context is not thread-safe (one host thread at a time).
+ *
+ * The drop-in target is the reference trainer class GPUCFRTrainer
+ * (poker_cuda/src/cfr_gpu.cuh:48-132).  include/mccfr/gpu_cfr_trainer.hpp maps
+ * that class onto these calls with the reference's signatures and defaults.
+ */
+#ifndef MCCFR_MI355X_H
+#define MCCFR_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCCFR_OK 0
+#define MCCFR_E_ARG (-1)      /* invalid argument or configuration */
+#define MCCFR_E_HIP (-2)      /* HIP runtime error (message has the HIP text) */
+#define MCCFR_E_TABLE (-3)    /* hand table missing, wrong size or failed the royal-flush check */
+#define MCCFR_E_IO (-4)       /* file could not be opened, read or written */
+#define MCCFR_E_FORMAT (-5)   /* file has a wrong magic, table size or action count */
+#define MCCFR_E_NOMEM (-6)    /* device allocation failed */
+#define MCCFR_E_STATE (-7)    /* call not valid in the context's current state */
+
+#define MCCFR_MAX_PLAYERS 6
+#define MCCFR_MAX_ACTIONS 16
+#define MCCFR_HAND_TABLE_WORDS 32487834u /* poker_cuda/src/hand_eval.cpp:14 */
+
+/* Action kinds of a betting menu.  The reference's fixed 8-action menu
+ * (poker_cuda/src/abstraction.h:42-53) is
+ *   {FOLD, CHECK, CALL, RAISE_MIN, POT_FRAC 1/2, POT_FRAC 2/3, POT_FRAC 1/1, ALL_IN}. */
+enum mccfr_action_kind {
+  MCCFR_ACT_FOLD = 0,
+  MCCFR_ACT_CHECK = 1,      /* legal iff to_call == 0 */
+  MCCFR_ACT_CALL = 2,       /* legal iff 0 < to_call < stack */
+  MCCFR_ACT_CHECK_CALL = 3, /* check when to_call == 0, call when 0 < to_call < stack */
+  MCCFR_ACT_RAISE_MIN = 4,  /* raise to the minimum legal total */
+  MCCFR_ACT_POT_FRAC = 5,   /* raise to current_bet + max(1, num*pot/den), floored at min raise */
+  MCCFR_ACT_ALL_IN = 6
+};
+
+typedef struct mccfr_action {
+  uint8_t kind; /* enum mccfr_action_kind */
+  uint8_t num;  /* POT_FRAC numerator */
+  uint8_t den;  /* POT_FRAC denominator (> 0) */
+  uint8_t reserved;
+} mccfr_action;
+
+/* Game and abstraction: the constructor arguments of GPUCFRTrainer
+ * (cfr_gpu.cuh:50-55) plus the knobs the reference hard-codes. */
+typedef struct mccfr_game_cfg {
+  int32_t num_players;      /* 2..6 */
+  int32_t starting_stack;   /* chips per seat (reference default 1000) */
+  int32_t small_blind;      /* reference default 10 */
+  int32_t big_blind;        /* reference default 20 */
+  int32_t use_cfr_plus;     /* clamp regrets at 0 after every launch */
+  int32_t use_linear_cfr;   /* strategy-sum weight t = iteration (else 1) */
+  int32_t num_actions;      /* menu length, 1..16 */
+  int32_t max_raises;       /* aggressive actions allowed per street; 0 = unlimited (reference) */
+  int32_t history_bits;     /* bits appended to the action history per action (reference: 3) */
+  int32_t table_log2;       /* info-set slots = 2^table_log2 (reference: 26) */
+  uint64_t seed;            /* Philox key of the traversal streams */
+  mccfr_action menu[MCCFR_MAX_ACTIONS];
+} mccfr_game_cfg;
+
+/* Fills `cfg` with the reference game: its 8-action menu, no raise cap,
+ * 3-bit history and 2^26 slots (cfr_gpu.cuh:17-25, abstraction.h:42-53). */
+void mccfr_game_cfg_reference(mccfr_game_cfg* cfg, int num_players, int starting_stack,
+                              int small_blind, int big_blind, int use_cfr_plus,
+                              int use_linear_cfr);
+
+/* Per-traversal record written by mccfr_run_launch when a trace buffer is
+ * given (test and debugging aid; costs one 16-byte store per traversal). */
+typedef struct mccfr_trace {
+  float utility;       /* terminal payoff of the update player */
+  uint32_t path_hash;  /* FNV-1a over (slot, action) of every decision on the path */
+  uint16_t decisions;  /* decision nodes on the path */
+  uint8_t updates;     /* update-player decision nodes on the path */
+  uint8_t terminal;    /* 1 fold, 2 showdown, 3 depth limit */
+  uint16_t update_legal; /* legal actions summed over the update-player nodes */
+  uint16_t reserved;
+} mccfr_trace;
+
+typedef struct mccfr_ctx mccfr_ctx;
+
+const char* mccfr_last_error(const mccfr_ctx* ctx); /* ctx may be NULL (global errors) */
+
+/* Creates a context on the current HIP device.  Allocates the FP32 regret and
+ * strategy-sum tables, the BF16 strategy snapshot and the launch scratch for
+ * `max_batch` traversals.  Replaces GPUCFRTrainer::GPUCFRTrainer +
+ * alloc_device_buffers (cfr_gpu.cu:1064-1136). */
+int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out);
+void mccfr_destroy(mccfr_ctx* ctx);
+
+/* Hand table.  mccfr_generate_hand_table writes the 32,487,834-word rank-walk
+ * table into a caller buffer (host code, no device needed).  mccfr_set_hand_table
+ * validates a host table (size + royal-flush check, hand_eval.cpp:59-86) and
+ * copies it to HBM; mccfr_load_hand_table does the same from a file
+ * (GPUCFRTrainer::load_hand_table, cfr_gpu.cu:1086-1104);
+ * mccfr_use_generated_hand_table generates it in host memory and uploads it. */
+int mccfr_generate_hand_table(int32_t* hr, size_t words, int threads);
+int mccfr_set_hand_table(mccfr_ctx* ctx, const int32_t* hr, size_t words);
+int mccfr_load_hand_table(mccfr_ctx* ctx, const char* path);
+int mccfr_use_generated_hand_table(mccfr_ctx* ctx);
+
+/* Preflop bucket table (52*52 bytes, [h0*52 + h1]) the trainer uses
+ * (cfr_gpu.cu:1164-1173 over abstraction.cpp:10-59). */
+int mccfr_preflop_lut(uint8_t* out_2704);
+
+/* Seven-card ranks 1..7462 of n hands (7 suit-major card codes each, rows of
+ * `stride` bytes, stride >= 7) already in device memory; ranks written to
+ * device memory.  evaluator 0 = rank-walk table in HBM (eval7_gpu,
+ * cfr_gpu.cu:126-137), 1 = arithmetic perfect-hash evaluator (no table).
+ * A row with a code above 51 or a repeated card gets rank 0. */
+int mccfr_eval7_device(mccfr_ctx* ctx, const uint8_t* d_cards, size_t stride, uint16_t* d_ranks,
+                       size_t n, int evaluator);
+/* Same over host buffers (copies in and out; for tests). */
+int mccfr_eval7_host(mccfr_ctx* ctx, const uint8_t* cards, size_t stride, uint16_t* ranks,
+                     size_t n, int evaluator);
+
+/* One launch of the hot path: traversals first_tid .. first_tid+n-1 for
+ * `update_player` with strategy-sum weight `t_weight`, then the touched-slot
+ * regret update (CFR+ clamp) and strategy refresh.  This is one replay of the
+ * reference graph [regret_matching -> simulate_batch -> clamp]
+ * (cfr_gpu.cu:1286-1322).  `trace` is a HOST array of n records or NULL. */
+int mccfr_run_launch(mccfr_ctx* ctx, int update_player, int64_t t_weight, uint64_t first_tid,
+                     uint32_t n, mccfr_trace* trace);
+
+/* The reference training loop (GPUCFRTrainer::train, cfr_gpu.cu:1244-1378):
+ * for iter = 1..iters, for p = 0..N-1, one launch of `batch` traversals,
+ * replayed from a captured hipGraph.  Traversal ids continue from the
+ * context's running counter. */
+int mccfr_train(mccfr_ctx* ctx, int64_t iters, uint32_t batch);
+
+/* Waits for all work of the context. */
+int mccfr_sync(mccfr_ctx* ctx);
+
+/* Table geometry: slots, padded actions per row. */
+int mccfr_table_shape(const mccfr_ctx* ctx, uint64_t* slots, uint32_t* row_actions);
+
+/* Table transfer.  layout 0 = slot-major rows (AoS, the device layout),
+ * 1 = action-major (SoA, the reference's checkpoint layout, cfr_gpu.cu:1479-1487).
+ * Sizes are element counts (slots * row_actions). Any pointer may be NULL. */
+int mccfr_download_tables(mccfr_ctx* ctx, float* regrets, float* strategy_sum,
+                          uint16_t* strategy_bf16, size_t elems, int layout);
+int mccfr_upload_tables(mccfr_ctx* ctx, const float* regrets, const float* strategy_sum,
+                        size_t elems, int layout);
+
+/* Average strategy S/sum(S) per slot, zero rows where sum <= 1e-7
+ * (kernel_normalize_strategy, cfr_gpu.cu:1029-1058), into a host array. */
+int mccfr_normalized_strategy(mccfr_ctx* ctx, float* out, size_t elems, int layout);
+
+/* Running counters: traversals issued, launches issued, next iteration. */
+int mccfr_counters(const mccfr_ctx* ctx, uint64_t* traversals, uint64_t* launches,
+                   int64_t* next_iteration);
+
+/* Kernel timing of the last `n` launches issued with timing on (ms, device
+ * time between HIP events recorded on the context stream around each kernel). */
+int mccfr_set_timing(mccfr_ctx* ctx, int enabled);
+int mccfr_kernel_times(mccfr_ctx* ctx, float* traverse_ms, float* update_ms, uint32_t n);
+
+/* Device copy of `bytes` bytes between two fresh buffers, `reps` times
+ * (calibration of the HBM byte counters; returns ms per copy). */
+int mccfr_copy_probe(mccfr_ctx* ctx, size_t bytes, int reps, float* ms_per_copy);
+
+/* Writes n hands of 7 distinct uniform cards (rows of 8 bytes, 8th byte 0) to
+ * device memory: hand i is the first 7 positions of a Fisher-Yates shuffle
+ * driven by draw stream (seed, id = i, tag). */
+int mccfr_random_hands_device(mccfr_ctx* ctx, uint64_t seed, uint32_t tag, uint8_t* d_cards8, size_t n);
+
+/* Device-memory plumbing for hosts without their own HIP bindings.
+ * kind: 0 host->device, 1 device->host, 2 device->device (synchronous). */
+int mccfr_device_alloc(mccfr_ctx* ctx, size_t bytes, void** out);
+int mccfr_device_free(mccfr_ctx* ctx, void* p);
+int mccfr_memcpy(mccfr_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCCFR_MI355X_H */

@@ -1,0 +1,94 @@
+/*
+ * oracle.h — CPU restatement of the reference MCCFR hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker or
+ * as the timed CPU baseline.  The product (libmccfr) never links or calls it.
+ *
+ * Every function restates the reference file:line given beside it.  Parity
+ * pins (what checks each function against something other than itself) are
+ * listed in oracle.cpp's header.
+ */
+#ifndef MCCFR_ORACLE_H
+#define MCCFR_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/mccfr/mccfr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- cards and the rank-walk table (hand_eval.cpp:23-57, 107-143; cfr_gpu.cu:87-150) */
+int orc_to_rm(int card);
+uint16_t orc_decode_packed(uint16_t packed);
+uint16_t orc_walk5(const int32_t* hr, const uint8_t* c);
+uint16_t orc_walk6(const int32_t* hr, const uint8_t* c);
+uint16_t orc_walk7(const int32_t* hr, const uint8_t* c);
+void orc_walk7_batch(const int32_t* hr, const uint8_t* cards, size_t stride, uint16_t* out,
+                     size_t n, int threads);
+int orc_validate_table(const int32_t* hr, size_t words); /* hand_eval.cpp:59-86: 1 = ok */
+
+/* ---- independent brute-force evaluator (best 5 of n by enumeration) */
+int orc_bf_class_count(int per_category[9]); /* distinct 5-card classes (7462) per category */
+int orc_bf_class(const uint8_t* cards, int n);
+/* All 133,784,560 seven-card hands: category histogram of the table walk
+ * (index 1..9 by decoded class) and, if check_bf, the number of hands whose
+ * walk rank differs from the brute-force class. */
+int orc_exhaustive7(const int32_t* hr, int threads, int check_bf, uint64_t cat_counts[10],
+                    uint64_t* mismatches);
+/* All 2,598,960 five-card hands: category histogram of the brute-force class. */
+int orc_exhaustive5_bf(uint64_t cat_counts[10]);
+
+/* ---- random streams: Philox4x32-10 (Salmon et al., SC'11) */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+uint32_t orc_draw_u32(uint64_t seed, uint64_t tid, uint32_t tag, uint32_t k);
+float orc_u01(uint32_t x);
+uint16_t orc_f32_to_bf16(float x);
+float orc_bf16_to_f32(uint16_t h);
+
+/* ---- abstraction (abstraction.cpp:10-59, 144-234; cfr_gpu.cu:165-244, 476-489) */
+void orc_preflop_lut(uint8_t out[2704]);
+uint32_t orc_info_hash(uint8_t player, uint8_t hole_b, uint8_t board_b, uint8_t street,
+                       uint32_t action_bits, int table_log2);
+uint32_t orc_valid_mask(const mccfr_game_cfg* cfg, int pot, int stack, int to_call,
+                        int current_bet, int last_full_raise, int raises);
+int orc_chips(const mccfr_game_cfg* cfg, int a, int pot, int stack, int to_call,
+              int current_bet, int last_full_raise);
+int orc_passive_fallback(const mccfr_game_cfg* cfg, uint32_t mask);
+int orc_board_bucket(const int32_t* hr, const uint8_t* hole2, const uint8_t* comm5, int street);
+
+/* Deal of one traversal (cfr_gpu.cu:460-470, 931-937, 963): 2N+5 cards then dealer. */
+void orc_deal(uint64_t seed, uint64_t tid, uint32_t tag, int num_players, uint8_t* cards_out,
+              int* dealer_out);
+
+/* ---- one launch replayed in FP64 (cfr_gpu.cu:318-351, 878-1011, 1018-1022).
+ * R, S: pre-launch regrets/strategy sums, slot-major rows of `row` floats
+ * (row = 8 for menus of <= 8 actions, else 16).  sigma: BF16 snapshot (same
+ * shape) or NULL to derive it from R by regret matching.  dR/dS receive the
+ * FP64 sum of the launch's contributions, aR/aS the FP64 sum of their absolute
+ * values (all four zero-filled by the caller).  trace: n records or NULL.
+ * With dR..aS all NULL only the trace is produced (trace must then be given). */
+int orc_replay_launch(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut,
+                      const float* R, const uint16_t* sigma, int update_player,
+                      int64_t t_weight, uint64_t first_tid, uint32_t n, uint32_t tag,
+                      double* dR, double* dS, double* aR, double* aS, mccfr_trace* trace);
+/* Regret matching of slot-major rows (kernel_regret_matching, cfr_gpu.cu:318-351). */
+void orc_regret_matching(const float* R, uint16_t* sigma, uint64_t slots, int row);
+
+/* ---- the timed CPU baseline: the reference launch sequence in FP32 with
+ * action-major (SoA) tables: full regret-matching sweep, traversals split over
+ * `threads` threads with private delta lists, merge, full CFR+ clamp sweep.
+ * Runs `launches` launches of `batch` traversals cycling the update player
+ * from `first_player`; returns seconds spent (steady clock), or < 0 on error. */
+double orc_cpu_baseline(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut,
+                        float* R_soa, float* S_soa, uint16_t* sigma_soa, int launches,
+                        uint32_t batch, int first_player, int64_t first_iter,
+                        uint64_t first_tid, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+"""ctypes binding of liboracle.so (oracle.cpp): the CPU restatement of the
+reference path.  TEST INFRASTRUCTURE ONLY -- imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_lib = None
+_P = C.c_void_p
+
+_SIGS = {
+    "orc_to_rm": (C.c_int, [C.c_int]),
+    "orc_decode_packed": (C.c_uint16, [C.c_uint16]),
+    "orc_walk5": (C.c_uint16, [_P, _P]),
+    "orc_walk6": (C.c_uint16, [_P, _P]),
+    "orc_walk7": (C.c_uint16, [_P, _P]),
+    "orc_walk7_batch": (None, [_P, _P, C.c_size_t, _P, C.c_size_t, C.c_int]),
+    "orc_validate_table": (C.c_int, [_P, C.c_size_t]),
+    "orc_bf_class_count": (C.c_int, [_P]),
+    "orc_bf_class": (C.c_int, [_P, C.c_int]),
+    "orc_exhaustive7": (C.c_int, [_P, C.c_int, C.c_int, _P, C.POINTER(C.c_uint64)]),
+    "orc_exhaustive5_bf": (C.c_int, [_P]),
+    "orc_philox4x32_10": (None, [_P, _P, _P]),
+    "orc_draw_u32": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_u01": (C.c_float, [C.c_uint32]),
+    "orc_f32_to_bf16": (C.c_uint16, [C.c_float]),
+    "orc_bf16_to_f32": (C.c_float, [C.c_uint16]),
+    "orc_preflop_lut": (None, [_P]),
+    "orc_info_hash": (C.c_uint32, [C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint32, C.c_int]),
+    "orc_valid_mask": (C.c_uint32, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_chips": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_passive_fallback": (C.c_int, [_P, C.c_uint32]),
+    "orc_board_bucket": (C.c_int, [_P, _P, _P, C.c_int]),
+    "orc_deal": (None, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, _P, C.POINTER(C.c_int)]),
+    "orc_replay_launch": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                    _P, _P, _P, _P, _P]),
+    "orc_regret_matching": (None, [_P, _P, C.c_uint64, C.c_int]),
+    "orc_cpu_baseline": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_uint32, C.c_int, C.c_int64,
+                                      C.c_uint64, C.c_int]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C {HERE}`")
+        L = C.CDLL(LIB_PATH)
+        for k, (r, a) in _SIGS.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def p(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def ref_lib():
+    """The reference's own CPU sources compiled under oracle/_ref (None when not built)."""
+    if not os.path.exists(REF_PATH):
+        return None
+    L = C.CDLL(REF_PATH)
+    L.ref_hand_eval_init.argtypes = [C.c_char_p]
+    L.ref_evaluate_7cards.argtypes = [_P]
+    L.ref_evaluate_best.argtypes = [_P, _P, C.c_int]
+    L.ref_preflop_bucket.argtypes = [C.c_int, C.c_int]
+    L.ref_valid_actions_mask.argtypes = [C.c_int] * 6
+    L.ref_action_to_chips.argtypes = [C.c_int] * 7
+    return L
+
+
+def replay_trace(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0):
+    """Per-traversal records only (no delta accumulation): for full-size tables."""
+    tr = np.zeros(n, dtype=np.dtype([("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"),
+                                     ("updates", "u1"), ("terminal", "u1"),
+                                     ("update_legal", "<u2"), ("reserved", "<u2")]))
+    R = np.ascontiguousarray(R, np.float32)
+    sig_p = None if sigma is None else p(np.ascontiguousarray(sigma, np.uint16))
+    rc = lib().orc_replay_launch(C.byref(cfg), p(hr), p(lut), p(R), sig_p, up, t_weight, first_tid, n, tag,
+                                 None, None, None, None, p(tr))
+    if rc != 0:
+        raise RuntimeError("orc_replay_launch rejected its arguments")
+    return tr
+
+
+def replay_launch(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, trace=True):
+    """FP64 replay of one launch from pre-launch tables (slot-major rows).
+    Returns (dR, dS, aR, aS, trace)."""
+    row = 8 if cfg.num_actions <= 8 else 16
+    size = (1 << cfg.table_log2) * row
+    dR = np.zeros(size); dS = np.zeros(size); aR = np.zeros(size); aS = np.zeros(size)
+    tr = np.zeros(n, dtype=np.dtype([("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"),
+                                     ("updates", "u1"), ("terminal", "u1"),
+                                     ("update_legal", "<u2"), ("reserved", "<u2")])) if trace else None
+    R = np.ascontiguousarray(R, np.float32)
+    sig_p = None if sigma is None else p(np.ascontiguousarray(sigma, np.uint16))
+    rc = lib().orc_replay_launch(C.byref(cfg), p(hr), p(lut), p(R), sig_p, up, t_weight, first_tid, n, tag,
+                                 p(dR), p(dS), p(aR), p(aS), p(tr) if trace else None)
+    if rc != 0:
+        raise RuntimeError("orc_replay_launch rejected its arguments")
+    return dR, dS, aR, aS, tr
+
+
+def regret_matching(R, row):
+    R = np.ascontiguousarray(R, np.float32)
+    out = np.empty(R.size, np.uint16)
+    lib().orc_regret_matching(p(R), p(out), R.size // row, row)
+    return out

@@ -1,0 +1,45 @@
+// Launch wrappers of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace mccfr {
+
+// Device tables of one game (slot-major rows of `row` entries).
+struct DevTables {
+  float* R;          // regrets
+  float* S;          // strategy sums
+  uint16_t* sig;     // BF16 regret-matched strategy snapshot
+  double* D;         // per-launch FP64 deltas: [slot][0..row) regret, [row..2row) strategy sum
+  uint32_t* stamp;   // last launch epoch that touched the slot
+  uint32_t* touched; // touched-slot list of the current launch
+  uint32_t touched_cap;
+};
+
+// Tables of the arithmetic 7-card evaluator (built on the host).
+constexpr int kNonFlushClasses = 49205;  // rank multisets of 7 cards, <= 4 per rank
+struct ArithTables {
+  uint16_t* nonflush;  // [49205]
+  uint16_t* flush;     // [8192]
+  uint16_t* offsets;   // [13][8][5]
+};
+
+void launch_set_state(hipStream_t s, LaunchState* st, int update_player, int64_t iteration,
+                      int64_t t_weight, uint64_t first_tid, uint32_t n);
+void launch_advance_state(hipStream_t s, LaunchState* st, int num_players, int use_linear_cfr,
+                          uint32_t batch);
+void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, const int32_t* hr,
+                     const uint8_t* lut, DevTables t, uint32_t n_lanes, mccfr_trace* trace);
+void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t);
+void launch_rm_sweep(hipStream_t s, int row, DevTables t, uint64_t slots);
+void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);
+void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, size_t stride,
+                       uint16_t* out, size_t n);
+void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size_t stride,
+                        uint16_t* out, size_t n);
+void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* out, size_t n);
+void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16);
+
+}  // namespace mccfr

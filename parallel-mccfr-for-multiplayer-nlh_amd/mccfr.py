@@ -1,0 +1,341 @@
+"""ctypes binding of libmccfr.so (the C ABI in include/mccfr/mccfr.h).
+
+Python-side mirror of the reference trainer interface for tests and the
+benchmark.  The library is the built ``libmccfr.so`` next to this file; a
+missing library raises ``MccfrError`` -- there is no CPU fallback.
+
+``Trainer`` follows ``GPUCFRTrainer`` (poker_cuda/src/cfr_gpu.cuh:48-132):
+constructor arguments, ``load_hand_table``, ``train``, ``get_strategy``,
+``export_tables``/``import_tables``, ``num_info_sets_active``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmccfr.so")
+
+MAX_ACTIONS = 16
+HAND_TABLE_WORDS = 32487834
+
+FOLD, CHECK, CALL, CHECK_CALL, RAISE_MIN, POT_FRAC, ALL_IN = range(7)
+
+
+class MccfrError(RuntimeError):
+    pass
+
+
+class Action(C.Structure):
+    _fields_ = [("kind", C.c_uint8), ("num", C.c_uint8), ("den", C.c_uint8), ("reserved", C.c_uint8)]
+
+
+class GameCfg(C.Structure):
+    _fields_ = [
+        ("num_players", C.c_int32),
+        ("starting_stack", C.c_int32),
+        ("small_blind", C.c_int32),
+        ("big_blind", C.c_int32),
+        ("use_cfr_plus", C.c_int32),
+        ("use_linear_cfr", C.c_int32),
+        ("num_actions", C.c_int32),
+        ("max_raises", C.c_int32),
+        ("history_bits", C.c_int32),
+        ("table_log2", C.c_int32),
+        ("seed", C.c_uint64),
+        ("menu", Action * MAX_ACTIONS),
+    ]
+
+
+TRACE_DTYPE = np.dtype(
+    [("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"), ("updates", "u1"), ("terminal", "u1"),
+     ("update_legal", "<u2"), ("reserved", "<u2")]
+)
+
+
+class Trace(C.Structure):
+    _fields_ = [
+        ("utility", C.c_float),
+        ("path_hash", C.c_uint32),
+        ("decisions", C.c_uint16),
+        ("updates", C.c_uint8),
+        ("terminal", C.c_uint8),
+        ("update_legal", C.c_uint16),
+        ("reserved", C.c_uint16),
+    ]
+
+
+
+_lib: Optional[C.CDLL] = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mccfr_last_error": (C.c_char_p, [C.c_void_p]),
+    "mccfr_game_cfg_reference": (None, [C.POINTER(GameCfg), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mccfr_create": (C.c_int, [C.POINTER(GameCfg), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mccfr_destroy": (None, [C.c_void_p]),
+    "mccfr_generate_hand_table": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_set_hand_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "mccfr_load_hand_table": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "mccfr_use_generated_hand_table": (C.c_int, [C.c_void_p]),
+    "mccfr_preflop_lut": (C.c_int, [C.c_void_p]),
+    "mccfr_eval7_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_eval7_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_run_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "mccfr_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
+    "mccfr_sync": (C.c_int, [C.c_void_p]),
+    "mccfr_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "mccfr_download_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_upload_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_normalized_strategy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
+    "mccfr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "mccfr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mccfr_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
+    "mccfr_random_hands_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "mccfr_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "mccfr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mccfr_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads libmccfr.so (raises MccfrError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MccfrError(f"{LIB_PATH} is missing: build it with `make -C {HERE}`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)
+
+
+def _check(rc: int, ctx=None) -> None:
+    if rc != 0:
+        msg = lib().mccfr_last_error(ctx)
+        raise MccfrError(f"libmccfr error {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------------------
+# Game configurations
+# ---------------------------------------------------------------------------
+def reference_cfg(num_players=2, starting_stack=1000, small_blind=10, big_blind=20, use_cfr_plus=True,
+                  use_linear_cfr=True, table_log2=26, seed=42) -> GameCfg:
+    """The reference game (GPUCFRTrainer defaults, cfr_gpu.cuh:50-55) with its 8-action menu."""
+    cfg = GameCfg()
+    lib().mccfr_game_cfg_reference(C.byref(cfg), num_players, starting_stack, small_blind, big_blind,
+                                   int(use_cfr_plus), int(use_linear_cfr))
+    cfg.table_log2 = table_log2
+    cfg.seed = seed
+    return cfg
+
+
+def menu_cfg(num_players, starting_stack, menu, max_raises=0, table_log2=26, seed=0, small_blind=10,
+             big_blind=20, use_cfr_plus=True, use_linear_cfr=True, history_bits=None) -> GameCfg:
+    """A game with a custom menu: list of (kind, num, den)."""
+    cfg = GameCfg()
+    cfg.num_players = num_players
+    cfg.starting_stack = starting_stack
+    cfg.small_blind = small_blind
+    cfg.big_blind = big_blind
+    cfg.use_cfr_plus = int(use_cfr_plus)
+    cfg.use_linear_cfr = int(use_linear_cfr)
+    cfg.num_actions = len(menu)
+    cfg.max_raises = max_raises
+    cfg.history_bits = history_bits if history_bits else max(1, (len(menu) - 1).bit_length())
+    cfg.table_log2 = table_log2
+    cfg.seed = seed
+    for i, (k, n, d) in enumerate(menu):
+        cfg.menu[i] = Action(k, n, d, 0)
+    return cfg
+
+
+# Menus of the benchmark configurations.
+MENU_FCPA = [(FOLD, 0, 1), (CHECK_CALL, 0, 1), (POT_FRAC, 1, 1), (ALL_IN, 0, 1)]
+MENU_6 = [(FOLD, 0, 1), (CHECK_CALL, 0, 1), (POT_FRAC, 1, 2), (POT_FRAC, 1, 1), (POT_FRAC, 2, 1), (ALL_IN, 0, 1)]
+MENU_10 = [(FOLD, 0, 1), (CHECK_CALL, 0, 1), (POT_FRAC, 1, 4), (POT_FRAC, 1, 2), (POT_FRAC, 3, 4),
+           (POT_FRAC, 1, 1), (POT_FRAC, 3, 2), (POT_FRAC, 2, 1), (POT_FRAC, 3, 1), (ALL_IN, 0, 1)]
+
+
+def cfg_3p_20bb(table_log2=22, seed=0) -> GameCfg:
+    """3 players, 20bb (400 chips at 10/20), {fold, check/call, pot, all-in}, <= 2 raises per street."""
+    return menu_cfg(3, 400, MENU_FCPA, max_raises=2, table_log2=table_log2, seed=seed)
+
+
+def cfg_6p_100bb(table_log2=26, seed=0) -> GameCfg:
+    """6 players, 100bb (2,000 chips), {fold, check/call, 1/2, 1, 2 pot, all-in}, <= 3 raises per street."""
+    return menu_cfg(6, 2000, MENU_6, max_raises=3, table_log2=table_log2, seed=seed)
+
+
+def cfg_6p_200bb_10(table_log2=28, seed=0) -> GameCfg:
+    """6 players, 200bb (4,000 chips), 10-action menu, <= 4 raises per street."""
+    return menu_cfg(6, 4000, MENU_10, max_raises=4, table_log2=table_log2, seed=seed)
+
+
+# ---------------------------------------------------------------------------
+# Hand table
+# ---------------------------------------------------------------------------
+_table_cache: Optional[np.ndarray] = None
+
+
+def generate_hand_table(threads: int = 0) -> np.ndarray:
+    """The 32,487,834-word rank-walk table (generated once per process)."""
+    global _table_cache
+    if _table_cache is None:
+        hr = np.empty(HAND_TABLE_WORDS, dtype=np.int32)
+        _check(lib().mccfr_generate_hand_table(_ptr(hr), hr.size, threads))
+        _table_cache = hr
+    return _table_cache
+
+
+def preflop_lut() -> np.ndarray:
+    out = np.empty(2704, dtype=np.uint8)
+    _check(lib().mccfr_preflop_lut(_ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Context
+# ---------------------------------------------------------------------------
+class Context:
+    """Owns one mccfr_ctx (one game, its device tables and stream)."""
+
+    def __init__(self, cfg: GameCfg, max_batch: int):
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(lib().mccfr_create(C.byref(cfg), max_batch, C.byref(h)))
+        self.h = h
+        self.max_batch = max_batch
+        s = C.c_uint64()
+        r = C.c_uint32()
+        _check(lib().mccfr_table_shape(h, C.byref(s), C.byref(r)), h)
+        self.slots, self.row = int(s.value), int(r.value)
+
+    def close(self):
+        if self.h:
+            lib().mccfr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc):
+        _check(rc, self.h)
+
+    # hand table -----------------------------------------------------------
+    def use_generated_hand_table(self):
+        self.check(lib().mccfr_use_generated_hand_table(self.h))
+
+    def set_hand_table(self, hr: np.ndarray):
+        hr = np.ascontiguousarray(hr, dtype=np.int32)
+        self.check(lib().mccfr_set_hand_table(self.h, _ptr(hr), hr.size))
+
+    def load_hand_table(self, path: str):
+        self.check(lib().mccfr_load_hand_table(self.h, path.encode()))
+
+    # evaluation ------------------------------------------------------------
+    def eval7(self, cards: np.ndarray, evaluator: int = 0) -> np.ndarray:
+        cards = np.ascontiguousarray(cards, dtype=np.uint8)
+        n, stride = cards.shape
+        out = np.empty(n, dtype=np.uint16)
+        self.check(lib().mccfr_eval7_host(self.h, _ptr(cards), stride, _ptr(out), n, evaluator))
+        return out
+
+    # launches --------------------------------------------------------------
+    def run_launch(self, update_player: int, t_weight: int, first_tid: int, n: int, trace: bool = False):
+        tr = np.zeros(n, dtype=TRACE_DTYPE) if trace else None
+        self.check(lib().mccfr_run_launch(self.h, update_player, t_weight, first_tid, n,
+                                          _ptr(tr) if trace else None))
+        return tr
+
+    def train(self, iters: int, batch: int):
+        self.check(lib().mccfr_train(self.h, iters, batch))
+
+    def sync(self):
+        self.check(lib().mccfr_sync(self.h))
+
+    def download(self, layout: int = 0):
+        n = self.slots * self.row
+        R = np.empty(n, np.float32)
+        S = np.empty(n, np.float32)
+        sig = np.empty(n, np.uint16)
+        self.check(lib().mccfr_download_tables(self.h, _ptr(R), _ptr(S), _ptr(sig), n, layout))
+        return R, S, sig
+
+    def upload(self, R: Optional[np.ndarray], S: Optional[np.ndarray], layout: int = 0):
+        n = self.slots * self.row
+        Rc = None if R is None else np.ascontiguousarray(R, dtype=np.float32)
+        Sc = None if S is None else np.ascontiguousarray(S, dtype=np.float32)
+        self.check(lib().mccfr_upload_tables(self.h, None if Rc is None else _ptr(Rc),
+                                             None if Sc is None else _ptr(Sc), n, layout))
+
+    def normalized_strategy(self, layout: int = 0) -> np.ndarray:
+        n = self.slots * self.row
+        out = np.empty(n, np.float32)
+        self.check(lib().mccfr_normalized_strategy(self.h, _ptr(out), n, layout))
+        return out
+
+    def counters(self):
+        t, l, i = C.c_uint64(), C.c_uint64(), C.c_int64()
+        self.check(lib().mccfr_counters(self.h, C.byref(t), C.byref(l), C.byref(i)))
+        return int(t.value), int(l.value), int(i.value)
+
+    def set_timing(self, on: bool):
+        self.check(lib().mccfr_set_timing(self.h, int(on)))
+
+    def kernel_times(self, n: int):
+        a = np.empty(n, np.float32)
+        b = np.empty(n, np.float32)
+        self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), n))
+        return a, b
+
+    def copy_probe(self, nbytes: int, reps: int) -> float:
+        ms = C.c_float()
+        self.check(lib().mccfr_copy_probe(self.h, nbytes, reps, C.byref(ms)))
+        return float(ms.value)
+
+    # device memory -----------------------------------------------------------
+    def dalloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(lib().mccfr_device_alloc(self.h, nbytes, C.byref(p)))
+        return int(p.value)
+
+    def dfree(self, p: int):
+        self.check(lib().mccfr_device_free(self.h, C.c_void_p(p)))
+
+    def h2d(self, dst: int, src: np.ndarray):
+        self.check(lib().mccfr_memcpy(self.h, C.c_void_p(dst), _ptr(src), src.nbytes, 0))
+
+    def d2h(self, dst: np.ndarray, src: int):
+        self.check(lib().mccfr_memcpy(self.h, _ptr(dst), C.c_void_p(src), dst.nbytes, 1))
+
+    def random_hands_device(self, seed: int, tag: int, dptr: int, n: int):
+        self.check(lib().mccfr_random_hands_device(self.h, seed, tag, C.c_void_p(dptr), n))
+
+    def eval7_device(self, dcards: int, stride: int, dranks: int, n: int, evaluator: int):
+        self.check(lib().mccfr_eval7_device(self.h, C.c_void_p(dcards), stride, C.c_void_p(dranks), n, evaluator))

@@ -1,0 +1,81 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own CPU
+code (poker_cuda/src/abstraction.cpp and hand_eval.cpp, compiled from where
+they lie into oracle/_ref/libref.so by `make -C oracle ref`).
+
+Fixtures (data only):
+  preflop_lut.bin   2,704 bytes: preflop_bucket(h0, h1) for every ordered pair,
+                    0 on the diagonal (abstraction.cpp:42-59, cfr_gpu.cu:1164-1173)
+  rules_grid.npz    valid_actions_mask and action_to_chips of the reference's
+                    8-action menu over a grid of betting states (abstraction.cpp:144-234)
+  hand_walks.npz    4,096 seeded 7-card deals with the reference's
+                    evaluate_best at 3/4/5 board cards and evaluate_7cards, run on
+                    the generated rank-walk table (hand_eval.cpp:107-173)
+
+Usage: python tests/golden/make_golden.py   (needs oracle/_ref/libref.so)
+"""
+import ctypes as C
+import itertools
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "parallel-mccfr-for-multiplayer-nlh_amd"))
+import oracle  # noqa: E402
+import mccfr  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+POTS = [0, 1, 15, 30, 45, 100, 333, 4000]
+STACKS = [0, 1, 19, 20, 40, 100, 400, 2000]
+TO_CALL = [0, 10, 20, 40, 100, 400, 2000]
+CUR_BET = [0, 20, 40, 100, 400]
+LFR = [0, 20, 40, 100]
+BBS = [2, 20]
+
+
+def main():
+    ref = oracle.ref_lib()
+    if ref is None:
+        sys.exit("oracle/_ref/libref.so not built (make -C oracle ref)")
+
+    ref.ref_abstraction_init()
+    lut = np.zeros(2704, np.uint8)
+    for a in range(52):
+        for b in range(52):
+            lut[a * 52 + b] = 0 if a == b else ref.ref_preflop_bucket(a, b)
+    lut.tofile(os.path.join(OUT, "preflop_lut.bin"))
+
+    states = np.array(list(itertools.product(POTS, STACKS, TO_CALL, CUR_BET, LFR, BBS)), dtype=np.int32)
+    masks = np.zeros(len(states), np.uint8)
+    chips = np.zeros((len(states), 8), np.int32)
+    for i, (pot, stack, tc, cb, lfr, bb) in enumerate(states.tolist()):
+        masks[i] = ref.ref_valid_actions_mask(pot, stack, tc, cb, lfr, bb)
+        for a in range(8):
+            chips[i, a] = ref.ref_action_to_chips(a, pot, stack, tc, cb, lfr, bb)
+    np.savez_compressed(os.path.join(OUT, "rules_grid.npz"), states=states, masks=masks, chips=chips)
+
+    hr = mccfr.generate_hand_table()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "handranks.dat")
+        hr.tofile(path)
+        if ref.ref_hand_eval_init(path.encode()) != 1:
+            sys.exit("reference loader rejected the generated table")
+    rng = np.random.default_rng(20261203)
+    cards = np.stack([rng.permutation(52)[:7] for _ in range(4096)]).astype(np.uint8)
+    ranks = np.zeros((len(cards), 4), np.uint16)  # flop, turn, river (evaluate_best), evaluate_7cards
+    for i, c in enumerate(cards):
+        h = np.ascontiguousarray(c[:2])
+        b = np.ascontiguousarray(c[2:])
+        for k, n_comm in enumerate((3, 4, 5)):
+            ranks[i, k] = ref.ref_evaluate_best(oracle.p(h), oracle.p(b), n_comm)
+        ranks[i, 3] = ref.ref_evaluate_7cards(oracle.p(np.ascontiguousarray(c)))
+    np.savez_compressed(os.path.join(OUT, "hand_walks.npz"), cards=cards, ranks=ranks)
+    print("wrote preflop_lut.bin, rules_grid.npz, hand_walks.npz")
+
+
+if __name__ == "__main__":
+    main()

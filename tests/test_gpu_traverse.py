@@ -1,0 +1,110 @@
+"""GPU traversal launches against the oracle's FP64 replay, launch by launch
+(teacher-forced from the GPU's own pre-launch tables):
+
+* per traversal, bit-exact: terminal utility (FP32 bits), path hash over every
+  (slot, action) decision, decision and update-node counts, terminal kind;
+* the BF16 strategy snapshot equals regret matching of the FP32 regrets, bit-exact;
+* regrets and strategy sums after the launch satisfy
+  |x - x_hat| <= 1e-5 * max(|x_hat|, sum|delta|), x_hat = pre + FP64 sum of contributions.
+"""
+import numpy as np
+import pytest
+
+import helpers
+import mccfr
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def run_stepwise(cfg, hand_table, lut, n, launches, first_tid=0):
+    row = 8 if cfg.num_actions <= 8 else 16
+    N = cfg.num_players
+    with mccfr.Context(cfg, n) as ctx:
+        ctx.set_hand_table(hand_table)
+        tid = first_tid
+        for L in range(launches):
+            up, t = L % N, L // N + 1
+            w = t if cfg.use_linear_cfr else 1
+            R0, S0, sig0 = ctx.download()
+            assert np.array_equal(sig0, oracle.regret_matching(R0, row)), f"snapshot, launch {L}"
+            print(f"launch {L}: up={up} t={w} tid={tid} n={n}", flush=True)
+            tr = ctx.run_launch(up, w, tid, n, trace=True)
+            dR, dS, aR, aS, otr = oracle.replay_launch(cfg, hand_table, lut, R0, sig0, up, w, tid, n)
+            for f in ("path_hash", "decisions", "updates", "terminal"):
+                assert np.array_equal(tr[f], otr[f]), (f, L)
+            assert np.array_equal(tr["utility"].view(np.uint32), otr["utility"].view(np.uint32)), L
+            R1, S1, sig1 = ctx.download()
+            eR, eS = helpers.expected_post(R0, S0, dR, dS, bool(cfg.use_cfr_plus))
+            okR = helpers.within_tol(R1, eR, aR)
+            okS = helpers.within_tol(S1, eS, aS)
+            assert okR.all(), (L, np.argwhere(~okR)[:5])
+            assert okS.all(), (L, np.argwhere(~okS)[:5])
+            tid += n
+        R, S, sig = ctx.download()
+        assert np.array_equal(sig, oracle.regret_matching(R, row))
+        return R, S, tid
+
+
+CONFIGS = {
+    "ref_2p_50bb": lambda: mccfr.reference_cfg(num_players=2, starting_stack=1000, table_log2=14, seed=1),
+    "ref_3p_20bb": lambda: mccfr.reference_cfg(num_players=3, starting_stack=400, table_log2=14, seed=2),
+    "ref_6p_100bb": lambda: mccfr.reference_cfg(num_players=6, starting_stack=2000, table_log2=16, seed=3),
+    "3p_20bb_fcpa_cap2": lambda: mccfr.cfg_3p_20bb(table_log2=14, seed=4),
+    "6p_100bb_menu6_cap3": lambda: mccfr.cfg_6p_100bb(table_log2=16, seed=5),
+    "6p_200bb_menu10_cap4": lambda: mccfr.cfg_6p_200bb_10(table_log2=16, seed=6),
+    "ref_2p_short_allin": lambda: mccfr.reference_cfg(num_players=2, starting_stack=30, table_log2=12, seed=7),
+    "ref_4p_no_cfrplus_no_lcfr": lambda: mccfr.reference_cfg(num_players=4, starting_stack=1000,
+                                                             use_cfr_plus=False, use_linear_cfr=False,
+                                                             table_log2=14, seed=8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_stepwise_parity(name, hand_table, golden_lut):
+    cfg = CONFIGS[name]()
+    R, S, _ = run_stepwise(cfg, hand_table, golden_lut, n=4096, launches=3 * cfg.num_players)
+    assert np.count_nonzero(S) > 0
+    if cfg.use_cfr_plus:
+        assert (R >= 0).all()
+
+
+def test_train_graph_equals_launch_sequence(hand_table, golden_lut):
+    """mccfr_train (captured hipGraph, device-advanced launch state) gives the
+    tables of the same launches issued one by one and checked against the oracle."""
+    cfg = mccfr.cfg_3p_20bb(table_log2=14, seed=11)
+    n, iters = 2048, 3
+    R_seq, S_seq, _ = run_stepwise(cfg, hand_table, golden_lut, n=n, launches=iters * cfg.num_players)
+    with mccfr.Context(cfg, n) as ctx:
+        ctx.set_hand_table(hand_table)
+        ctx.train(iters, n)
+        R, S, sig = ctx.download()
+        assert ctx.counters() == (iters * cfg.num_players * n, iters * cfg.num_players, iters + 1)
+    assert np.allclose(R, R_seq, rtol=1e-6, atol=1e-6)
+    assert np.allclose(S, S_seq, rtol=1e-6, atol=1e-6)
+
+
+def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
+    """6 players, 100bb, 2^26-slot tables, 2^20 traversals per launch: after
+    warm-up launches, every lane of a launch is replayed by the oracle
+    (trace-only) from the downloaded pre-launch tables: bit-exact; table-wide
+    strategy-sum mass equals t * (update nodes)."""
+    cfg = mccfr.cfg_6p_100bb(table_log2=26, seed=0)
+    n = 1 << 20
+    with mccfr.Context(cfg, n) as ctx:
+        ctx.set_hand_table(hand_table)
+        tid = 0
+        for L in range(6):  # one iteration, all six seats
+            ctx.run_launch(L, 1, tid, n)
+            tid += n
+        R0, S0, sig0 = ctx.download()
+        tr = ctx.run_launch(0, 2, tid, n, trace=True)
+        R1, S1, _ = ctx.download()
+        otr = oracle.replay_trace(cfg, hand_table, golden_lut, R0, sig0, 0, 2, tid, n)
+        for f in ("path_hash", "decisions", "updates", "terminal"):
+            assert np.array_equal(tr[f], otr[f]), f
+        assert np.array_equal(tr["utility"].view(np.uint32), otr["utility"].view(np.uint32))
+        mass = float(S1.astype(np.float64).sum() - S0.astype(np.float64).sum())
+        nodes = int(tr["updates"].astype(np.int64).sum())
+        assert abs(mass - 2.0 * nodes) <= 1e-4 * 2.0 * nodes
+        assert (R1 >= 0).all()
