@@ -100,10 +100,14 @@ def main():
     ap.add_argument("--table-log2", type=int, default=26)
     ap.add_argument("--hand-log2", type=int, default=28)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cpus)")
-    ap.add_argument("--cpu-launches", type=int, default=2)
+    ap.add_argument("--cpu-launches", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hand-bench", action="store_true")
     ap.add_argument("--profile", action="store_true", help="timed launches only (for rocprofv3 runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="with --profile: also run a plain device copy of this size (byte-counter calibration)")
+    ap.add_argument("--table-seed", type=int, default=2026,
+                    help="seed of the synthetic regret state the timed launches start from")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -128,6 +132,24 @@ def main():
         state["tid"] += B
         return tr
 
+    fresh = None
+    if not args.profile:
+        # Fresh (all-zero) tables, as the reference's benchmark mode runs them
+        # (main.cpp:380-398): every node takes the passive fallback, and the
+        # 6-handed check-down ends at the depth guard.
+        for _ in range(3):
+            step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(6):
+            step()
+        ctx.sync()
+        fresh = 6 * B / (time.perf_counter() - t0)
+        log(f"bench: fresh tables {fresh / 1e6:.1f} M traversals/s")
+
+    # Timed state: seeded random regrets (mixed strategies at every info set),
+    # so paths mix folds, calls, raises, all-ins and showdowns.
+    ctx.randomize_tables(args.table_seed)
     t0 = time.perf_counter()
     for _ in range(args.warmup):
         step()
@@ -154,8 +176,13 @@ def main():
                                "{F, C/C, 1/2P, 1P, 2P, AI}, <=3 raises/street",
                    "table_slots": 1 << args.table_log2, "traversals_per_launch": B, "parallelism": "single GPU"},
         "kernel_ms": {"traverse": float(trav_ms.mean()), "update": float(upd_ms.mean())},
+        "table_state": f"seeded random regrets (seed {args.table_seed}) + {args.warmup} warm-up launches",
+        "fresh_tables_traversals_per_s": fresh,
     }
     if args.profile:
+        if args.copy_probe_mib:
+            result["copy_probe"] = {"bytes": args.copy_probe_mib << 20,
+                                    "ms": ctx.copy_probe(args.copy_probe_mib << 20, 2)}
         print(json.dumps(result), flush=True)
         return
 

@@ -140,6 +140,24 @@ int mccfr_run_launch(mccfr_ctx* ctx, int update_player, int64_t t_weight, uint64
  * context's running counter. */
 int mccfr_train(mccfr_ctx* ctx, int64_t iters, uint32_t batch);
 
+/* Grows (or shrinks) the per-launch scratch to `max_batch` traversals; the
+ * tables are kept.  Needed when a later train() uses a larger batch. */
+int mccfr_set_max_batch(mccfr_ctx* ctx, uint32_t max_batch);
+
+/* Checkpoint in the reference's .ckpt layout (cfr_gpu.cu:1464-1538):
+ * u64 magic 0x43465247505500, u32 table_size, u32 num_actions (row width),
+ * f32 regrets[num_actions][table_size], f32 strategy_sum[num_actions][table_size].
+ * Loading checks table_size and num_actions against the context
+ * (MCCFR_E_FORMAT) and refreshes the strategy snapshot before the next launch. */
+int mccfr_save_checkpoint(mccfr_ctx* ctx, const char* path);
+int mccfr_load_checkpoint(mccfr_ctx* ctx, const char* path);
+
+/* Synthetic table state for benchmarks and tests: regret[slot][a] = a uniform
+ * (0, 1] draw of stream (seed, id = slot, tag 0x72616E64) for the menu's
+ * actions (0 in padding), strategy sums 0, snapshot refreshed.  Paths sampled
+ * from it mix folds, calls, raises and all-ins like a partly trained table. */
+int mccfr_randomize_tables(mccfr_ctx* ctx, uint64_t seed);
+
 /* Waits for all work of the context. */
 int mccfr_sync(mccfr_ctx* ctx);
 

@@ -653,6 +653,24 @@ __global__ __launch_bounds__(256) void k_random_hands(uint32_t seed_lo, uint32_t
   }
 }
 
+template <int ROW>
+__global__ __launch_bounds__(256) void k_randomize(DevTables T, uint64_t slots, int num_actions, uint32_t seed_lo,
+                                                   uint32_t seed_hi) {
+  for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
+       slot += (uint64_t)gridDim.x * blockDim.x) {
+    DrawStream rng;
+    rng.init(seed_lo, seed_hi, slot, 0x72616E64u);
+    float r[ROW];
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) r[a] = a < num_actions ? u01(rng.next()) : 0.0f;
+#pragma unroll
+    for (int v = 0; v < ROW / 4; ++v) {
+      reinterpret_cast<float4*>(T.R + slot * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
+      reinterpret_cast<float4*>(T.S + slot * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -714,6 +732,15 @@ void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size
 void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* out, size_t n) {
   hipLaunchKernelGGL(k_random_hands, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, (uint32_t)seed,
                      (uint32_t)(seed >> 32), tag, out, n);
+}
+void launch_randomize(hipStream_t s, int row, int num_actions, DevTables t, uint64_t slots, uint64_t seed) {
+  const unsigned grid = grid_for(slots, 256, 65536);
+  if (row == 8)
+    hipLaunchKernelGGL(k_randomize<8>, dim3(grid), dim3(256), 0, s, t, slots, num_actions, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
+  else
+    hipLaunchKernelGGL(k_randomize<16>, dim3(grid), dim3(256), 0, s, t, slots, num_actions, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
 }
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 256, 1u << 16)), dim3(256), 0, s, src, dst, n16);

@@ -40,6 +40,7 @@ void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, s
 void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size_t stride,
                         uint16_t* out, size_t n);
 void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* out, size_t n);
+void launch_randomize(hipStream_t s, int row, int num_actions, DevTables t, uint64_t slots, uint64_t seed);
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16);
 
 }  // namespace mccfr

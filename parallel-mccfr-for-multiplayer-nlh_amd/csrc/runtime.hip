@@ -471,6 +471,83 @@ int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
   return MCCFR_OK;
 }
 
+int mccfr_set_max_batch(mccfr_ctx* c, uint32_t max_batch) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  if (max_batch == 0 || max_batch > (1u << 26)) return fail(c, MCCFR_E_ARG, "max_batch must be 1..2^26");
+  HIPX(c, hipStreamSynchronize(c->stream));
+  const uint64_t cap = std::min<uint64_t>(c->slots, (uint64_t)max_batch * kMaxDepth);
+  if (cap != c->t.touched_cap) {
+    uint32_t* nt = nullptr;
+    HIPX(c, hipMalloc(&nt, cap * sizeof(uint32_t)));
+    (void)hipFree(c->t.touched);
+    c->t.touched = nt;
+    c->t.touched_cap = (uint32_t)cap;
+  }
+  if (c->dtrace) {
+    (void)hipFree(c->dtrace);
+    c->dtrace = nullptr;
+  }
+  if (c->gexec) {  // the graph holds the old scratch pointers
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  c->max_batch = max_batch;
+  return MCCFR_OK;
+}
+
+static constexpr uint64_t kCheckpointMagic = 0x43465247505500ull;  // cfr_gpu.cu:1480
+
+int mccfr_save_checkpoint(mccfr_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
+  if (c->slots > 0xFFFFFFFFull) return fail(c, MCCFR_E_ARG, "table too large for the .ckpt header");
+  const size_t n = (size_t)c->slots * c->row;
+  std::vector<float> R(n), S(n);
+  int rc = mccfr_download_tables(c, R.data(), S.data(), nullptr, n, 1);
+  if (rc) return rc;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return fail(c, MCCFR_E_IO, std::string("cannot open ") + path);
+  const uint64_t magic = kCheckpointMagic;
+  const uint32_t ts = (uint32_t)c->slots, na = (uint32_t)c->row;
+  f.write(reinterpret_cast<const char*>(&magic), 8);
+  f.write(reinterpret_cast<const char*>(&ts), 4);
+  f.write(reinterpret_cast<const char*>(&na), 4);
+  f.write(reinterpret_cast<const char*>(R.data()), (std::streamsize)(n * sizeof(float)));
+  f.write(reinterpret_cast<const char*>(S.data()), (std::streamsize)(n * sizeof(float)));
+  if (!f) return fail(c, MCCFR_E_IO, std::string("write failed: ") + path);
+  return MCCFR_OK;
+}
+
+int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(c, MCCFR_E_IO, std::string("cannot open ") + path);
+  uint64_t magic = 0;
+  uint32_t ts = 0, na = 0;
+  f.read(reinterpret_cast<char*>(&magic), 8);
+  f.read(reinterpret_cast<char*>(&ts), 4);
+  f.read(reinterpret_cast<char*>(&na), 4);
+  if (!f) return fail(c, MCCFR_E_IO, std::string("cannot read header of ") + path);
+  if (magic != kCheckpointMagic) return fail(c, MCCFR_E_FORMAT, std::string(path) + " is not a checkpoint");
+  if ((uint64_t)ts != c->slots || na != (uint32_t)c->row)
+    return fail(c, MCCFR_E_FORMAT, std::string(path) + " was saved with table_size=" + std::to_string(ts) +
+                                       " num_actions=" + std::to_string(na) + ", this context has " +
+                                       std::to_string(c->slots) + " and " + std::to_string(c->row));
+  const size_t n = (size_t)ts * na;
+  std::vector<float> R(n), S(n);
+  f.read(reinterpret_cast<char*>(R.data()), (std::streamsize)(n * sizeof(float)));
+  f.read(reinterpret_cast<char*>(S.data()), (std::streamsize)(n * sizeof(float)));
+  if (!f) return fail(c, MCCFR_E_IO, std::string("truncated checkpoint ") + path);
+  return mccfr_upload_tables(c, R.data(), S.data(), n, 1);
+}
+
+int mccfr_randomize_tables(mccfr_ctx* c, uint64_t seed) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  launch_randomize(c->stream, c->row, c->cfg.num_actions, c->t, c->slots, seed);
+  HIPX(c, hipGetLastError());
+  c->sigma_dirty = true;
+  return rm_sweep_if_dirty(c);
+}
+
 int mccfr_sync(mccfr_ctx* c) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   HIPX(c, hipStreamSynchronize(c->stream));

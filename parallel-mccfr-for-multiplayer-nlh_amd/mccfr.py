@@ -88,6 +88,10 @@ _SIGS = {
     "mccfr_run_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_uint32, C.c_void_p]),
     "mccfr_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
     "mccfr_sync": (C.c_int, [C.c_void_p]),
+    "mccfr_set_max_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mccfr_randomize_tables": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "mccfr_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "mccfr_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "mccfr_download_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_upload_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
@@ -275,6 +279,19 @@ class Context:
 
     def train(self, iters: int, batch: int):
         self.check(lib().mccfr_train(self.h, iters, batch))
+
+    def set_max_batch(self, n: int):
+        self.check(lib().mccfr_set_max_batch(self.h, n))
+        self.max_batch = n
+
+    def randomize_tables(self, seed: int):
+        self.check(lib().mccfr_randomize_tables(self.h, seed))
+
+    def save_checkpoint(self, path: str):
+        self.check(lib().mccfr_save_checkpoint(self.h, path.encode()))
+
+    def load_checkpoint(self, path: str):
+        self.check(lib().mccfr_load_checkpoint(self.h, path.encode()))
 
     def sync(self):
         self.check(lib().mccfr_sync(self.h))

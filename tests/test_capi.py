@@ -52,3 +52,14 @@ def test_bad_arguments_are_errors_not_crashes():
     assert b"num_players" in L.mccfr_last_error(None)
     cfg = mccfr.reference_cfg()
     assert L.mccfr_create(C.byref(cfg), 0, C.byref(h)) == -1
+
+
+def test_cpp_wrapper_compiles_and_links(tmp_path):
+    """include/mccfr/gpu_cfr_trainer.hpp builds against the reference CLI's call pattern."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "wrapper_usage.cpp")
+    out = tmp_path / "wrapper_usage"
+    pkg = os.path.dirname(mccfr.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src, "-L", pkg, "-lmccfr",
+                    "-Wl,-rpath," + pkg, "-o", str(out)], check=True)
+    assert out.exists()

@@ -17,11 +17,13 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-def run_stepwise(cfg, hand_table, lut, n, launches, first_tid=0):
+def run_stepwise(cfg, hand_table, lut, n, launches, first_tid=0, random_seed=None):
     row = 8 if cfg.num_actions <= 8 else 16
     N = cfg.num_players
     with mccfr.Context(cfg, n) as ctx:
         ctx.set_hand_table(hand_table)
+        if random_seed is not None:
+            ctx.randomize_tables(random_seed)
         tid = first_tid
         for L in range(launches):
             up, t = L % N, L // N + 1
@@ -60,10 +62,12 @@ CONFIGS = {
 }
 
 
+@pytest.mark.parametrize("start", ["zero", "random"])
 @pytest.mark.parametrize("name", sorted(CONFIGS))
-def test_stepwise_parity(name, hand_table, golden_lut):
+def test_stepwise_parity(name, start, hand_table, golden_lut):
     cfg = CONFIGS[name]()
-    R, S, _ = run_stepwise(cfg, hand_table, golden_lut, n=4096, launches=3 * cfg.num_players)
+    R, S, _ = run_stepwise(cfg, hand_table, golden_lut, n=4096, launches=3 * cfg.num_players,
+                           random_seed=None if start == "zero" else 99)
     assert np.count_nonzero(S) > 0
     if cfg.use_cfr_plus:
         assert (R >= 0).all()
@@ -84,15 +88,32 @@ def test_train_graph_equals_launch_sequence(hand_table, golden_lut):
     assert np.allclose(S, S_seq, rtol=1e-6, atol=1e-6)
 
 
+def test_randomized_tables_follow_the_stream(hand_table):
+    """mccfr_randomize_tables: regret[slot][a] = u01(draw a of stream (seed, slot, 'rand'))."""
+    cfg = mccfr.cfg_6p_100bb(table_log2=12, seed=0)
+    with mccfr.Context(cfg, 64) as ctx:
+        ctx.randomize_tables(1234)
+        R, S, sig = ctx.download()
+    R = R.reshape(-1, 8)
+    O = oracle.lib()
+    for slot in (0, 1, 77, 4095):
+        want = [O.orc_u01(O.orc_draw_u32(1234, slot, 0x72616E64, a)) for a in range(6)]
+        assert np.array_equal(R[slot, :6], np.array(want, np.float32))
+        assert (R[slot, 6:] == 0).all()
+    assert (S == 0).all()
+    assert np.array_equal(sig, oracle.regret_matching(R.reshape(-1), 8))
+
+
 def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
-    """6 players, 100bb, 2^26-slot tables, 2^20 traversals per launch: after
-    warm-up launches, every lane of a launch is replayed by the oracle
-    (trace-only) from the downloaded pre-launch tables: bit-exact; table-wide
-    strategy-sum mass equals t * (update nodes)."""
+    """6 players, 100bb, 2^26-slot tables, 2^20 traversals per launch, from a
+    randomized regret state: after warm-up launches, every lane of a launch is
+    replayed by the oracle (trace-only) from the downloaded pre-launch tables:
+    bit-exact; table-wide strategy-sum mass equals t * (update nodes)."""
     cfg = mccfr.cfg_6p_100bb(table_log2=26, seed=0)
     n = 1 << 20
     with mccfr.Context(cfg, n) as ctx:
         ctx.set_hand_table(hand_table)
+        ctx.randomize_tables(5)
         tid = 0
         for L in range(6):  # one iteration, all six seats
             ctx.run_launch(L, 1, tid, n)
