@@ -58,7 +58,7 @@ typedef struct mccfr_action {
  * (cfr_gpu.cuh:50-55) plus the knobs the reference hard-codes. */
 typedef struct mccfr_game_cfg {
   int32_t num_players;      /* 2..6 */
-  int32_t starting_stack;   /* chips per seat (reference default 1000) */
+  int32_t starting_stack;   /* chips per seat (reference default 1000), at most 32767 */
   int32_t small_blind;      /* reference default 10 */
   int32_t big_blind;        /* reference default 20 */
   int32_t use_cfr_plus;     /* clamp regrets at 0 after every launch */
@@ -151,6 +151,12 @@ int mccfr_set_max_batch(mccfr_ctx* ctx, uint32_t max_batch);
  * (MCCFR_E_FORMAT) and refreshes the strategy snapshot before the next launch. */
 int mccfr_save_checkpoint(mccfr_ctx* ctx, const char* path);
 int mccfr_load_checkpoint(mccfr_ctx* ctx, const char* path);
+
+/* Launch shape of the persistent walk kernel: number of 256-lane workgroups
+ * (each lane walks traversal after traversal); 0 = one lane per traversal.
+ * Default 0.  Results do not depend on it. */
+int mccfr_set_walk_blocks(mccfr_ctx* ctx, uint32_t blocks);
+int mccfr_get_walk_blocks(const mccfr_ctx* ctx, uint32_t* blocks);
 
 /* Synthetic table state for benchmarks and tests: regret[slot][a] = a uniform
  * (0, 1] draw of stream (seed, id = slot, tag 0x72616E64) for the menu's

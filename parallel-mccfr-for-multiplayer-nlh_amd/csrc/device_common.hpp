@@ -48,9 +48,8 @@ struct LaunchState {
   int64_t t_weight;        // strategy-sum weight: iteration under linear CFR, else 1
   uint64_t first_tid;      // traversal id of lane 0
   uint32_t epoch;          // touched-slot stamp of this launch (never 0)
-  uint32_t touched;        // slots appended to the touched list
-  uint32_t overflow;       // touched-list overflow flag (must stay 0)
-  uint32_t pad;
+  uint32_t runs;           // entries in the launch's update list
+  uint32_t pad[2];
 };
 
 // ---------------------------------------------------------------------------

@@ -7,15 +7,27 @@
 
 namespace mccfr {
 
-// Device tables of one game (slot-major rows of `row` entries).
+// Update-player nodes a lane keeps in registers; deeper ones go to the
+// lane's overflow entries in HBM.
+constexpr int kRegPending = 4;
+
+// Device tables of one game (slot-major rows of `row` entries) and the
+// per-launch node lists (lane-strided: entry k of lane l at [k * lanes + l]).
 struct DevTables {
-  float* R;          // regrets
-  float* S;          // strategy sums
-  uint16_t* sig;     // BF16 regret-matched strategy snapshot
-  double* D;         // per-launch FP64 deltas: [slot][0..row) regret, [row..2row) strategy sum
-  uint32_t* stamp;   // last launch epoch that touched the slot
-  uint32_t* touched; // touched-slot list of the current launch
-  uint32_t touched_cap;
+  float* R;             // regrets
+  float* S;             // strategy sums
+  uint16_t* sig;        // BF16 regret-matched strategy snapshot
+  double* D;            // per-launch FP64 deltas: [slot][0..row) regret, [row..2row) strategy sum
+  uint32_t* stamp;      // last launch epoch whose update pass handled the slot
+  uint32_t* node_slot;  // [kMaxDepth][lanes] info-set slot of each update-player node
+  uint32_t* node_av;    // [kMaxDepth][lanes] sampled action | legal mask << 8
+  uint8_t* node_count;  // [lanes] update-player nodes of the traversal
+  float* node_u;        // [lanes] terminal utility of the update player
+  uint32_t* root_bkt;   // [kMaxPlayers][lanes] hole | flop << 8 | turn << 16 | river << 24 buckets
+  uint16_t* root_rank;  // [kMaxPlayers][lanes] 7-card rank (showdown)
+  uint8_t* root_dealer; // [lanes]
+  uint32_t* upd_list;   // [kMaxDepth * lanes] slots to update (one entry per slot per scatter tile)
+  uint32_t lanes;       // lane capacity of the lists (max batch)
 };
 
 // Tables of the arithmetic 7-card evaluator (built on the host).
@@ -30,9 +42,11 @@ void launch_set_state(hipStream_t s, LaunchState* st, int update_player, int64_t
                       int64_t t_weight, uint64_t first_tid, uint32_t n);
 void launch_advance_state(hipStream_t s, LaunchState* st, int num_players, int use_linear_cfr,
                           uint32_t batch);
+// Deal + walk + scatter of one launch (traverse.hip).  walk_blocks = number of
+// persistent 256-lane walk workgroups (0: one lane per traversal).
 void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, const int32_t* hr,
-                     const uint8_t* lut, DevTables t, uint32_t n_lanes, mccfr_trace* trace);
-void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t);
+                     const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks, mccfr_trace* trace);
+void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t, uint32_t n_lanes);
 void launch_rm_sweep(hipStream_t s, int row, DevTables t, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);
 void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, size_t stride,

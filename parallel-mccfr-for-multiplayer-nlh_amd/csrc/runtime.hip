@@ -43,6 +43,7 @@ struct mccfr_ctx {
 
   hipGraphExec_t gexec = nullptr;
   uint32_t graph_batch = 0;
+  uint32_t walk_blocks = 0;  // persistent walk workgroups per launch (0: one lane per traversal)
 
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 3 events per timed launch: start, after traverse, after update
@@ -100,11 +101,41 @@ bool table_ok(const int32_t* hr, size_t words) {  // hand_eval.cpp:59-86
   return decode_packed((uint32_t)hr[p + card_rm(royal[6])] & 0xFFFFu) == 7462u;
 }
 
+// Per-launch node lists for `lanes` traversals (kMaxDepth entries per lane).
+hipError_t alloc_node_lists(DevTables& t, uint32_t lanes) {
+  hipError_t e = hipMalloc(&t.node_slot, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&t.node_av, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&t.node_count, (size_t)lanes);
+  if (e == hipSuccess) e = hipMalloc(&t.node_u, (size_t)lanes * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&t.root_bkt, (size_t)kMaxPlayers * lanes * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&t.root_rank, (size_t)kMaxPlayers * lanes * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc(&t.root_dealer, (size_t)lanes);
+  if (e == hipSuccess) e = hipMalloc(&t.upd_list, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
+  t.lanes = lanes;
+  return e;
+}
+
+void free_node_lists(DevTables& t) {
+  void* ptrs[] = {t.node_slot, t.node_av, t.node_count, t.node_u, t.root_bkt, t.root_rank, t.root_dealer,
+                  t.upd_list};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  t.node_slot = nullptr;
+  t.node_av = nullptr;
+  t.node_count = nullptr;
+  t.node_u = nullptr;
+  t.root_bkt = nullptr;
+  t.root_rank = nullptr;
+  t.root_dealer = nullptr;
+  t.upd_list = nullptr;
+}
+
 void free_all(mccfr_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  void* ptrs[] = {c->dg, c->t.R, c->t.S, c->t.sig, c->t.D, c->t.stamp, c->t.touched, c->st,
-                  c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets, c->dtrace};
+  free_node_lists(c->t);
+  void* ptrs[] = {c->dg, c->t.R, c->t.S, c->t.sig, c->t.D, c->t.stamp, c->st, c->hr, c->lut,
+                  c->arith.nonflush, c->arith.flush, c->arith.offsets, c->dtrace};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -144,7 +175,7 @@ bool make_dev_game(const mccfr_game_cfg& cfg, DevGame& g, const char** why) {
   if (cfg.table_log2 < 4 || cfg.table_log2 > 32) return bad("table_log2 must be 4..32");
   if (cfg.history_bits < 1 || cfg.history_bits > 8) return bad("history_bits must be 1..8");
   if (cfg.starting_stack < 0 || cfg.small_blind < 0 || cfg.big_blind < 0) return bad("negative chips");
-  if ((int64_t)cfg.starting_stack * cfg.num_players > (1 << 30)) return bad("stacks too large");
+  if (cfg.starting_stack > 32767) return bad("starting_stack must be <= 32767 chips");
   std::memset(&g, 0, sizeof g);
   g.N = cfg.num_players;
   g.sb = cfg.small_blind;
@@ -256,6 +287,12 @@ int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out)
     }                                                                                           \
   } while (0)
   CHK(hipGetDevice(&c->device));
+  {
+    int cus = 0;
+    CHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    (void)cus;
+    c->walk_blocks = 0;  // one lane per traversal: measured fastest on MI355X (persistent grids 5-80% slower)
+  }
   CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t elems = (size_t)c->slots * c->row;
   CHK(hipMalloc(&c->t.R, elems * sizeof(float)));
@@ -263,10 +300,7 @@ int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out)
   CHK(hipMalloc(&c->t.sig, elems * sizeof(uint16_t)));
   CHK(hipMalloc(&c->t.D, elems * 2 * sizeof(double)));
   CHK(hipMalloc(&c->t.stamp, (size_t)c->slots * sizeof(uint32_t)));
-  // at most kMaxDepth update nodes per traversal
-  const uint64_t cap = std::min<uint64_t>(c->slots, (uint64_t)max_batch * kMaxDepth);
-  c->t.touched_cap = (uint32_t)cap;
-  CHK(hipMalloc(&c->t.touched, cap * sizeof(uint32_t)));
+  CHK(alloc_node_lists(c->t, max_batch));
   CHK(hipMalloc(&c->st, sizeof(LaunchState)));
   CHK(hipMalloc(&c->dg, sizeof(DevGame)));
   CHK(hipMalloc(&c->lut, 2704));
@@ -404,15 +438,16 @@ int mccfr_run_launch(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (!c->hr) return fail(c, MCCFR_E_STATE, "no hand table loaded");
   if (update_player < 0 || update_player >= c->cfg.num_players) return fail(c, MCCFR_E_ARG, "bad update player");
-  if (n == 0 || n > c->max_batch) return fail(c, MCCFR_E_ARG, "n must be 1..max_batch");
+  if (n == 0 || n > c->max_batch) return fail(c, MCCFR_E_ARG, "n must be 1..max_batch (see mccfr_set_max_batch)");
   int rc = rm_sweep_if_dirty(c);
   if (rc) return rc;
   if (trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->max_batch * sizeof(mccfr_trace)));
   launch_set_state(c->stream, c->st, update_player, c->next_iter, t_weight, first_tid, n);
   if ((rc = record_event(c, 0))) return rc;
-  launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, n, trace ? c->dtrace : nullptr);
+  launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, n, c->walk_blocks,
+                  trace ? c->dtrace : nullptr);
   if ((rc = record_event(c, 1))) return rc;
-  launch_update(c->stream, c->row, c->dg, c->st, c->t);
+  launch_update(c->stream, c->row, c->dg, c->st, c->t, n);
   if ((rc = record_event(c, 2))) return rc;
   HIPX(c, hipGetLastError());
   if (c->timing) {
@@ -423,10 +458,7 @@ int mccfr_run_launch(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t
   ++c->launches;
   if (trace) {
     HIPX(c, hipMemcpyAsync(trace, c->dtrace, (size_t)n * sizeof(mccfr_trace), hipMemcpyDeviceToHost, c->stream));
-    LaunchState hs;
-    HIPX(c, hipMemcpyAsync(&hs, c->st, sizeof hs, hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
-    if (hs.overflow) return fail(c, MCCFR_E_STATE, "touched-slot list overflow");
   }
   return MCCFR_OK;
 }
@@ -446,8 +478,8 @@ int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
     // one player-update: [traverse -> touched-slot update -> advance launch state]
     hipGraph_t g = nullptr;
     HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, batch, nullptr);
-    launch_update(c->stream, c->row, c->dg, c->st, c->t);
+    launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, batch, c->walk_blocks, nullptr);
+    launch_update(c->stream, c->row, c->dg, c->st, c->t, batch);
     launch_advance_state(c->stream, c->st, c->cfg.num_players, c->cfg.use_linear_cfr, batch);
     HIPX(c, hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
@@ -464,10 +496,7 @@ int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
   c->traversals += (uint64_t)iters * N * batch;
   c->launches += (uint64_t)iters * N;
   c->next_iter += iters;
-  LaunchState hs;
-  HIPX(c, hipMemcpyAsync(&hs, c->st, sizeof hs, hipMemcpyDeviceToHost, c->stream));
   HIPX(c, hipStreamSynchronize(c->stream));
-  if (hs.overflow) return fail(c, MCCFR_E_STATE, "touched-slot list overflow");
   return MCCFR_OK;
 }
 
@@ -475,13 +504,15 @@ int mccfr_set_max_batch(mccfr_ctx* c, uint32_t max_batch) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (max_batch == 0 || max_batch > (1u << 26)) return fail(c, MCCFR_E_ARG, "max_batch must be 1..2^26");
   HIPX(c, hipStreamSynchronize(c->stream));
-  const uint64_t cap = std::min<uint64_t>(c->slots, (uint64_t)max_batch * kMaxDepth);
-  if (cap != c->t.touched_cap) {
-    uint32_t* nt = nullptr;
-    HIPX(c, hipMalloc(&nt, cap * sizeof(uint32_t)));
-    (void)hipFree(c->t.touched);
-    c->t.touched = nt;
-    c->t.touched_cap = (uint32_t)cap;
+  if (max_batch != c->t.lanes) {
+    free_node_lists(c->t);
+    const hipError_t e = alloc_node_lists(c->t, max_batch);
+    if (e != hipSuccess) {
+      free_node_lists(c->t);
+      c->t.lanes = 0;
+      c->max_batch = 0;
+      return fail(c, MCCFR_E_NOMEM, std::string("node lists: ") + hipGetErrorString(e));
+    }
   }
   if (c->dtrace) {
     (void)hipFree(c->dtrace);
@@ -538,6 +569,22 @@ int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
   f.read(reinterpret_cast<char*>(S.data()), (std::streamsize)(n * sizeof(float)));
   if (!f) return fail(c, MCCFR_E_IO, std::string("truncated checkpoint ") + path);
   return mccfr_upload_tables(c, R.data(), S.data(), n, 1);
+}
+
+int mccfr_set_walk_blocks(mccfr_ctx* c, uint32_t blocks) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  c->walk_blocks = blocks;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return MCCFR_OK;
+}
+
+int mccfr_get_walk_blocks(const mccfr_ctx* c, uint32_t* blocks) {
+  if (!c || !blocks) return fail(nullptr, MCCFR_E_ARG, "null argument");
+  *blocks = c->walk_blocks;
+  return MCCFR_OK;
 }
 
 int mccfr_randomize_tables(mccfr_ctx* c, uint64_t seed) {

@@ -90,6 +90,8 @@ _SIGS = {
     "mccfr_sync": (C.c_int, [C.c_void_p]),
     "mccfr_set_max_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mccfr_randomize_tables": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "mccfr_set_walk_blocks": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mccfr_get_walk_blocks": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "mccfr_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
@@ -283,6 +285,14 @@ class Context:
     def set_max_batch(self, n: int):
         self.check(lib().mccfr_set_max_batch(self.h, n))
         self.max_batch = n
+
+    def set_walk_blocks(self, blocks: int):
+        self.check(lib().mccfr_set_walk_blocks(self.h, blocks))
+
+    def walk_blocks(self) -> int:
+        b = C.c_uint32()
+        self.check(lib().mccfr_get_walk_blocks(self.h, C.byref(b)))
+        return int(b.value)
 
     def randomize_tables(self, seed: int):
         self.check(lib().mccfr_randomize_tables(self.h, seed))
