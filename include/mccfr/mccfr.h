@@ -157,6 +157,9 @@ int mccfr_load_checkpoint(mccfr_ctx* ctx, const char* path);
  * Default 0.  Results do not depend on it. */
 int mccfr_set_walk_blocks(mccfr_ctx* ctx, uint32_t blocks);
 int mccfr_get_walk_blocks(const mccfr_ctx* ctx, uint32_t* blocks);
+/* Scatter workgroup size: 256, 512 or 1024 threads (4 traversals each).
+ * Results do not depend on it beyond FP64 summation order. */
+int mccfr_set_scatter_wg(mccfr_ctx* ctx, uint32_t wg);
 
 /* Synthetic table state for benchmarks and tests: regret[slot][a] = a uniform
  * (0, 1] draw of stream (seed, id = slot, tag 0x72616E64) for the menu's

@@ -54,3 +54,71 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
 }
 
 }  // namespace mccfr
+
+namespace mccfr {
+
+// C(n, K) for 0 <= n < 64 and a compile-time K <= 7 (0 when n < K: the
+// product then contains the factor n - n).
+template <int K>
+__host__ __device__ __forceinline__ uint32_t binom(uint32_t n) {
+  uint64_t p = 1;
+#pragma unroll
+  for (int j = 0; j < K; ++j) p *= (uint64_t)(n - (uint32_t)j);
+  constexpr uint64_t f = K == 0 ? 1 : K == 1 ? 1 : K == 2 ? 2 : K == 3 ? 6 : K == 4 ? 24 : K == 5 ? 120 : K == 6 ? 720 : 5040;
+  return (uint32_t)(p / f);
+}
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+// Colex rank of a 5- or 6-card set given unsorted: sum_i C(c_(i), i+1) over the
+// ascending order c_(0) < ... < c_(K-1).
+__device__ __forceinline__ uint32_t colex5(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e) {
+  cswap(a, b); cswap(d, e); cswap(c, e); cswap(c, d); cswap(b, e);
+  cswap(a, d); cswap(a, c); cswap(b, d); cswap(b, c);
+  return binom<1>(a) + binom<2>(b) + binom<3>(c) + binom<4>(d) + binom<5>(e);
+}
+__device__ __forceinline__ uint32_t colex6(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e, uint32_t f) {
+  cswap(a, b); cswap(c, d); cswap(e, f); cswap(a, c); cswap(b, e); cswap(d, f);
+  cswap(a, b); cswap(c, d); cswap(e, f); cswap(b, c); cswap(d, e); cswap(c, d);
+  return binom<1>(a) + binom<2>(b) + binom<3>(c) + binom<4>(d) + binom<5>(e) + binom<6>(f);
+}
+
+// 7-card rank 1..7462 of distinct suit-major cards by the arithmetic
+// evaluator: a suit holding >= 5 cards makes a flush or straight flush, which
+// no other hand of seven cards beats, ranked by its 13-bit rank mask; otherwise
+// the rank multiset (counts c_0..c_12, sum 7, each <= 4) is ranked
+// lexicographically, index = sum_i off[i][k_i][c_i] with k_i the cards left
+// before rank i (prefix counts from one multiply of packed 3-bit fields).
+__device__ __forceinline__ uint32_t arith_rank7(const uint32_t c[7], const uint16_t* __restrict__ nonflush,
+                                                const uint16_t* __restrict__ flush, const uint16_t* off) {
+  uint64_t cnt = 0, cmask = 0;
+  uint32_t scnt = 0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const uint32_t s = (c[k] * 79u) >> 10;  // c / 13 for c < 52
+    const uint32_t r = c[k] - 13u * s;
+    cnt += 1ull << (3 * r);
+    scnt += 1u << (4 * s);
+    cmask |= 1ull << c[k];
+  }
+  const uint32_t fl = (scnt + 0x3333u) & 0x8888u;  // a nibble >= 5
+  if (fl) {
+    const uint32_t s = (uint32_t)__builtin_ctz(fl) >> 2;
+    return flush[(uint32_t)(cmask >> (13 * s)) & 0x1FFFu];
+  }
+  const uint64_t pre = cnt * 0x0000001249249249ull;  // field i = c_0 + .. + c_i
+  uint32_t idx = 0;
+#pragma unroll
+  for (int r = 0; r < 13; ++r) {
+    const uint32_t before = r == 0 ? 0u : (uint32_t)(pre >> (3 * (r - 1))) & 7u;
+    const uint32_t cr = (uint32_t)(cnt >> (3 * r)) & 7u;
+    idx += off[(r * 8 + (7 - before)) * 5 + cr];
+  }
+  return nonflush[idx];
+}
+
+}  // namespace mccfr

@@ -134,13 +134,7 @@ __global__ __launch_bounds__(256) void k_eval7_walk(const int32_t* __restrict__ 
   }
 }
 
-// Arithmetic evaluator.  A suit holding >= 5 of the 7 cards makes the hand a
-// flush or straight flush (no full house or quads fits beside five suited
-// cards among seven), ranked by a 13-bit rank-mask table.  Otherwise the rank
-// multiset (counts c_0..c_12, sum 7, each <= 4) is ranked lexicographically
-// by a minimal perfect hash: index = sum_i off[i][k_i][c_i] with k_i the cards
-// left before rank i; the 13 lookups are independent because k_i comes from a
-// prefix sum of 3-bit count fields done by one multiply.
+// Arithmetic evaluator (arith_rank7) with its tables staged in LDS.
 constexpr int kArithWG = 512;
 __global__ __launch_bounds__(kArithWG) void k_eval7_arith(ArithTables t, const uint8_t* __restrict__ cards,
                                                           size_t stride, uint16_t* __restrict__ out, size_t n) {
@@ -158,32 +152,7 @@ __global__ __launch_bounds__(kArithWG) void k_eval7_arith(ArithTables t, const u
       out[i] = 0;
       continue;
     }
-    uint64_t cnt = 0, cmask = 0;
-    uint32_t scnt = 0;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      const uint32_t s = (c[k] * 79u) >> 10;  // c / 13 for c < 52
-      const uint32_t r = c[k] - 13u * s;
-      cnt += 1ull << (3 * r);
-      scnt += 1u << (4 * s);
-      cmask |= 1ull << c[k];
-    }
-    const uint32_t fl = (scnt + 0x3333u) & 0x8888u;  // nibble >= 5
-    uint32_t rank;
-    if (fl) {
-      const uint32_t s = (uint32_t)__builtin_ctz(fl) >> 2;
-      rank = s_fl[(uint32_t)(cmask >> (13 * s)) & 0x1FFFu];
-    } else {
-      const uint64_t pre = cnt * 0x0000001249249249ull;  // field i = c_0 + .. + c_i
-      uint32_t idx = 0;
-#pragma unroll
-      for (int r = 0; r < 13; ++r) {
-        const uint32_t before = r == 0 ? 0u : (uint32_t)(pre >> (3 * (r - 1))) & 7u;
-        const uint32_t cr = (uint32_t)(cnt >> (3 * r)) & 7u;
-        idx += s_off[(r * 8 + (7 - before)) * 5 + cr];
-      }
-      rank = s_nf[idx];
-    }
+    const uint32_t rank = arith_rank7(c, s_nf, s_fl, s_off);
     out[i] = (uint16_t)rank;
   }
 }
@@ -227,6 +196,45 @@ __global__ __launch_bounds__(256) void k_randomize(DevTables T, uint64_t slots, 
       reinterpret_cast<float4*>(T.R + slot * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
       reinterpret_cast<float4*>(T.S + slot * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+  }
+}
+
+// Bucket of every K-card set (K = 5: flop, K = 6: turn) indexed by colex rank:
+// the bucket the reference derives from the rank-walk word after K cards
+// (cfr_gpu.cu:114-122, 140-150, 476-489).  The walk state depends only on the
+// card set, so walking the sorted cards gives the reference's word.
+template <int K>
+__global__ __launch_bounds__(256) void k_build_buckets(const int32_t* __restrict__ hr, uint8_t* __restrict__ out,
+                                                       uint32_t count) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+    uint32_t cards[K];
+    uint32_t r = idx, top = 52;
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {  // unrank: largest c with C(c, i+1) <= r
+      uint32_t cc = top - 1;
+      while (true) {
+        uint32_t b;
+        switch (i) {
+          case 0: b = binom<1>(cc); break;
+          case 1: b = binom<2>(cc); break;
+          case 2: b = binom<3>(cc); break;
+          case 3: b = binom<4>(cc); break;
+          case 4: b = binom<5>(cc); break;
+          default: b = binom<6>(cc); break;
+        }
+        if (b <= r) {
+          r -= b;
+          break;
+        }
+        --cc;
+      }
+      cards[i] = cc;
+      top = cc;
+    }
+    int p = hr[53 + card_rm(cards[0])];
+#pragma unroll
+    for (int i = 1; i < K; ++i) p = hr[p + card_rm(cards[i])];
+    out[idx] = (uint8_t)bucket_of_rank(decode_packed((uint32_t)p & 0xFFFFu));
   }
 }
 
@@ -280,6 +288,12 @@ void launch_randomize(hipStream_t s, int row, int num_actions, DevTables t, uint
   else
     hipLaunchKernelGGL(k_randomize<16>, dim3(grid), dim3(256), 0, s, t, slots, num_actions, (uint32_t)seed,
                        (uint32_t)(seed >> 32));
+}
+void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b) {
+  hipLaunchKernelGGL(k_build_buckets<5>, dim3(grid_for(kFlopSets, 256, 65536)), dim3(256), 0, s, hr, b.flop5,
+                     (uint32_t)kFlopSets);
+  hipLaunchKernelGGL(k_build_buckets<6>, dim3(grid_for(kTurnSets, 256, 65536)), dim3(256), 0, s, hr, b.turn6,
+                     (uint32_t)kTurnSets);
 }
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 256, 1u << 16)), dim3(256), 0, s, src, dst, n16);

@@ -38,14 +38,25 @@ struct ArithTables {
   uint16_t* offsets;   // [13][8][5]
 };
 
+// Flop and turn buckets of every 5- and 6-card set, indexed by colex rank.
+constexpr uint32_t kFlopSets = 2598960u;   // C(52, 5)
+constexpr uint32_t kTurnSets = 20358520u;  // C(52, 6)
+struct BucketTables {
+  uint8_t* flop5;  // [kFlopSets]
+  uint8_t* turn6;  // [kTurnSets]
+};
+
+void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b);
 void launch_set_state(hipStream_t s, LaunchState* st, int update_player, int64_t iteration,
                       int64_t t_weight, uint64_t first_tid, uint32_t n);
 void launch_advance_state(hipStream_t s, LaunchState* st, int num_players, int use_linear_cfr,
                           uint32_t batch);
 // Deal + walk + scatter of one launch (traverse.hip).  walk_blocks = number of
-// persistent 256-lane walk workgroups (0: one lane per traversal).
-void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, const int32_t* hr,
-                     const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks, mccfr_trace* trace);
+// persistent 256-lane walk workgroups (0: one lane per traversal); scatter_wg =
+// scatter workgroup size (256, 512 or 1024 threads, 4 traversals per thread).
+void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, BucketTables bt,
+                     ArithTables at, const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks,
+                     uint32_t scatter_wg, mccfr_trace* trace);
 void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t, uint32_t n_lanes);
 void launch_rm_sweep(hipStream_t s, int row, DevTables t, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);

@@ -35,6 +35,7 @@ struct mccfr_ctx {
   int32_t* hr = nullptr;
   uint8_t* lut = nullptr;
   ArithTables arith{};
+  BucketTables buckets{};
   mccfr_trace* dtrace = nullptr;
 
   uint64_t traversals = 0, launches = 0;
@@ -43,7 +44,8 @@ struct mccfr_ctx {
 
   hipGraphExec_t gexec = nullptr;
   uint32_t graph_batch = 0;
-  uint32_t walk_blocks = 0;  // persistent walk workgroups per launch (0: one lane per traversal)
+  uint32_t walk_blocks = 0;   // persistent walk workgroups per launch (0: one lane per traversal)
+  uint32_t scatter_wg = 1024; // scatter workgroup size (4 traversals per thread)
 
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 3 events per timed launch: start, after traverse, after update
@@ -135,7 +137,8 @@ void free_all(mccfr_ctx* c) {
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   free_node_lists(c->t);
   void* ptrs[] = {c->dg, c->t.R, c->t.S, c->t.sig, c->t.D, c->t.stamp, c->st, c->hr, c->lut,
-                  c->arith.nonflush, c->arith.flush, c->arith.offsets, c->dtrace};
+                  c->arith.nonflush, c->arith.flush, c->arith.offsets, c->dtrace,
+                  c->buckets.flop5, c->buckets.turn6};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -304,6 +307,8 @@ int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out)
   CHK(hipMalloc(&c->st, sizeof(LaunchState)));
   CHK(hipMalloc(&c->dg, sizeof(DevGame)));
   CHK(hipMalloc(&c->lut, 2704));
+  CHK(hipMalloc(&c->buckets.flop5, kFlopSets));
+  CHK(hipMalloc(&c->buckets.turn6, kTurnSets));
   CHK(hipMemsetAsync(c->t.R, 0, elems * sizeof(float), c->stream));
   CHK(hipMemsetAsync(c->t.S, 0, elems * sizeof(float), c->stream));
   CHK(hipMemsetAsync(c->t.sig, 0, elems * sizeof(uint16_t), c->stream));
@@ -349,6 +354,8 @@ int mccfr_set_hand_table(mccfr_ctx* c, const int32_t* hr, size_t words) {
                 "incompatible hand table: expected 32,487,834 words with As Ks Qs Js Ts 2c 3d = 7462");
   if (!c->hr) HIPX(c, hipMalloc(&c->hr, words * sizeof(int32_t)));
   HIPX(c, hipMemcpyAsync(c->hr, hr, words * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  launch_build_buckets(c->stream, c->hr, c->buckets);  // flop/turn buckets of every 5-/6-card set
+  HIPX(c, hipGetLastError());
   HIPX(c, hipStreamSynchronize(c->stream));
   return MCCFR_OK;
 }
@@ -444,8 +451,8 @@ int mccfr_run_launch(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t
   if (trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->max_batch * sizeof(mccfr_trace)));
   launch_set_state(c->stream, c->st, update_player, c->next_iter, t_weight, first_tid, n);
   if ((rc = record_event(c, 0))) return rc;
-  launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, n, c->walk_blocks,
-                  trace ? c->dtrace : nullptr);
+  launch_traverse(c->stream, c->row, c->dg, c->st, c->buckets, c->arith, c->lut, c->t, n, c->walk_blocks,
+                  c->scatter_wg, trace ? c->dtrace : nullptr);
   if ((rc = record_event(c, 1))) return rc;
   launch_update(c->stream, c->row, c->dg, c->st, c->t, n);
   if ((rc = record_event(c, 2))) return rc;
@@ -478,7 +485,8 @@ int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
     // one player-update: [traverse -> touched-slot update -> advance launch state]
     hipGraph_t g = nullptr;
     HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_traverse(c->stream, c->row, c->dg, c->st, c->hr, c->lut, c->t, batch, c->walk_blocks, nullptr);
+    launch_traverse(c->stream, c->row, c->dg, c->st, c->buckets, c->arith, c->lut, c->t, batch, c->walk_blocks,
+                    c->scatter_wg, nullptr);
     launch_update(c->stream, c->row, c->dg, c->st, c->t, batch);
     launch_advance_state(c->stream, c->st, c->cfg.num_players, c->cfg.use_linear_cfr, batch);
     HIPX(c, hipStreamEndCapture(c->stream, &g));
@@ -574,6 +582,17 @@ int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
 int mccfr_set_walk_blocks(mccfr_ctx* c, uint32_t blocks) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   c->walk_blocks = blocks;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return MCCFR_OK;
+}
+
+int mccfr_set_scatter_wg(mccfr_ctx* c, uint32_t wg) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  if (wg != 256 && wg != 512 && wg != 1024) return fail(c, MCCFR_E_ARG, "scatter workgroup must be 256, 512 or 1024");
+  c->scatter_wg = wg;
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;

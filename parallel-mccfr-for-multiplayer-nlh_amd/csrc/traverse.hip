@@ -2,8 +2,9 @@
 // cfr_gpu.cu:878-1011, and its clamp / regret-matching sweeps, :1018-1022 and
 // :318-351), as four gfx950 kernels:
 //
-//   k_deal      one lane per traversal: Philox deal, hole and board buckets,
-//               showdown ranks, dealer -> a compact root record in HBM.
+//   k_deal      one lane per traversal: Philox deal, hole and board buckets
+//               (precomputed bucket tables), showdown ranks, dealer -> a
+//               compact root record in HBM.
 //   k_walk      persistent lanes walk traversal after traversal from the root
 //               records: both node types follow exactly one child
 //               (cfr_gpu.cu:800, :844), so a traversal is one root-to-leaf
@@ -28,15 +29,22 @@ constexpr int kDealWG = 256;
 constexpr int kWalkWG = 256;
 
 // ---------------------------------------------------------------------------
-// k_deal
+// k_deal.  Buckets come from precomputed tables instead of the reference's
+// per-player table walks: the rank-walk word after 5 (6) cards depends only on
+// the card set, so the flop (turn) bucket of every 5- (6-) card set is stored
+// once, indexed by colex rank (kernels.hip k_build_buckets); the river rank and
+// bucket come from the arithmetic evaluator.  Per player: three independent
+// loads instead of a chain of dependent gathers into the 130 MB table.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kDealWG) void k_deal(const DevGame* __restrict__ gp, const LaunchState* __restrict__ st,
-                                                  const int32_t* __restrict__ hr, const uint8_t* __restrict__ lut_g,
+                                                  BucketTables bt, ArithTables at, const uint8_t* __restrict__ lut_g,
                                                   DevTables T, uint32_t n_lanes) {
   __shared__ uint32_t s_lut[2704 / 4];
+  __shared__ uint16_t s_off[13 * 8 * 5];
   __shared__ uint8_t s_deck[kDeckStride * kDealWG];
   const DevGame& G = *gp;
   for (int i = threadIdx.x; i < 2704 / 4; i += kDealWG) s_lut[i] = reinterpret_cast<const uint32_t*>(lut_g)[i];
+  for (int i = threadIdx.x; i < 13 * 8 * 5; i += kDealWG) s_off[i] = at.offsets[i];
   __syncthreads();
   uint32_t n = (uint32_t)st->n;
   if (n > n_lanes) n = n_lanes;
@@ -57,29 +65,18 @@ __global__ __launch_bounds__(kDealWG) void k_deal(const DevGame* __restrict__ gp
     dk[i] = b;
     dk[j] = a;
   }
-  // The rank-walk state depends only on the set of cards seen, so the board's
-  // 3-card prefix is walked once and each player adds hole cards to it: the
-  // 5-card and 6-card words equal the reference's eval5/eval6 reads
-  // (cfr_gpu.cu:114-150) and the 7-card word its eval7, in 3 + 4N gathers
-  // instead of 18N.
-  const uint32_t c0 = dk[2 * N], c1 = dk[2 * N + 1], c2 = dk[2 * N + 2];
-  const int rm3 = card_rm(dk[2 * N + 3]), rm4 = card_rm(dk[2 * N + 4]);
-  int pw = hr[53 + card_rm(c0)];
-  pw = hr[pw + card_rm(c1)];
-  const int p3 = hr[pw + card_rm(c2)];
+  const uint32_t f0 = dk[2 * N], f1 = dk[2 * N + 1], f2 = dk[2 * N + 2], c3 = dk[2 * N + 3], c4 = dk[2 * N + 4];
   const uint8_t* lut = reinterpret_cast<const uint8_t*>(s_lut);
 #pragma unroll
   for (int pl = 0; pl < kMaxPlayers; ++pl) {
     if (pl < N) {
       const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
       const uint32_t hb = lut[h0 * 52 + h1];
-      const int q4 = hr[p3 + card_rm(h0)];
-      const int p5 = hr[q4 + card_rm(h1)];
-      const int p6 = hr[p5 + rm3];
-      const uint32_t r7 = decode_packed((uint32_t)hr[p6 + rm4] & 0xFFFFu);
-      const uint32_t bf = bucket_of_rank(decode_packed((uint32_t)p5 & 0xFFFFu));
-      const uint32_t bt = bucket_of_rank(decode_packed((uint32_t)p6 & 0xFFFFu));
-      T.root_bkt[(size_t)pl * T.lanes + t] = hb | (bf << 8) | (bt << 16) | (bucket_of_rank(r7) << 24);
+      const uint32_t bf = bt.flop5[colex5(h0, h1, f0, f1, f2)];
+      const uint32_t btn = bt.turn6[colex6(h0, h1, f0, f1, f2, c3)];
+      const uint32_t c7[7] = {h0, h1, f0, f1, f2, c3, c4};
+      const uint32_t r7 = arith_rank7(c7, at.nonflush, at.flush, s_off);
+      T.root_bkt[(size_t)pl * T.lanes + t] = hb | (bf << 8) | (btn << 16) | (bucket_of_rank(r7) << 24);
       T.root_rank[(size_t)pl * T.lanes + t] = (uint16_t)r7;
     }
   }
@@ -367,14 +364,13 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
 // preflop info sets collect tens of thousands of records per launch).  The
 // run's slot is appended to the launch's update list.
 // ---------------------------------------------------------------------------
-constexpr int kScatterWG = 1024;
-constexpr int kScatterTile = 4096;   // traversals per workgroup
-constexpr int kScatterCap = 8192;    // records sorted per pass
-
+// Workgroup sizes the scatter is built for; 4 traversals per thread, sort
+// capacity 2 records per traversal per pass.
+template <int WG>
 __device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, int n_pow2) {
   for (int k = 2; k <= n_pow2; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n_pow2; i += kScatterWG) {
+      for (int i = threadIdx.x; i < n_pow2; i += WG) {
         const int ixj = i ^ j;
         if (ixj > i) {
           const uint64_t a = keys[i], b = keys[ixj];
@@ -389,10 +385,12 @@ __device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, int n_pow2) {
     }
 }
 
-template <int ROW>
+template <int ROW, int kScatterWG>
 __global__ __launch_bounds__(kScatterWG) void k_scatter(const DevGame* __restrict__ gp,
                                                         LaunchState* __restrict__ st, DevTables T,
                                                         uint32_t n_lanes) {
+  constexpr int kScatterTile = 4 * kScatterWG;  // traversals per workgroup
+  constexpr int kScatterCap = 2 * kScatterTile; // records sorted per pass
   __shared__ uint64_t s_keys[kScatterCap];
   __shared__ uint32_t s_run[kScatterCap];
   __shared__ uint32_t s_wsum[kScatterWG / 64];
@@ -452,7 +450,7 @@ __global__ __launch_bounds__(kScatterWG) void k_scatter(const DevGame* __restric
       }
     }
     __syncthreads();
-    block_bitonic_sort(s_keys, np2);
+    block_bitonic_sort<kScatterWG>(s_keys, np2);
     // each run of equal slots is summed by the thread holding its first record
     for (uint32_t i = tx; i < cn; i += kScatterWG) {
       const uint32_t slot = (uint32_t)(s_keys[i] >> 32);
@@ -595,19 +593,29 @@ __global__ __launch_bounds__(256) void k_update(const DevGame* __restrict__ gp, 
 
 }  // namespace
 
-void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, const int32_t* hr,
-                     const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks, mccfr_trace* trace) {
-  hipLaunchKernelGGL(k_deal, dim3(grid_for(n_lanes, kDealWG, 1u << 30)), dim3(kDealWG), 0, s, g, st, hr, lut, t,
+void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, BucketTables bt,
+                     ArithTables at, const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks,
+                     uint32_t scatter_wg, mccfr_trace* trace) {
+  hipLaunchKernelGGL(k_deal, dim3(grid_for(n_lanes, kDealWG, 1u << 30)), dim3(kDealWG), 0, s, g, st, bt, at, lut, t,
                      n_lanes);
   const dim3 wgrid(grid_for(n_lanes, kWalkWG, walk_blocks ? walk_blocks : 1u << 30));
-  const dim3 sgrid(grid_for(n_lanes, kScatterTile, 1u << 30));
-  if (row == 8) {
+  if (row == 8)
     hipLaunchKernelGGL(k_walk<8>, wgrid, dim3(kWalkWG), 0, s, g, st, t, n_lanes, trace);
-    hipLaunchKernelGGL(k_scatter<8>, sgrid, dim3(kScatterWG), 0, s, g, st, t, n_lanes);
-  } else {
+  else
     hipLaunchKernelGGL(k_walk<16>, wgrid, dim3(kWalkWG), 0, s, g, st, t, n_lanes, trace);
-    hipLaunchKernelGGL(k_scatter<16>, sgrid, dim3(kScatterWG), 0, s, g, st, t, n_lanes);
+  const unsigned wg = scatter_wg == 256 || scatter_wg == 512 ? scatter_wg : 1024;
+  const dim3 sgrid(grid_for(n_lanes, 4 * wg, 1u << 30));
+#define MCCFR_SCATTER(R, W) hipLaunchKernelGGL((k_scatter<R, W>), sgrid, dim3(W), 0, s, g, st, t, n_lanes)
+  if (row == 8) {
+    if (wg == 256) MCCFR_SCATTER(8, 256);
+    else if (wg == 512) MCCFR_SCATTER(8, 512);
+    else MCCFR_SCATTER(8, 1024);
+  } else {
+    if (wg == 256) MCCFR_SCATTER(16, 256);
+    else if (wg == 512) MCCFR_SCATTER(16, 512);
+    else MCCFR_SCATTER(16, 1024);
   }
+#undef MCCFR_SCATTER
 }
 
 void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t, uint32_t n_lanes) {

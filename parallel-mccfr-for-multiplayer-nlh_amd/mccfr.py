@@ -92,6 +92,7 @@ _SIGS = {
     "mccfr_randomize_tables": (C.c_int, [C.c_void_p, C.c_uint64]),
     "mccfr_set_walk_blocks": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mccfr_get_walk_blocks": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "mccfr_set_scatter_wg": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mccfr_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
@@ -288,6 +289,9 @@ class Context:
 
     def set_walk_blocks(self, blocks: int):
         self.check(lib().mccfr_set_walk_blocks(self.h, blocks))
+
+    def set_scatter_wg(self, wg: int):
+        self.check(lib().mccfr_set_scatter_wg(self.h, wg))
 
     def walk_blocks(self) -> int:
         b = C.c_uint32()
