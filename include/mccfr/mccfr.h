@@ -152,14 +152,12 @@ int mccfr_set_max_batch(mccfr_ctx* ctx, uint32_t max_batch);
 int mccfr_save_checkpoint(mccfr_ctx* ctx, const char* path);
 int mccfr_load_checkpoint(mccfr_ctx* ctx, const char* path);
 
-/* Launch shape of the persistent walk kernel: number of 256-lane workgroups
- * (each lane walks traversal after traversal); 0 = one lane per traversal.
- * Default 0.  Results do not depend on it. */
-int mccfr_set_walk_blocks(mccfr_ctx* ctx, uint32_t blocks);
-int mccfr_get_walk_blocks(const mccfr_ctx* ctx, uint32_t* blocks);
-/* Scatter workgroup size: 256, 512 or 1024 threads (4 traversals each).
- * Results do not depend on it beyond FP64 summation order. */
-int mccfr_set_scatter_wg(mccfr_ctx* ctx, uint32_t wg);
+/* Launch shape of the walk kernel: workgroup size (64, 128, 256, 512 or
+ * 1024 lanes) and traversals per lane (1, 2 or 4; a lane walks its traversals
+ * one after another, so a wave covers 64, 128 or 256 traversals).  Defaults
+ * 256 and 1.  Results do not depend on the shape. */
+int mccfr_set_launch_shape(mccfr_ctx* ctx, uint32_t walk_wg, uint32_t traversals_per_lane);
+int mccfr_get_launch_shape(const mccfr_ctx* ctx, uint32_t* walk_wg, uint32_t* traversals_per_lane);
 
 /* Synthetic table state for benchmarks and tests: regret[slot][a] = a uniform
  * (0, 1] draw of stream (seed, id = slot, tag 0x72616E64) for the menu's
@@ -208,6 +206,136 @@ int mccfr_random_hands_device(mccfr_ctx* ctx, uint64_t seed, uint32_t tag, uint8
 int mccfr_device_alloc(mccfr_ctx* ctx, size_t bytes, void** out);
 int mccfr_device_free(mccfr_ctx* ctx, void* p);
 int mccfr_memcpy(mccfr_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* ===========================================================================
+ * Tenants and batched launches.  No reference counterpart: the reference
+ * trains one game per process from the preflop root (cfr_gpu.cu:931-981,
+ * 1185-1198).  A context holds several tenants (game configuration, root,
+ * tables, update-player schedule) sharing its hand table and launch scratch;
+ * one batched launch runs traversals of several tenants together.  Tenant 0 is
+ * the context's own game, the one every call above works on.
+ * ======================================================================== */
+
+/* Root of a tenant's game tree.
+ * street 0: the reference's preflop root (fresh deal of 2N+5 cards, dealer
+ *   drawn after it, blinds posted; cfr_gpu.cu:931-981); other fields ignored.
+ * street 1/2/3: a public flop/turn/river state (subgame re-solve).  board[0..2],
+ *   [0..3] or [0..4] are fixed; every seat's hole cards and the missing board
+ *   cards are dealt from the remaining deck (Fisher-Yates, 2N + 5 - fixed
+ *   draws, no dealer draw).  The betting round starts with no bets, from the
+ *   first seat after `dealer` that is in the hand with chips behind.  Folded
+ *   seats (not in active_mask) keep their invested chips in the pot.  Payoffs
+ *   count all invested chips: pot = sum(invested) + chips bet after the root.
+ *   The tenant's update-player schedule is its active seats in seat order. */
+typedef struct mccfr_root_state {
+  int32_t street;       /* 0 preflop (reference root), 1 flop, 2 turn, 3 river */
+  int32_t dealer;       /* button seat, 0..N-1 */
+  uint32_t active_mask; /* seats still in the hand; at least two */
+  uint32_t history;     /* action-history bits the info-set keys continue from */
+  int32_t stack[MCCFR_MAX_PLAYERS];    /* chips behind at the root */
+  int32_t invested[MCCFR_MAX_PLAYERS]; /* chips put into the pot before the root */
+  uint8_t board[5];     /* suit-major card codes 0..51, distinct */
+  uint8_t reserved[3];
+} mccfr_root_state;
+
+/* Device bytes of one tenant's tables (FP32 regrets, FP32 strategy sums, BF16
+ * snapshot: 10 bytes per padded action per slot). */
+int mccfr_tenant_footprint(const mccfr_game_cfg* cfg, uint64_t* bytes);
+
+/* Creates a tenant with zeroed tables (root NULL = the preflop root).  The
+ * context's max_batch bounds the traversals of every launch, over all tenants. */
+int mccfr_tenant_create(mccfr_ctx* ctx, const mccfr_game_cfg* cfg, const mccfr_root_state* root,
+                        uint32_t* tenant);
+int mccfr_tenant_destroy(mccfr_ctx* ctx, uint32_t tenant); /* not tenant 0 */
+
+/* One tenant's share of a batched launch. */
+typedef struct mccfr_segment {
+  uint32_t tenant;
+  uint32_t n;            /* traversals, >= 1 */
+  int32_t update_player; /* seat whose regrets this share updates */
+  uint32_t reserved;
+  int64_t t_weight;      /* strategy-sum weight */
+  uint64_t first_tid;    /* traversal id of the share's first traversal */
+} mccfr_segment;
+
+/* One batched launch: each segment's traversals, then each tenant's table
+ * update, exactly as if every segment ran alone through mccfr_run_launch
+ * (a tenant's result does not depend on the other segments).  Segments name
+ * distinct tenants with the same padded row width (menus of <= 8 or of 9..16
+ * actions); each is padded to a multiple of 256 lanes and the padded total must
+ * not exceed max_batch.  trace: HOST array of sum(n) records, segment after
+ * segment, or NULL. */
+int mccfr_run_segments(mccfr_ctx* ctx, const mccfr_segment* segs, uint32_t n_segs, mccfr_trace* trace);
+
+/* Training loop over several tenants: `launches` batched launches replayed
+ * from one captured hipGraph.  In every launch tenant tenants[i] runs counts[i]
+ * traversals for the next seat of its own schedule; each tenant advances its
+ * own iteration count (t_weight = iteration under linear CFR) and traversal
+ * ids, continuing across calls. */
+int mccfr_train_tenants(mccfr_ctx* ctx, const uint32_t* tenants, const uint32_t* counts, uint32_t n,
+                        int64_t launches);
+
+/* A tenant's counters: traversals issued (= its next traversal id), launches,
+ * next iteration and next seat of its schedule. */
+int mccfr_tenant_counters(const mccfr_ctx* ctx, uint32_t tenant, uint64_t* traversals, uint64_t* launches,
+                          int64_t* next_iteration, int32_t* next_player);
+
+/* Per-tenant forms of the table calls above (tenant 0 = the plain calls). */
+int mccfr_tenant_table_shape(const mccfr_ctx* ctx, uint32_t tenant, uint64_t* slots, uint32_t* row_actions);
+int mccfr_tenant_download(mccfr_ctx* ctx, uint32_t tenant, float* regrets, float* strategy_sum,
+                          uint16_t* strategy_bf16, size_t elems, int layout);
+int mccfr_tenant_upload(mccfr_ctx* ctx, uint32_t tenant, const float* regrets, const float* strategy_sum,
+                        size_t elems, int layout);
+int mccfr_tenant_normalized_strategy(mccfr_ctx* ctx, uint32_t tenant, float* out, size_t elems, int layout);
+int mccfr_tenant_randomize(mccfr_ctx* ctx, uint32_t tenant, uint64_t seed);
+
+/* Autotuner: times the deal + walk of the given segment mix (tables are not
+ * modified) for every launch shape -- walk workgroup 64/128/256/512/1024 x
+ * 1/2/4 traversals per lane -- over `reps` launches each, keeps the fastest as
+ * the context's shape and reports it (ms = mean deal + walk time). */
+int mccfr_autotune(mccfr_ctx* ctx, const mccfr_segment* segs, uint32_t n_segs, int reps, uint32_t* walk_wg,
+                   uint32_t* traversals_per_lane, float* ms);
+
+/* ===========================================================================
+ * Re-solve server: continuous batching of subgame re-solves on one context.
+ * A request is a game configuration, a root and an iteration budget.  It waits
+ * until its tables fit in the server's HBM budget and a running place is
+ * free, then runs as a tenant: round r = 1, 2, ... gives every active seat one
+ * launch of min(batch, iterations - (r-1)*batch) traversals (one iteration =
+ * one traversal per active seat; t_weight = r under linear CFR).  Every server
+ * step issues one batched launch holding the next launch of every running
+ * request, so requests that arrive while others run join the next launch.
+ * No reference counterpart.
+ * ======================================================================== */
+typedef struct mccfr_server mccfr_server;
+
+/* hbm_budget: device bytes the server may hold for request tables (one pool,
+ * allocated here); max_running: requests in one batched launch (<= 4096). */
+int mccfr_server_create(mccfr_ctx* ctx, uint64_t hbm_budget, uint32_t max_running, mccfr_server** out);
+void mccfr_server_destroy(mccfr_server* srv);
+
+/* Queues a request (root must be a postflop or preflop root of cfg).  batch:
+ * traversals per launch, 1..max_batch. */
+int mccfr_server_submit(mccfr_server* srv, const mccfr_game_cfg* cfg, const mccfr_root_state* root,
+                        int64_t iterations, uint32_t batch, uint64_t* request);
+
+/* One scheduling step: retires launches the device has finished, admits
+ * waiting requests that fit (first come, first served), and issues the next
+ * batched launch if fewer than two are in flight.  Never waits for the device
+ * unless `wait` is non-zero, in which case it first waits for the oldest launch
+ * in flight.  Reports requests running and waiting. */
+int mccfr_server_step(mccfr_server* srv, int wait, uint32_t* running, uint32_t* waiting);
+
+/* state: 0 waiting, 1 running, 2 done (tables final), 3 released. */
+int mccfr_server_poll(mccfr_server* srv, uint64_t request, int* state, uint64_t* traversals);
+
+/* A done request's tenant (for mccfr_tenant_download and the like) and its
+ * average strategy (normalized strategy sums, host array). */
+int mccfr_server_tenant(mccfr_server* srv, uint64_t request, uint32_t* tenant);
+int mccfr_server_result(mccfr_server* srv, uint64_t request, float* strategy, size_t elems, int layout);
+
+/* Frees a done request's tables. */
+int mccfr_server_release(mccfr_server* srv, uint64_t request);
 
 #ifdef __cplusplus
 }

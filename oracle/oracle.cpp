@@ -625,21 +625,39 @@ struct Walker {
   }
 };
 
-// cfr_gpu.cu:921-981: deal, buckets, blinds and the preflop queue.
-void setup_game(const mccfr_game_cfg& cfg, const int32_t* hr, const uint8_t* lut, Stream& rng,
-                Game& g) {
+// Deal + buckets of one traversal.  Preflop root (cfr_gpu.cu:921-937):
+// 2N+5 draws over the full deck.  Postflop root (the project's re-solve root,
+// mccfr.h mccfr_root_state): the fixed board cards leave the deck (the rest in
+// ascending order), then 2N + 5 - fixed draws over the remaining cards.
+void deal_cards(const mccfr_game_cfg& cfg, const mccfr_root_state* root, Stream& rng, Game& g) {
   const int N = cfg.num_players;
-  g.N = N;
+  const int nb = (root && root->street > 0) ? root->street + 2 : 0;
   uint8_t deck[52];
-  for (int i = 0; i < 52; ++i) deck[i] = (uint8_t)i;
-  const int n_deal = 2 * N + 5;
+  int M = 0;
+  for (int c = 0; c < 52; ++c) {
+    bool fixed = false;
+    for (int k = 0; k < nb; ++k) fixed = fixed || root->board[k] == c;
+    if (!fixed) deck[M++] = (uint8_t)c;
+  }
+  const int n_deal = 2 * N + 5 - nb;
   for (int i = 0; i < n_deal; ++i) {
     const uint32_t r = rng.next();
-    const int j = i + (int)(((uint64_t)r * (uint64_t)(52 - i)) >> 32);
+    const int j = i + (int)(((uint64_t)r * (uint64_t)(M - i)) >> 32);
     std::swap(deck[i], deck[j]);
   }
   for (int i = 0; i < 2 * N; ++i) g.hole[i] = deck[i];
-  for (int i = 0; i < 5; ++i) g.comm[i] = deck[2 * N + i];
+  for (int k = 0; k < 5; ++k) g.comm[k] = k < nb ? root->board[k] : deck[2 * N + k - nb];
+}
+
+// cfr_gpu.cu:921-981: deal, buckets, blinds and the preflop queue; or the
+// postflop root: stacks and investments as given, seats outside active_mask
+// folded, no bets, queue from the seat after the dealer (as at a street
+// change, cfr_gpu.cu:676-686).
+void setup_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                Stream& rng, Game& g) {
+  const int N = cfg.num_players;
+  g.N = N;
+  deal_cards(cfg, root, rng, g);
   for (int p = 0; p < N; ++p) g.hb[p] = lut[g.hole[2 * p] * 52 + g.hole[2 * p + 1]];
   for (int p = 0; p < N; ++p) {
     g.bbk[p][0] = 0;
@@ -649,15 +667,40 @@ void setup_game(const mccfr_game_cfg& cfg, const int32_t* hr, const uint8_t* lut
     g.bbk[p][2] = bucket_of(walk6(hr, c));
     g.bbk[p][3] = bucket_of(walk7(hr, c));
   }
+  g.current_bet = 0;
+  g.lfr = cfg.big_blind;
+  g.bb_amt = cfg.big_blind;
+  g.raises = 0;
+  g.n_to_act = 0;
+  if (root && root->street > 0) {
+    g.folded = 0;
+    g.all_in = 0;
+    g.pot = 0;
+    for (int p = 0; p < N; ++p) {
+      g.stacks[p] = root->stack[p];
+      g.bets[p] = 0;
+      g.invested[p] = root->invested[p];
+      g.pot += root->invested[p];
+      const bool act = root->active_mask >> p & 1u;
+      if (!act) g.folded |= 1u << p;
+      if (act && root->stack[p] == 0) g.all_in |= 1u << p;
+    }
+    g.ab = root->history & 0x3FFFFFFFu;
+    g.street = root->street;
+    g.dealer = root->dealer;
+    for (int i = 1; i <= N; ++i) {
+      const int p = (g.dealer + i) % N;
+      if (!(g.folded >> p & 1u) && !(g.all_in >> p & 1u)) g.to_act[g.n_to_act++] = p;
+    }
+    return;
+  }
   for (int i = 0; i < N; ++i) {
     g.stacks[i] = cfg.starting_stack;
     g.bets[i] = 0;
     g.invested[i] = 0;
   }
-  g.folded = 0; g.all_in = 0; g.pot = 0; g.current_bet = 0;
-  g.lfr = cfg.big_blind;
-  g.bb_amt = cfg.big_blind;
-  g.ab = 0; g.street = 0; g.raises = 0;
+  g.folded = 0; g.all_in = 0; g.pot = 0;
+  g.ab = 0; g.street = 0;
   g.dealer = (int)(rng.next() % (uint32_t)N);
   const int sb_pos = (N == 2) ? g.dealer : (g.dealer + 1) % N;
   const int bb_pos = (N == 2) ? (g.dealer + 1) % N : (g.dealer + 2) % N;
@@ -669,7 +712,6 @@ void setup_game(const mccfr_game_cfg& cfg, const int32_t* hr, const uint8_t* lut
   g.current_bet = bb_amt;
   if (g.stacks[sb_pos] == 0) g.all_in |= 1u << sb_pos;
   if (g.stacks[bb_pos] == 0) g.all_in |= 1u << bb_pos;
-  g.n_to_act = 0;
   for (int i = 0; i < N; ++i) {
     const int p = (bb_pos + 1 + i) % N;
     if (!(g.all_in >> p & 1u)) g.to_act[g.n_to_act++] = p;
@@ -677,11 +719,11 @@ void setup_game(const mccfr_game_cfg& cfg, const int32_t* hr, const uint8_t* lut
 }
 
 template <class Tables, class Sink>
-void run_traversal(const mccfr_game_cfg& cfg, const int32_t* hr, const uint8_t* lut, const Tables& T,
-                   Sink& sink, int up, float tw, uint64_t tid, uint32_t tag, mccfr_trace* tr) {
+void run_traversal(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                   const Tables& T, Sink& sink, int up, float tw, uint64_t tid, uint32_t tag, mccfr_trace* tr) {
   Stream rng(cfg.seed, tid, tag);
   Game g;
-  setup_game(cfg, hr, lut, rng, g);
+  setup_game(cfg, root, hr, lut, rng, g);
   Walker<Tables, Sink> w{cfg, hr, T, sink, rng, up, tw};
   const float u = w.traverse(g, 0);
   if (tr) {
@@ -878,6 +920,16 @@ void orc_deal(uint64_t seed, uint64_t tid, uint32_t tag, int N, uint8_t* cards_o
   *dealer_out = (int)(rng.next() % (uint32_t)N);
 }
 
+void orc_deal_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, uint64_t tid, uint32_t tag,
+                   uint8_t* hole_out, uint8_t* board_out) {
+  Stream rng(cfg->seed, tid, tag);
+  Game g;
+  g.N = cfg->num_players;
+  deal_cards(*cfg, root, rng, g);
+  for (int i = 0; i < 2 * g.N; ++i) hole_out[i] = g.hole[i];
+  for (int k = 0; k < 5; ++k) board_out[k] = g.comm[k];
+}
+
 void orc_regret_matching(const float* R, uint16_t* sigma, uint64_t slots, int row) {
   for (uint64_t s = 0; s < slots; ++s) {
     const float* r = R + s * row;
@@ -892,6 +944,14 @@ int orc_replay_launch(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_
                       const float* R, const uint16_t* sigma, int up, int64_t t_weight,
                       uint64_t first_tid, uint32_t n, uint32_t tag, double* dR, double* dS,
                       double* aR, double* aS, mccfr_trace* trace) {
+  return orc_replay_launch_root(cfg, nullptr, hr, lut, R, sigma, up, t_weight, first_tid, n, tag, dR, dS, aR, aS,
+                                trace);
+}
+
+int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr,
+                           const uint8_t* lut, const float* R, const uint16_t* sigma, int up, int64_t t_weight,
+                           uint64_t first_tid, uint32_t n, uint32_t tag, double* dR, double* dS, double* aR,
+                           double* aS, mccfr_trace* trace) {
   if (!cfg_ok(cfg) || !hr || !lut || !R) return -1;
   const bool accumulate = dR && dS && aR && aS;
   if (!accumulate && !trace) return -1;
@@ -909,10 +969,11 @@ int orc_replay_launch(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_
   if (accumulate) {
     SinkF64 sink{dR, dS, aR, aS, row};
     for (uint32_t i = 0; i < n; ++i)
-      run_traversal(*cfg, hr, lut, T, sink, up, tw, first_tid + i, tag, trace ? trace + i : nullptr);
+      run_traversal(*cfg, root, hr, lut, T, sink, up, tw, first_tid + i, tag, trace ? trace + i : nullptr);
   } else {
     SinkNull sink;
-    for (uint32_t i = 0; i < n; ++i) run_traversal(*cfg, hr, lut, T, sink, up, tw, first_tid + i, tag, trace + i);
+    for (uint32_t i = 0; i < n; ++i)
+      run_traversal(*cfg, root, hr, lut, T, sink, up, tw, first_tid + i, tag, trace + i);
   }
   return 0;
 }
@@ -948,7 +1009,7 @@ double orc_cpu_baseline(const mccfr_game_cfg* cfg, const int32_t* hr, const uint
       lists[t].clear();
       SinkList sink{&lists[t]};
       for (int64_t i = b; i < e; ++i)
-        run_traversal(*cfg, hr, lut, T, sink, player, tw, tid + (uint64_t)i, 0u, nullptr);
+        run_traversal(*cfg, nullptr, hr, lut, T, sink, player, tw, tid + (uint64_t)i, 0u, nullptr);
     });
     for (auto& l : lists)
       for (const Contribution& c : l) {

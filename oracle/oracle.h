@@ -75,6 +75,15 @@ int orc_replay_launch(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_
                       const float* R, const uint16_t* sigma, int update_player,
                       int64_t t_weight, uint64_t first_tid, uint32_t n, uint32_t tag,
                       double* dR, double* dS, double* aR, double* aS, mccfr_trace* trace);
+/* Same from a tenant root (mccfr.h mccfr_root_state; NULL or street 0 = the
+ * reference's preflop root). */
+int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr,
+                           const uint8_t* lut, const float* R, const uint16_t* sigma, int update_player,
+                           int64_t t_weight, uint64_t first_tid, uint32_t n, uint32_t tag, double* dR,
+                           double* dS, double* aR, double* aS, mccfr_trace* trace);
+/* Hole cards (2N) and full board (5) of traversal tid from a root. */
+void orc_deal_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, uint64_t tid, uint32_t tag,
+                   uint8_t* hole_out, uint8_t* board_out);
 /* Regret matching of slot-major rows (kernel_regret_matching, cfr_gpu.cu:318-351). */
 void orc_regret_matching(const float* R, uint16_t* sigma, uint64_t slots, int row);
 

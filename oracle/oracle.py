@@ -42,6 +42,9 @@ _SIGS = {
     "orc_deal": (None, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, _P, C.POINTER(C.c_int)]),
     "orc_replay_launch": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32, C.c_uint32,
                                     _P, _P, _P, _P, _P]),
+    "orc_replay_launch_root": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, _P, _P, _P, _P, _P]),
+    "orc_deal_root": (None, [_P, _P, C.c_uint64, C.c_uint32, _P, _P]),
     "orc_regret_matching": (None, [_P, _P, C.c_uint64, C.c_int]),
     "orc_cpu_baseline": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_uint32, C.c_int, C.c_int64,
                                       C.c_uint64, C.c_int]),
@@ -80,36 +83,48 @@ def ref_lib():
     return L
 
 
-def replay_trace(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0):
+TRACE_DTYPE = np.dtype([("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"), ("updates", "u1"),
+                        ("terminal", "u1"), ("update_legal", "<u2"), ("reserved", "<u2")])
+
+
+def _root_ptr(root):
+    return None if root is None else C.byref(root)
+
+
+def replay_trace(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, root=None):
     """Per-traversal records only (no delta accumulation): for full-size tables."""
-    tr = np.zeros(n, dtype=np.dtype([("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"),
-                                     ("updates", "u1"), ("terminal", "u1"),
-                                     ("update_legal", "<u2"), ("reserved", "<u2")]))
+    tr = np.zeros(n, dtype=TRACE_DTYPE)
     R = np.ascontiguousarray(R, np.float32)
     sig_p = None if sigma is None else p(np.ascontiguousarray(sigma, np.uint16))
-    rc = lib().orc_replay_launch(C.byref(cfg), p(hr), p(lut), p(R), sig_p, up, t_weight, first_tid, n, tag,
-                                 None, None, None, None, p(tr))
+    rc = lib().orc_replay_launch_root(C.byref(cfg), _root_ptr(root), p(hr), p(lut), p(R), sig_p, up, t_weight,
+                                      first_tid, n, tag, None, None, None, None, p(tr))
     if rc != 0:
-        raise RuntimeError("orc_replay_launch rejected its arguments")
+        raise RuntimeError("orc_replay_launch_root rejected its arguments")
     return tr
 
 
-def replay_launch(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, trace=True):
+def replay_launch(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, trace=True, root=None):
     """FP64 replay of one launch from pre-launch tables (slot-major rows).
     Returns (dR, dS, aR, aS, trace)."""
     row = 8 if cfg.num_actions <= 8 else 16
     size = (1 << cfg.table_log2) * row
     dR = np.zeros(size); dS = np.zeros(size); aR = np.zeros(size); aS = np.zeros(size)
-    tr = np.zeros(n, dtype=np.dtype([("utility", "<f4"), ("path_hash", "<u4"), ("decisions", "<u2"),
-                                     ("updates", "u1"), ("terminal", "u1"),
-                                     ("update_legal", "<u2"), ("reserved", "<u2")])) if trace else None
+    tr = np.zeros(n, dtype=TRACE_DTYPE) if trace else None
     R = np.ascontiguousarray(R, np.float32)
     sig_p = None if sigma is None else p(np.ascontiguousarray(sigma, np.uint16))
-    rc = lib().orc_replay_launch(C.byref(cfg), p(hr), p(lut), p(R), sig_p, up, t_weight, first_tid, n, tag,
-                                 p(dR), p(dS), p(aR), p(aS), p(tr) if trace else None)
+    rc = lib().orc_replay_launch_root(C.byref(cfg), _root_ptr(root), p(hr), p(lut), p(R), sig_p, up, t_weight,
+                                      first_tid, n, tag, p(dR), p(dS), p(aR), p(aS), p(tr) if trace else None)
     if rc != 0:
-        raise RuntimeError("orc_replay_launch rejected its arguments")
+        raise RuntimeError("orc_replay_launch_root rejected its arguments")
     return dR, dS, aR, aS, tr
+
+
+def deal_root(cfg, root, tid, tag=0):
+    """(hole cards [2N], board [5]) of traversal tid from a root (None = preflop)."""
+    hole = np.zeros(2 * cfg.num_players, np.uint8)
+    board = np.zeros(5, np.uint8)
+    lib().orc_deal_root(C.byref(cfg), _root_ptr(root), tid, tag, p(hole), p(board))
+    return hole, board
 
 
 def regret_matching(R, row):

@@ -16,7 +16,9 @@ constexpr int kMaxDepth = 20;  // MAX_TRAVERSE_DEPTH, cfr_gpu.cu:642
 constexpr uint32_t kHistoryMask = 0x3FFFFFFFu;  // cfr_gpu.cu:553
 
 // ---------------------------------------------------------------------------
-// Game descriptor as the kernels see it (one per game configuration).
+// Tenant descriptor: one game configuration, the root its traversals start
+// from, its tables and its update-player schedule.  The context's own game is
+// tenant 0; a batched launch runs segments of several tenants.
 // ---------------------------------------------------------------------------
 struct DevGame {
   int32_t N;
@@ -30,6 +32,9 @@ struct DevGame {
   int32_t use_linear_cfr;
   uint32_t seed_lo, seed_hi;
   uint32_t tag;          // Philox counter word 3 (stream family)
+  // stack[p]: chips of seat p at the start of the hand, so that invested =
+  // stack[p] - chips behind (preflop root: the starting stack; postflop root:
+  // chips behind at the root + chips invested before it).
   int32_t stack[kMaxPlayers];
   uint8_t kind[MCCFR_MAX_ACTIONS];
   uint8_t num[MCCFR_MAX_ACTIONS];
@@ -37,19 +42,60 @@ struct DevGame {
   uint32_t passive_order[MCCFR_MAX_ACTIONS];  // fallback preference list (menu indices)
   int32_t n_passive;
   uint32_t check_mask, call_mask, fold_mask, allin_mask, raise_mask;  // menu indices by kind
+
+  // root of the tree (street 0 = the reference's preflop root)
+  int32_t root_street;
+  int32_t root_dealer;        // postflop roots
+  uint32_t root_folded;       // seats out of the hand at the root
+  uint32_t root_allin;        // seats with no chips behind at the root
+  uint32_t root_ab;           // action-history bits at the root
+  int32_t root_pot;
+  int32_t root_stack[kMaxPlayers];  // chips behind at the root
+  uint32_t board_lo, board_hi;      // fixed board cards as a 52-bit mask
+  uint8_t board[5];
+  uint8_t n_board;            // fixed board cards: 0, 3, 4 or 5
+  uint8_t deck_size;          // 52 - n_board
+  uint8_t n_deal;             // cards dealt per traversal: 2N + 5 - n_board
+  uint32_t walk_k0;           // first draw index of the walk (after the deal draws)
+  uint8_t upd_seats[kMaxPlayers];  // update-player schedule
+  int32_t n_upd;
+
+  // tables (slot-major rows of `row` entries)
+  float* R;                   // regrets
+  float* S;                   // strategy sums
+  uint16_t* sig;              // BF16 regret-matched strategy snapshot
 };
 
-// Launch state in device memory.  The captured training graph advances it
-// with a device kernel, so a graph replay needs no host writes.
-struct LaunchState {
+// One tenant's share of a launch: lanes [lane0, lane0 + n) of the launch's
+// lane space; lane0 is a multiple of kSegAlign so every wave's lanes belong
+// to one segment.  The captured training graph advances it on the device.
+struct DevSeg {
+  uint32_t tenant;
+  uint32_t lane0;
+  uint32_t n;
   int32_t update_player;
-  int32_t n;               // traversals in this launch
-  int64_t iteration;       // training-loop iteration (1-based)
-  int64_t t_weight;        // strategy-sum weight: iteration under linear CFR, else 1
-  uint64_t first_tid;      // traversal id of lane 0
-  uint32_t epoch;          // touched-slot stamp of this launch (never 0)
-  uint32_t runs;           // entries in the launch's update list
-  uint32_t pad[2];
+  int64_t iteration;     // training-loop iteration (1-based)
+  int64_t t_weight;      // strategy-sum weight: iteration under linear CFR, else 1
+  uint64_t first_tid;    // traversal id of the segment's first lane
+  uint32_t upd_idx;      // position of update_player in the tenant's schedule
+  uint32_t pad;
+};
+constexpr uint32_t kSegAlign = 256;
+
+// Per-launch constants passed by value to the kernels.
+struct LaunchCfg {
+  uint32_t n_segs;
+  uint32_t lanes;        // lane space of the launch (sum of aligned segment sizes)
+  uint32_t slot_bits;    // record key = segment << slot_bits | slot
+  uint32_t cap_lanes;    // lane capacity of the scratch lists
+};
+
+// Counters the kernels of one launch write (reset by the launch itself).
+struct LaunchCounters {
+  uint32_t n_rec;        // update-player records emitted
+  uint32_t n_long;       // runs handed to the long-run reduction
+  uint32_t n_pieces;     // pieces of the long runs
+  uint32_t pad;
 };
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,6 @@
 // gfx950 kernels of the MCCFR hot path.
 //
-//   (the traversal pipeline -- deal, walk, scatter, update -- is in traverse.hip)
+//   (the launch pipeline -- deal, walk, record sort, table update -- is in traverse.hip)
 //   k_rm_sweep<ROW>   full-table regret matching (after tables are uploaded).
 //   k_normalize<ROW>  average strategy (kernel_normalize_strategy, cfr_gpu.cu:1029-1058).
 //   k_eval7_walk      7-card rank via the rank-walk table (eval7_gpu, cfr_gpu.cu:126-137).
@@ -14,11 +14,12 @@ namespace mccfr {
 namespace {
 
 template <int ROW>
-__global__ __launch_bounds__(256) void k_rm_sweep(DevTables T, uint64_t slots) {
+__global__ __launch_bounds__(256) void k_rm_sweep(const float* __restrict__ R, uint16_t* __restrict__ sig,
+                                                  uint64_t slots) {
   for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
        slot += (uint64_t)gridDim.x * blockDim.x) {
     float r[ROW];
-    load_f32_row<ROW>(T.R + slot * ROW, r);
+    load_f32_row<ROW>(R + slot * ROW, r);
     float sum = 0.0f;
 #pragma unroll
     for (int a = 0; a < ROW; ++a) sum += fmaxf(0.0f, r[a]);
@@ -31,7 +32,7 @@ __global__ __launch_bounds__(256) void k_rm_sweep(DevTables T, uint64_t slots) {
     }
 #pragma unroll
     for (int v = 0; v < ROW / 8; ++v)
-      reinterpret_cast<uint4*>(T.sig + slot * ROW)[v] =
+      reinterpret_cast<uint4*>(sig + slot * ROW)[v] =
           make_uint4(packed[4 * v], packed[4 * v + 1], packed[4 * v + 2], packed[4 * v + 3]);
   }
 }
@@ -58,37 +59,45 @@ __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ S, 
   }
 }
 
-__global__ void k_set_state(LaunchState* st, int up, int64_t iteration, int64_t t_weight,
-                            uint64_t first_tid, uint32_t n) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->update_player = up;
-    st->iteration = iteration;
-    st->t_weight = t_weight;
-    st->first_tid = first_tid;
-    st->n = (int32_t)n;
-    uint32_t e = st->epoch + 1u;
-    st->epoch = e ? e : 1u;
-    st->runs = 0;
+// Writes one tenant descriptor (passed by value, so the host copy may change
+// as soon as the launch is queued).
+__global__ void k_set_tenant(DevGame* dst, DevGame g) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = g;
+}
+
+// Advances every segment to its tenant's next launch of the training loop
+// (cfr_gpu.cu:1303-1318: the next seat of the schedule, or the first seat of
+// the next iteration).
+__global__ void k_advance_segments(DevSeg* segs, const DevGame* __restrict__ tenants, uint32_t n_segs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_segs; i += gridDim.x * blockDim.x) {
+    DevSeg s = segs[i];
+    const DevGame& G = tenants[s.tenant];
+    s.first_tid += s.n;
+    if (++s.upd_idx >= (uint32_t)G.n_upd) {
+      s.upd_idx = 0;
+      s.iteration += 1;
+    }
+    s.update_player = G.upd_seats[s.upd_idx];
+    s.t_weight = G.use_linear_cfr ? s.iteration : 1;
+    segs[i] = s;
   }
 }
 
-// Advances the state to the next launch of the training loop
-// (cfr_gpu.cu:1303-1318: player p+1, or player 0 of iteration+1).
-__global__ void k_advance_state(LaunchState* st, int num_players, int use_linear_cfr, uint32_t batch) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->first_tid += batch;
-    st->n = (int32_t)batch;
-    int p = st->update_player + 1;
-    if (p >= num_players) {
-      p = 0;
-      st->iteration += 1;
-    }
-    st->update_player = p;
-    st->t_weight = use_linear_cfr ? st->iteration : 1;
-    uint32_t e = st->epoch + 1u;
-    st->epoch = e ? e : 1u;
-    st->runs = 0;
+// kernel_clamp_regrets (cfr_gpu.cu:1018-1022) over a whole table.
+__global__ __launch_bounds__(256) void k_clamp(float4* __restrict__ R, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+    float4 v = R[i];
+    v.x = fmaxf(0.0f, v.x);
+    v.y = fmaxf(0.0f, v.y);
+    v.z = fmaxf(0.0f, v.z);
+    v.w = fmaxf(0.0f, v.w);
+    R[i] = v;
   }
+}
+
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,8 +191,8 @@ __global__ __launch_bounds__(256) void k_random_hands(uint32_t seed_lo, uint32_t
 }
 
 template <int ROW>
-__global__ __launch_bounds__(256) void k_randomize(DevTables T, uint64_t slots, int num_actions, uint32_t seed_lo,
-                                                   uint32_t seed_hi) {
+__global__ __launch_bounds__(256) void k_randomize(float* __restrict__ R, float* __restrict__ S, uint64_t slots,
+                                                   int num_actions, uint32_t seed_lo, uint32_t seed_hi) {
   for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
        slot += (uint64_t)gridDim.x * blockDim.x) {
     DrawStream rng;
@@ -193,8 +202,8 @@ __global__ __launch_bounds__(256) void k_randomize(DevTables T, uint64_t slots, 
     for (int a = 0; a < ROW; ++a) r[a] = a < num_actions ? u01(rng.next()) : 0.0f;
 #pragma unroll
     for (int v = 0; v < ROW / 4; ++v) {
-      reinterpret_cast<float4*>(T.R + slot * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
-      reinterpret_cast<float4*>(T.S + slot * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+      reinterpret_cast<float4*>(R + slot * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
+      reinterpret_cast<float4*>(S + slot * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 }
@@ -246,19 +255,24 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 }  // namespace
 
-void launch_set_state(hipStream_t s, LaunchState* st, int up, int64_t iteration, int64_t t_weight,
-                      uint64_t first_tid, uint32_t n) {
-  hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, s, st, up, iteration, t_weight, first_tid, n);
+void launch_set_tenant(hipStream_t s, DevGame* dst, const DevGame& g) {
+  hipLaunchKernelGGL(k_set_tenant, dim3(1), dim3(64), 0, s, dst, g);
 }
-void launch_advance_state(hipStream_t s, LaunchState* st, int num_players, int use_linear_cfr, uint32_t batch) {
-  hipLaunchKernelGGL(k_advance_state, dim3(1), dim3(64), 0, s, st, num_players, use_linear_cfr, batch);
+void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants, uint32_t n_segs) {
+  hipLaunchKernelGGL(k_advance_segments, dim3(grid_for(n_segs, 256, 64)), dim3(256), 0, s, segs, tenants, n_segs);
 }
-void launch_rm_sweep(hipStream_t s, int row, DevTables t, uint64_t slots) {
+void launch_zero(hipStream_t s, void* p, size_t bytes) {  // bytes: a multiple of 16
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(bytes / 16, 256, 1u << 16)), dim3(256), 0, s, (uint4*)p, bytes / 16);
+}
+void launch_clamp(hipStream_t s, float* R, size_t elems) {  // elems: a multiple of 4
+  hipLaunchKernelGGL(k_clamp, dim3(grid_for(elems / 4, 256, 1u << 16)), dim3(256), 0, s, (float4*)R, elems / 4);
+}
+void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots) {
   const unsigned grid = grid_for(slots, 256, 65536);
   if (row == 8)
-    hipLaunchKernelGGL(k_rm_sweep<8>, dim3(grid), dim3(256), 0, s, t, slots);
+    hipLaunchKernelGGL(k_rm_sweep<8>, dim3(grid), dim3(256), 0, s, R, sig, slots);
   else
-    hipLaunchKernelGGL(k_rm_sweep<16>, dim3(grid), dim3(256), 0, s, t, slots);
+    hipLaunchKernelGGL(k_rm_sweep<16>, dim3(grid), dim3(256), 0, s, R, sig, slots);
 }
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots) {
   const unsigned grid = grid_for(slots, 256, 65536);
@@ -280,13 +294,13 @@ void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* ou
   hipLaunchKernelGGL(k_random_hands, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, (uint32_t)seed,
                      (uint32_t)(seed >> 32), tag, out, n);
 }
-void launch_randomize(hipStream_t s, int row, int num_actions, DevTables t, uint64_t slots, uint64_t seed) {
+void launch_randomize(hipStream_t s, int row, int num_actions, float* R, float* S, uint64_t slots, uint64_t seed) {
   const unsigned grid = grid_for(slots, 256, 65536);
   if (row == 8)
-    hipLaunchKernelGGL(k_randomize<8>, dim3(grid), dim3(256), 0, s, t, slots, num_actions, (uint32_t)seed,
+    hipLaunchKernelGGL(k_randomize<8>, dim3(grid), dim3(256), 0, s, R, S, slots, num_actions, (uint32_t)seed,
                        (uint32_t)(seed >> 32));
   else
-    hipLaunchKernelGGL(k_randomize<16>, dim3(grid), dim3(256), 0, s, t, slots, num_actions, (uint32_t)seed,
+    hipLaunchKernelGGL(k_randomize<16>, dim3(grid), dim3(256), 0, s, R, S, slots, num_actions, (uint32_t)seed,
                        (uint32_t)(seed >> 32));
 }
 void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b) {

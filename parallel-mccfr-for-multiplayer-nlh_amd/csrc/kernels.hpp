@@ -1,4 +1,4 @@
-// Launch wrappers of the gfx950 kernels (defined in kernels.hip).
+// Launch wrappers of the gfx950 kernels (defined in kernels.hip and traverse.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -7,28 +7,42 @@
 
 namespace mccfr {
 
-// Update-player nodes a lane keeps in registers; deeper ones go to the
-// lane's overflow entries in HBM.
+// Update-player nodes a lane keeps in LDS during its walk; deeper ones are
+// written to the lane's node list in HBM when they are reached.
 constexpr int kRegPending = 4;
 
-// Device tables of one game (slot-major rows of `row` entries) and the
-// per-launch node lists (lane-strided: entry k of lane l at [k * lanes + l]).
-struct DevTables {
-  float* R;             // regrets
-  float* S;             // strategy sums
-  uint16_t* sig;        // BF16 regret-matched strategy snapshot
-  double* D;            // per-launch FP64 deltas: [slot][0..row) regret, [row..2row) strategy sum
-  uint32_t* stamp;      // last launch epoch whose update pass handled the slot
-  uint32_t* node_slot;  // [kMaxDepth][lanes] info-set slot of each update-player node
-  uint32_t* node_av;    // [kMaxDepth][lanes] sampled action | legal mask << 8
-  uint8_t* node_count;  // [lanes] update-player nodes of the traversal
-  float* node_u;        // [lanes] terminal utility of the update player
-  uint32_t* root_bkt;   // [kMaxPlayers][lanes] hole | flop << 8 | turn << 16 | river << 24 buckets
-  uint16_t* root_rank;  // [kMaxPlayers][lanes] 7-card rank (showdown)
-  uint8_t* root_dealer; // [lanes]
-  uint32_t* upd_list;   // [kMaxDepth * lanes] slots to update (one entry per slot per scatter tile)
-  uint32_t lanes;       // lane capacity of the lists (max batch)
+// Radix sort of the launch's records: 8-bit digits, tiles of
+// kSortWG threads x kSortItems keys.
+constexpr int kSortWG = 256;
+constexpr int kSortItems = 16;
+constexpr uint32_t kSortTile = kSortWG * kSortItems;
+constexpr uint32_t kSortBins = 256;
+
+// Scratch of one launch, sized for `cap_lanes` lanes.  Node lists are
+// lane-strided: entry k of lane l at [k * cap_lanes + l].
+struct DevScratch {
+  uint32_t* node_slot;   // [kMaxDepth][lanes] info-set slot of each update-player node
+  uint32_t* node_av;     // [kMaxDepth][lanes] sampled action | legal mask << 8
+  uint8_t* node_count;   // [lanes] update-player nodes of the traversal (0 on padding lanes)
+  float* node_u;         // [lanes] terminal utility of the update player
+  uint32_t* root_bkt;    // [kMaxPlayers][lanes] hole | flop << 8 | turn << 16 | river << 24 buckets
+  uint16_t* root_rank;   // [kMaxPlayers][lanes] 7-card rank (showdown)
+  uint8_t* root_dealer;  // [lanes]
+  uint32_t* blk_cnt;     // [lanes / kEmitLanes] records per emission block, then their offsets
+  uint32_t* key[2];      // [kMaxDepth * lanes] record keys (segment << slot_bits | slot), ping-pong
+  uint32_t* val[2];      // [kMaxDepth * lanes] record references (lane << 5 | node index)
+  uint32_t* hist;        // [kSortBins][max_tiles] tile digit counts, then per-digit prefixes over tiles
+  uint32_t* bin_total;   // [kSortBins] keys per digit in the current pass
+  uint4* long_runs;      // [max_long] long runs: first record, records, first piece
+  uint32_t* piece_run;   // [max_pieces] long run of each piece
+  double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
+  LaunchCounters* cnt;
+  uint32_t cap_lanes;
+  uint32_t max_tiles;
 };
+constexpr uint32_t kEmitLanes = 1024;  // lanes per emission block
+constexpr uint32_t kShortRun = 16;     // longer runs are split into pieces reduced by whole workgroups
+constexpr uint32_t kPiece = 1024;      // records per piece of a long run
 
 // Tables of the arithmetic 7-card evaluator (built on the host).
 constexpr int kNonFlushClasses = 49205;  // rank multisets of 7 cards, <= 4 per rank
@@ -46,26 +60,41 @@ struct BucketTables {
   uint8_t* turn6;  // [kTurnSets]
 };
 
+// Shared, read-only inputs of the traversal.
+struct DealInputs {
+  BucketTables bt;
+  ArithTables at;
+  const uint8_t* lut;  // [52][52] preflop buckets
+};
+
+// Shape of the walk launch (autotuned): workgroup size and traversals per lane.
+struct WalkShape {
+  uint32_t wg = 256;
+  uint32_t tpl = 1;
+};
+
 void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b);
-void launch_set_state(hipStream_t s, LaunchState* st, int update_player, int64_t iteration,
-                      int64_t t_weight, uint64_t first_tid, uint32_t n);
-void launch_advance_state(hipStream_t s, LaunchState* st, int num_players, int use_linear_cfr,
-                          uint32_t batch);
-// Deal + walk + scatter of one launch (traverse.hip).  walk_blocks = number of
-// persistent 256-lane walk workgroups (0: one lane per traversal); scatter_wg =
-// scatter workgroup size (256, 512 or 1024 threads, 4 traversals per thread).
-void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, BucketTables bt,
-                     ArithTables at, const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks,
-                     uint32_t scatter_wg, mccfr_trace* trace);
-void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t, uint32_t n_lanes);
-void launch_rm_sweep(hipStream_t s, int row, DevTables t, uint64_t slots);
+void launch_set_tenant(hipStream_t s, DevGame* dst, const DevGame& g);
+void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants, uint32_t n_segs);
+
+// One launch (traverse.hip): deal and walk of every segment's traversals
+// (launch_walk), then records -> sort -> owner-computes table update
+// (launch_reduce).  `trace` (device, indexed by lane) may be null.
+void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace);
+void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                   const DevScratch& sc, uint32_t key_bits);
+
+void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);
+void launch_zero(hipStream_t s, void* p, size_t bytes);
+void launch_clamp(hipStream_t s, float* R, size_t elems);
 void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, size_t stride,
                        uint16_t* out, size_t n);
 void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size_t stride,
                         uint16_t* out, size_t n);
 void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* out, size_t n);
-void launch_randomize(hipStream_t s, int row, int num_actions, DevTables t, uint64_t slots, uint64_t seed);
+void launch_randomize(hipStream_t s, int row, int num_actions, float* R, float* S, uint64_t slots, uint64_t seed);
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16);
 
 }  // namespace mccfr

@@ -1,6 +1,7 @@
-// Host runtime behind the C ABI (include/mccfr/mccfr.h): contexts, device
-// buffers, table upload/download, launch sequencing and the hipGraph
-// training loop.  Replaces GPUCFRTrainer's host side (cfr_gpu.cu:1064-1538).
+// Host runtime behind the C ABI (include/mccfr/mccfr.h): contexts, tenants,
+// device buffers, table upload/download, batched launch sequencing, the
+// hipGraph training loop and the launch-shape autotuner.  Replaces
+// GPUCFRTrainer's host side (cfr_gpu.cu:1064-1538).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,55 +18,14 @@
 #include "handranks.hpp"
 #include "host_tables.hpp"
 #include "kernels.hpp"
+#include "runtime.hpp"
 
 using namespace mccfr;
 
-struct mccfr_ctx {
-  mccfr_game_cfg cfg{};
-  DevGame hg{};
-  DevGame* dg = nullptr;
-  int row = 8;
-  uint64_t slots = 0;
-  uint32_t max_batch = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-
-  DevTables t{};
-  LaunchState* st = nullptr;
-  int32_t* hr = nullptr;
-  uint8_t* lut = nullptr;
-  ArithTables arith{};
-  BucketTables buckets{};
-  mccfr_trace* dtrace = nullptr;
-
-  uint64_t traversals = 0, launches = 0;
-  int64_t next_iter = 1;
-  bool sigma_dirty = false;
-
-  hipGraphExec_t gexec = nullptr;
-  uint32_t graph_batch = 0;
-  uint32_t walk_blocks = 0;   // persistent walk workgroups per launch (0: one lane per traversal)
-  uint32_t scatter_wg = 1024; // scatter workgroup size (4 traversals per thread)
-
-  bool timing = false;
-  std::vector<hipEvent_t> ev;  // 3 events per timed launch: start, after traverse, after update
-  uint32_t ev_next = 0, ev_used = 0;
-
-  std::string err;
-};
-
 namespace {
 
-constexpr uint32_t kTimingRing = 1024;
 std::string g_err;
 std::mutex g_err_mu;
-
-int fail(mccfr_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  std::lock_guard<std::mutex> lk(g_err_mu);
-  g_err = msg;
-  return code;
-}
 
 #define HIPX(ctx, call)                                                                      \
   do {                                                                                      \
@@ -103,52 +63,80 @@ bool table_ok(const int32_t* hr, size_t words) {  // hand_eval.cpp:59-86
   return decode_packed((uint32_t)hr[p + card_rm(royal[6])] & 0xFFFFu) == 7462u;
 }
 
-// Per-launch node lists for `lanes` traversals (kMaxDepth entries per lane).
-hipError_t alloc_node_lists(DevTables& t, uint32_t lanes) {
-  hipError_t e = hipMalloc(&t.node_slot, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&t.node_av, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&t.node_count, (size_t)lanes);
-  if (e == hipSuccess) e = hipMalloc(&t.node_u, (size_t)lanes * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(&t.root_bkt, (size_t)kMaxPlayers * lanes * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&t.root_rank, (size_t)kMaxPlayers * lanes * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMalloc(&t.root_dealer, (size_t)lanes);
-  if (e == hipSuccess) e = hipMalloc(&t.upd_list, (size_t)kMaxDepth * lanes * sizeof(uint32_t));
-  t.lanes = lanes;
+uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+// Launch scratch for `cap` lanes (a multiple of kSegAlign).
+hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
+  std::memset(&s, 0, sizeof s);
+  const size_t L = cap;
+  const size_t recs = L * kMaxDepth;
+  s.cap_lanes = cap;
+  s.max_tiles = (uint32_t)((recs + kSortTile - 1) / kSortTile);
+  hipError_t e = hipMalloc(&s.node_slot, recs * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.node_av, recs * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.node_count, L);
+  if (e == hipSuccess) e = hipMalloc(&s.node_u, L * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&s.root_bkt, (size_t)kMaxPlayers * L * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.root_rank, (size_t)kMaxPlayers * L * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc(&s.root_dealer, L);
+  if (e == hipSuccess) e = hipMalloc(&s.blk_cnt, ((L + kEmitLanes - 1) / kEmitLanes) * sizeof(uint32_t));
+  for (int k = 0; k < 2; ++k) {
+    if (e == hipSuccess) e = hipMalloc(&s.key[k], recs * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s.val[k], recs * sizeof(uint32_t));
+  }
+  if (e == hipSuccess) e = hipMalloc(&s.hist, (size_t)kSortBins * s.max_tiles * sizeof(uint32_t));
+  const size_t max_long = recs / (kShortRun + 1) + 1;
+  const size_t max_pieces = recs / kPiece + max_long;
+  if (e == hipSuccess) e = hipMalloc(&s.bin_total, kSortBins * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.long_runs, max_long * sizeof(uint4));
+  if (e == hipSuccess) e = hipMalloc(&s.piece_run, max_pieces * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.partial, max_pieces * 32 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&s.cnt, sizeof(LaunchCounters));
+  if (e == hipSuccess) e = hipMemset(s.cnt, 0, sizeof(LaunchCounters));
   return e;
 }
 
-void free_node_lists(DevTables& t) {
-  void* ptrs[] = {t.node_slot, t.node_av, t.node_count, t.node_u, t.root_bkt, t.root_rank, t.root_dealer,
-                  t.upd_list};
+void free_scratch(DevScratch& s) {
+  void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_dealer,
+                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.long_runs,
+                  s.piece_run, s.partial, s.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  t.node_slot = nullptr;
-  t.node_av = nullptr;
-  t.node_count = nullptr;
-  t.node_u = nullptr;
-  t.root_bkt = nullptr;
-  t.root_rank = nullptr;
-  t.root_dealer = nullptr;
-  t.upd_list = nullptr;
+  std::memset(&s, 0, sizeof s);
+}
+
+void drop_graph(mccfr_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->graph_key.clear();
 }
 
 void free_all(mccfr_ctx* c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  drop_graph(c);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  free_node_lists(c->t);
-  void* ptrs[] = {c->dg, c->t.R, c->t.S, c->t.sig, c->t.D, c->t.stamp, c->st, c->hr, c->lut,
-                  c->arith.nonflush, c->arith.flush, c->arith.offsets, c->dtrace,
-                  c->buckets.flop5, c->buckets.turn6};
+  for (hipEvent_t e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
+  free_scratch(c->sc);
+  for (auto& t : c->tenants)
+    if (t.used && t.owned) (void)hipFree(t.owned);
+  void* ptrs[] = {c->d_tenants, c->d_segs, c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets,
+                  c->dtrace, c->buckets.flop5, c->buckets.turn6};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-int rm_sweep_if_dirty(mccfr_ctx* c) {
-  if (!c->sigma_dirty) return MCCFR_OK;
-  launch_rm_sweep(c->stream, c->row, c->t, c->slots);
+Tenant* get_tenant(mccfr_ctx* c, uint32_t id) {
+  if (!c || id >= c->tenants.size() || !c->tenants[id].used) return nullptr;
+  return &c->tenants[id];
+}
+
+int rm_sweep_if_dirty(mccfr_ctx* c, Tenant& t) {
+  if (!t.sigma_dirty) return MCCFR_OK;
+  launch_rm_sweep(c->stream, t.row, t.hg.R, t.hg.sig, t.slots);
   HIPX(c, hipGetLastError());
-  c->sigma_dirty = false;
+  t.sigma_dirty = false;
   return MCCFR_OK;
 }
 
@@ -164,23 +152,131 @@ void to_aos(const T* soa, T* aos, uint64_t slots, int row) {
     for (int a = 0; a < row; ++a) aos[s * row + a] = soa[(uint64_t)a * slots + s];
 }
 
+// Checked, laid-out segments of one launch.
+struct Prepared {
+  std::vector<DevSeg> segs;
+  LaunchCfg lc{};
+  uint32_t key_bits = 0;
+  int row = 8;
+};
+
+int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, Prepared& p) {
+  if (!c->hr) return fail(c, MCCFR_E_STATE, "no hand table loaded");
+  if (n == 0 || n > kMaxSegs) return fail(c, MCCFR_E_ARG, "a launch needs 1..4096 segments");
+  p.segs.resize(n);
+  std::vector<uint8_t> seen(c->tenants.size(), 0);
+  uint32_t lanes = 0, slot_bits = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    Tenant* t = get_tenant(c, hs[i].tenant);
+    if (!t) return fail(c, MCCFR_E_ARG, "segment " + std::to_string(i) + " names no tenant");
+    if (seen[hs[i].tenant]++) return fail(c, MCCFR_E_ARG, "a tenant appears in two segments of one launch");
+    if (i == 0) p.row = t->row;
+    if (t->row != p.row) return fail(c, MCCFR_E_ARG, "segments of one launch need the same row width");
+    if (hs[i].n == 0) return fail(c, MCCFR_E_ARG, "segment with no traversals");
+    if (hs[i].update_player < 0 || hs[i].update_player >= t->cfg.num_players)
+      return fail(c, MCCFR_E_ARG, "bad update player");
+    const uint32_t padded = round_up(hs[i].n, kSegAlign);
+    if ((uint64_t)lanes + padded > c->sc.cap_lanes)
+      return fail(c, MCCFR_E_ARG,
+                  "launch needs more lanes than max_batch (segments are padded to 256; see mccfr_set_max_batch)");
+    DevSeg& d = p.segs[i];
+    std::memset(&d, 0, sizeof d);
+    d.tenant = hs[i].tenant;
+    d.lane0 = lanes;
+    d.n = hs[i].n;
+    d.update_player = hs[i].update_player;
+    d.iteration = hs[i].iteration;
+    d.t_weight = hs[i].t_weight;
+    d.first_tid = hs[i].first_tid;
+    d.upd_idx = hs[i].upd_idx;
+    lanes += padded;
+    slot_bits = std::max(slot_bits, (uint32_t)t->cfg.table_log2);
+  }
+  const uint32_t seg_bits = n > 1 ? 32u - (uint32_t)__builtin_clz(n - 1) : 0u;
+  if (slot_bits + seg_bits > 32)
+    return fail(c, MCCFR_E_ARG, "too many segments for the tables' slot bits (slot bits + segment bits > 32)");
+  p.lc.n_segs = n;
+  p.lc.lanes = lanes;
+  p.lc.slot_bits = slot_bits;
+  p.lc.cap_lanes = c->sc.cap_lanes;
+  p.key_bits = slot_bits + seg_bits;
+  return MCCFR_OK;
+}
+
+// Copies the segment table to the device through a pinned staging slot.
+int stage_segments(mccfr_ctx* c, const std::vector<DevSeg>& segs) {
+  const uint32_t k = c->stage_next++ % kStageRing;
+  HIPX(c, hipEventSynchronize(c->stage_ev[k]));  // the slot's previous copy has left it
+  DevSeg* h = c->h_stage + (size_t)k * kMaxSegs;
+  std::memcpy(h, segs.data(), segs.size() * sizeof(DevSeg));
+  HIPX(c, hipMemcpyAsync(c->d_segs, h, segs.size() * sizeof(DevSeg), hipMemcpyHostToDevice, c->stream));
+  HIPX(c, hipEventRecord(c->stage_ev[k], c->stream));
+  return MCCFR_OK;
+}
+
+int record_event(mccfr_ctx* c, int k) {
+  if (!c->timing) return MCCFR_OK;
+  hipEvent_t e = c->ev[(size_t)(c->ev_next % kTimingRing) * 3 + k];
+  HIPX(c, hipEventRecord(e, c->stream));
+  return MCCFR_OK;
+}
+
+DealInputs deal_inputs(const mccfr_ctx* c) {
+  DealInputs in;
+  in.bt = c->buckets;
+  in.at = c->arith;
+  in.lut = c->lut;
+  return in;
+}
+
+// The host mirror of k_advance_segments.
+void advance(const Tenant& t, HostSeg& s) {
+  s.first_tid += s.n;
+  if (++s.upd_idx >= (uint32_t)t.hg.n_upd) {
+    s.upd_idx = 0;
+    s.iteration += 1;
+  }
+  s.update_player = t.hg.upd_seats[s.upd_idx];
+  s.t_weight = t.cfg.use_linear_cfr ? s.iteration : 1;
+}
+
+HostSeg next_of(uint32_t id, const Tenant& t, uint32_t n) {
+  HostSeg s;
+  s.tenant = id;
+  s.n = n;
+  s.upd_idx = t.upd_idx;
+  s.update_player = t.hg.upd_seats[t.upd_idx];
+  s.iteration = t.next_iter;
+  s.t_weight = t.cfg.use_linear_cfr ? t.next_iter : 1;
+  s.first_tid = t.traversals;
+  return s;
+}
+
 }  // namespace
 
 namespace mccfr {
 
-bool make_dev_game(const mccfr_game_cfg& cfg, DevGame& g, const char** why) {
+int fail(mccfr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = msg;
+  return code;
+}
+
+bool make_dev_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, DevGame& g, const char** why) {
   auto bad = [&](const char* m) {
     if (why) *why = m;
     return false;
   };
   if (cfg.num_players < 2 || cfg.num_players > kMaxPlayers) return bad("num_players must be 2..6");
   if (cfg.num_actions < 1 || cfg.num_actions > MCCFR_MAX_ACTIONS) return bad("num_actions must be 1..16");
-  if (cfg.table_log2 < 4 || cfg.table_log2 > 32) return bad("table_log2 must be 4..32");
+  if (cfg.table_log2 < 4 || cfg.table_log2 > 31) return bad("table_log2 must be 4..31");
   if (cfg.history_bits < 1 || cfg.history_bits > 8) return bad("history_bits must be 1..8");
   if (cfg.starting_stack < 0 || cfg.small_blind < 0 || cfg.big_blind < 0) return bad("negative chips");
   if (cfg.starting_stack > 32767) return bad("starting_stack must be <= 32767 chips");
   std::memset(&g, 0, sizeof g);
-  g.N = cfg.num_players;
+  const int N = cfg.num_players;
+  g.N = N;
   g.sb = cfg.small_blind;
   g.bb = cfg.big_blind;
   g.num_actions = cfg.num_actions;
@@ -193,7 +289,6 @@ bool make_dev_game(const mccfr_game_cfg& cfg, DevGame& g, const char** why) {
   g.seed_lo = (uint32_t)cfg.seed;
   g.seed_hi = (uint32_t)(cfg.seed >> 32);
   g.tag = 0;
-  for (int p = 0; p < kMaxPlayers; ++p) g.stack[p] = p < cfg.num_players ? cfg.starting_stack : 0;
   int have_passive = 0;
   for (int a = 0; a < cfg.num_actions; ++a) {
     const mccfr_action& m = cfg.menu[a];
@@ -212,6 +307,7 @@ bool make_dev_game(const mccfr_game_cfg& cfg, DevGame& g, const char** why) {
       default: g.raise_mask |= bit; break;
     }
   }
+  for (int a = cfg.num_actions; a < MCCFR_MAX_ACTIONS; ++a) g.den[a] = 1;
   if (!have_passive) return bad("menu needs a check or check/call action");
   if (!g.fold_mask) return bad("menu needs a fold action");
   if (!g.allin_mask) return bad("menu needs an all-in action");
@@ -226,7 +322,169 @@ bool make_dev_game(const mccfr_game_cfg& cfg, DevGame& g, const char** why) {
     if (g.raise_mask >> a & 1u) g.passive_order[g.n_passive++] = (uint32_t)a;
   for (int a = 0; a < cfg.num_actions; ++a)
     if (g.allin_mask >> a & 1u) g.passive_order[g.n_passive++] = (uint32_t)a;
+
+  if (!root || root->street == 0) {  // the reference's preflop root
+    for (int p = 0; p < N; ++p) g.stack[p] = g.root_stack[p] = cfg.starting_stack;
+    g.n_board = 0;
+    g.deck_size = 52;
+    g.n_deal = (uint8_t)(2 * N + 5);
+    g.walk_k0 = (uint32_t)(2 * N + 6);
+    for (int p = 0; p < N; ++p) g.upd_seats[p] = (uint8_t)p;
+    g.n_upd = N;
+    return true;
+  }
+  if (root->street < 1 || root->street > 3) return bad("root street must be 0 (preflop), 1, 2 or 3");
+  if (root->dealer < 0 || root->dealer >= N) return bad("root dealer must be a seat");
+  const uint32_t all = (1u << N) - 1u;
+  if (root->active_mask & ~all) return bad("root active_mask names seats beyond num_players");
+  if (__builtin_popcount(root->active_mask) < 2) return bad("a root needs at least two active seats");
+  const int nb = root->street + 2;
+  uint64_t bm = 0;
+  for (int k = 0; k < nb; ++k) {
+    if (root->board[k] >= 52) return bad("board card codes must be 0..51");
+    if (bm >> root->board[k] & 1ull) return bad("board cards must be distinct");
+    bm |= 1ull << root->board[k];
+    g.board[k] = root->board[k];
+  }
+  int pot = 0;
+  for (int p = 0; p < N; ++p) {
+    const int st = root->stack[p], inv = root->invested[p];
+    if (st < 0 || inv < 0) return bad("negative root stack or investment");
+    if (st + inv > 32767) return bad("root stack + investment must be <= 32767 chips");
+    g.root_stack[p] = st;
+    g.stack[p] = st + inv;
+    pot += inv;
+    const bool act = root->active_mask >> p & 1u;
+    if (!act) g.root_folded |= 1u << p;
+    if (act && st == 0) g.root_allin |= 1u << p;
+  }
+  g.root_street = root->street;
+  g.root_dealer = root->dealer;
+  g.root_pot = pot;
+  g.root_ab = root->history & kHistoryMask;
+  g.board_lo = (uint32_t)bm;
+  g.board_hi = (uint32_t)(bm >> 32);
+  g.n_board = (uint8_t)nb;
+  g.deck_size = (uint8_t)(52 - nb);
+  g.n_deal = (uint8_t)(2 * N + 5 - nb);
+  g.walk_k0 = (uint32_t)(2 * N + 5 - nb);
+  g.n_upd = 0;
+  for (int p = 0; p < N; ++p)
+    if (root->active_mask >> p & 1u) g.upd_seats[g.n_upd++] = (uint8_t)p;
   return true;
+}
+
+uint64_t tenant_bytes(const mccfr_game_cfg& cfg) {
+  const uint64_t row = cfg.num_actions <= 8 ? 8 : 16;
+  return (1ull << cfg.table_log2) * row * (4 + 4 + 2);
+}
+
+int tenant_create(mccfr_ctx* c, const mccfr_game_cfg* cfg, const mccfr_root_state* root, void* mem, uint32_t* id) {
+  if (!cfg || !id) return fail(c, MCCFR_E_ARG, "null argument");
+  Tenant t;
+  const char* why = nullptr;
+  if (!make_dev_game(*cfg, root, t.hg, &why)) return fail(c, MCCFR_E_ARG, why);
+  t.cfg = *cfg;
+  t.row = t.hg.row;
+  t.slots = 1ull << cfg->table_log2;
+  uint32_t k = 0;
+  while (k < c->tenants.size() && c->tenants[k].used) ++k;
+  if (k >= kMaxTenants) return fail(c, MCCFR_E_ARG, "too many tenants (4096)");
+  const size_t e = t.elems();
+  if (!mem) {
+    HIPX(c, hipMalloc(&t.owned, e * 10));
+    mem = t.owned;
+  }
+  t.hg.R = static_cast<float*>(mem);
+  t.hg.S = t.hg.R + e;
+  t.hg.sig = reinterpret_cast<uint16_t*>(t.hg.S + e);
+  launch_zero(c->stream, mem, e * 10);
+  launch_set_tenant(c->stream, c->d_tenants + k, t.hg);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    if (t.owned) (void)hipFree(t.owned);
+    return fail(c, MCCFR_E_HIP, std::string("tenant set-up: ") + hipGetErrorString(err));
+  }
+  t.used = true;
+  if (k == c->tenants.size()) c->tenants.push_back(t);
+  else c->tenants[k] = t;
+  *id = k;
+  return MCCFR_OK;
+}
+
+int tenant_destroy(mccfr_ctx* c, uint32_t id) {
+  Tenant* t = get_tenant(c, id);
+  if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+  if (t->owned) {
+    HIPX(c, hipStreamSynchronize(c->stream));
+    HIPX(c, hipFree(t->owned));
+  }
+  *t = Tenant();
+  drop_graph(c);  // the graph may hold launches of this tenant id
+  return MCCFR_OK;
+}
+
+int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done) {
+  Prepared p;
+  int rc = prepare(c, segs, n, p);
+  if (rc) return rc;
+  for (uint32_t i = 0; i < n; ++i)
+    if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;
+  if (host_trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->sc.cap_lanes * sizeof(mccfr_trace)));
+  if ((rc = stage_segments(c, p.segs))) return rc;
+  if ((rc = record_event(c, 0))) return rc;
+  launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape,
+              host_trace ? c->dtrace : nullptr);
+  if ((rc = record_event(c, 1))) return rc;
+  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+  if ((rc = record_event(c, 2))) return rc;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    Tenant& t = c->tenants[segs[i].tenant];
+    if (t.clamp_pending) {  // the reference clamps every slot after every launch (cfr_gpu.cu:1018-1022)
+      launch_clamp(c->stream, t.hg.R, t.elems());
+      t.clamp_pending = false;
+    }
+    t.traversals += segs[i].n;
+    ++t.launches;
+    total += segs[i].n;
+  }
+  HIPX(c, hipGetLastError());
+  if (done) HIPX(c, hipEventRecord(done, c->stream));
+  if (c->timing) {
+    ++c->ev_next;
+    c->ev_used = std::min(c->ev_used + 1, kTimingRing);
+  }
+  c->traversals += total;
+  ++c->launches;
+  if (host_trace) {
+    size_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      HIPX(c, hipMemcpyAsync(host_trace + off, c->dtrace + p.segs[i].lane0, (size_t)segs[i].n * sizeof(mccfr_trace),
+                             hipMemcpyDeviceToHost, c->stream));
+      off += segs[i].n;
+    }
+    HIPX(c, hipStreamSynchronize(c->stream));
+  }
+  return MCCFR_OK;
+}
+
+int tenant_normalized(mccfr_ctx* c, uint32_t id, float* out, size_t elems, int layout) {
+  Tenant* t = get_tenant(c, id);
+  if (!t || !out) return fail(c, MCCFR_E_ARG, "no such tenant or null output");
+  const size_t want = t->elems();
+  if (elems != want) return fail(c, MCCFR_E_ARG, "elems must equal slots * row_actions");
+  if (layout != 0 && layout != 1) return fail(c, MCCFR_E_ARG, "layout must be 0 (rows) or 1 (planes)");
+  float* d = nullptr;
+  HIPX(c, hipMalloc(&d, want * sizeof(float)));
+  launch_normalize(c->stream, t->row, t->hg.S, d, t->slots);
+  std::vector<float> tmp(layout ? want : 0);
+  hipError_t e = hipMemcpyAsync(layout ? tmp.data() : out, d, want * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, MCCFR_E_HIP, hipGetErrorString(e));
+  if (layout) to_soa(tmp.data(), out, t->slots, t->row);
+  return MCCFR_OK;
 }
 
 }  // namespace mccfr
@@ -264,16 +522,13 @@ int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out)
   if (!cfg || !out) return fail(nullptr, MCCFR_E_ARG, "null argument");
   *out = nullptr;
   if (max_batch == 0 || max_batch > (1u << 26)) return fail(nullptr, MCCFR_E_ARG, "max_batch must be 1..2^26");
+  {
+    DevGame probe;
+    const char* why = nullptr;
+    if (!make_dev_game(*cfg, nullptr, probe, &why)) return fail(nullptr, MCCFR_E_ARG, why);
+  }
   auto* c = new (std::nothrow) mccfr_ctx();
   if (!c) return fail(nullptr, MCCFR_E_NOMEM, "host allocation failed");
-  const char* why = nullptr;
-  if (!make_dev_game(*cfg, c->hg, &why)) {
-    delete c;
-    return fail(nullptr, MCCFR_E_ARG, why);
-  }
-  c->cfg = *cfg;
-  c->row = c->hg.row;
-  c->slots = 1ull << cfg->table_log2;
   c->max_batch = max_batch;
   auto bail = [&](int code) {
     std::string m = c->err;
@@ -290,45 +545,31 @@ int mccfr_create(const mccfr_game_cfg* cfg, uint32_t max_batch, mccfr_ctx** out)
     }                                                                                           \
   } while (0)
   CHK(hipGetDevice(&c->device));
-  {
-    int cus = 0;
-    CHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    (void)cus;
-    c->walk_blocks = 0;  // one lane per traversal: measured fastest on MI355X (persistent grids 5-80% slower)
-  }
   CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  const size_t elems = (size_t)c->slots * c->row;
-  CHK(hipMalloc(&c->t.R, elems * sizeof(float)));
-  CHK(hipMalloc(&c->t.S, elems * sizeof(float)));
-  CHK(hipMalloc(&c->t.sig, elems * sizeof(uint16_t)));
-  CHK(hipMalloc(&c->t.D, elems * 2 * sizeof(double)));
-  CHK(hipMalloc(&c->t.stamp, (size_t)c->slots * sizeof(uint32_t)));
-  CHK(alloc_node_lists(c->t, max_batch));
-  CHK(hipMalloc(&c->st, sizeof(LaunchState)));
-  CHK(hipMalloc(&c->dg, sizeof(DevGame)));
+  CHK(alloc_scratch(c->sc, round_up(max_batch, kSegAlign)));
+  CHK(hipMalloc(&c->d_tenants, (size_t)kMaxTenants * sizeof(DevGame)));
+  CHK(hipMalloc(&c->d_segs, (size_t)kMaxSegs * sizeof(DevSeg)));
+  CHK(hipHostMalloc(&c->h_stage, (size_t)kStageRing * kMaxSegs * sizeof(DevSeg)));
+  for (auto& e : c->stage_ev) CHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   CHK(hipMalloc(&c->lut, 2704));
   CHK(hipMalloc(&c->buckets.flop5, kFlopSets));
   CHK(hipMalloc(&c->buckets.turn6, kTurnSets));
-  CHK(hipMemsetAsync(c->t.R, 0, elems * sizeof(float), c->stream));
-  CHK(hipMemsetAsync(c->t.S, 0, elems * sizeof(float), c->stream));
-  CHK(hipMemsetAsync(c->t.sig, 0, elems * sizeof(uint16_t), c->stream));
-  CHK(hipMemsetAsync(c->t.D, 0, elems * 2 * sizeof(double), c->stream));
-  CHK(hipMemsetAsync(c->t.stamp, 0, (size_t)c->slots * sizeof(uint32_t), c->stream));
-  CHK(hipMemsetAsync(c->st, 0, sizeof(LaunchState), c->stream));
-  CHK(hipMemcpyAsync(c->dg, &c->hg, sizeof(DevGame), hipMemcpyHostToDevice, c->stream));
   uint8_t lut[2704];
   build_preflop_lut(lut);
-  CHK(hipMemcpyAsync(c->lut, lut, sizeof lut, hipMemcpyHostToDevice, c->stream));
+  CHK(hipMemcpy(c->lut, lut, sizeof lut, hipMemcpyHostToDevice));
   std::vector<uint16_t> nf, fl, off;
   build_arith_tables(nf, fl, off);
   CHK(hipMalloc(&c->arith.nonflush, nf.size() * 2));
   CHK(hipMalloc(&c->arith.flush, fl.size() * 2));
   CHK(hipMalloc(&c->arith.offsets, off.size() * 2));
-  CHK(hipMemcpyAsync(c->arith.nonflush, nf.data(), nf.size() * 2, hipMemcpyHostToDevice, c->stream));
-  CHK(hipMemcpyAsync(c->arith.flush, fl.data(), fl.size() * 2, hipMemcpyHostToDevice, c->stream));
-  CHK(hipMemcpyAsync(c->arith.offsets, off.data(), off.size() * 2, hipMemcpyHostToDevice, c->stream));
-  CHK(hipStreamSynchronize(c->stream));
+  CHK(hipMemcpy(c->arith.nonflush, nf.data(), nf.size() * 2, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->arith.flush, fl.data(), fl.size() * 2, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->arith.offsets, off.data(), off.size() * 2, hipMemcpyHostToDevice));
 #undef CHK
+  uint32_t id = 0;
+  int rc = tenant_create(c, cfg, nullptr, nullptr, &id);  // tenant 0: the context's own game
+  if (rc == MCCFR_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MCCFR_E_HIP;
+  if (rc) return bail(rc);
   *out = c;
   return MCCFR_OK;
 }
@@ -353,7 +594,8 @@ int mccfr_set_hand_table(mccfr_ctx* c, const int32_t* hr, size_t words) {
     return fail(c, MCCFR_E_TABLE,
                 "incompatible hand table: expected 32,487,834 words with As Ks Qs Js Ts 2c 3d = 7462");
   if (!c->hr) HIPX(c, hipMalloc(&c->hr, words * sizeof(int32_t)));
-  HIPX(c, hipMemcpyAsync(c->hr, hr, words * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPX(c, hipStreamSynchronize(c->stream));
+  HIPX(c, hipMemcpy(c->hr, hr, words * sizeof(int32_t), hipMemcpyHostToDevice));
   launch_build_buckets(c->stream, c->hr, c->buckets);  // flop/turn buckets of every 5-/6-card set
   HIPX(c, hipGetLastError());
   HIPX(c, hipStreamSynchronize(c->stream));
@@ -433,40 +675,101 @@ int mccfr_random_hands_device(mccfr_ctx* c, uint64_t seed, uint32_t tag, uint8_t
   return MCCFR_OK;
 }
 
-static int record_event(mccfr_ctx* c, int k) {
-  if (!c->timing) return MCCFR_OK;
-  hipEvent_t e = c->ev[(size_t)(c->ev_next % kTimingRing) * 3 + k];
-  HIPX(c, hipEventRecord(e, c->stream));
-  return MCCFR_OK;
-}
-
 int mccfr_run_launch(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t first_tid, uint32_t n,
                      mccfr_trace* trace) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  if (!c->hr) return fail(c, MCCFR_E_STATE, "no hand table loaded");
-  if (update_player < 0 || update_player >= c->cfg.num_players) return fail(c, MCCFR_E_ARG, "bad update player");
   if (n == 0 || n > c->max_batch) return fail(c, MCCFR_E_ARG, "n must be 1..max_batch (see mccfr_set_max_batch)");
-  int rc = rm_sweep_if_dirty(c);
-  if (rc) return rc;
-  if (trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->max_batch * sizeof(mccfr_trace)));
-  launch_set_state(c->stream, c->st, update_player, c->next_iter, t_weight, first_tid, n);
-  if ((rc = record_event(c, 0))) return rc;
-  launch_traverse(c->stream, c->row, c->dg, c->st, c->buckets, c->arith, c->lut, c->t, n, c->walk_blocks,
-                  c->scatter_wg, trace ? c->dtrace : nullptr);
-  if ((rc = record_event(c, 1))) return rc;
-  launch_update(c->stream, c->row, c->dg, c->st, c->t, n);
-  if ((rc = record_event(c, 2))) return rc;
-  HIPX(c, hipGetLastError());
-  if (c->timing) {
-    ++c->ev_next;
-    c->ev_used = std::min(c->ev_used + 1, kTimingRing);
+  HostSeg s{};
+  s.tenant = 0;
+  s.n = n;
+  s.update_player = update_player;
+  s.t_weight = t_weight;
+  s.first_tid = first_tid;
+  s.iteration = c->tenants[0].next_iter;
+  s.upd_idx = 0;
+  return issue_launch(c, &s, 1, trace, nullptr);
+}
+
+int mccfr_run_segments(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, mccfr_trace* trace) {
+  if (!c || (!segs && n_segs)) return fail(c, MCCFR_E_ARG, "null argument");
+  std::vector<HostSeg> hs(n_segs);
+  for (uint32_t i = 0; i < n_segs; ++i) {
+    hs[i] = HostSeg{};
+    hs[i].tenant = segs[i].tenant;
+    hs[i].n = segs[i].n;
+    hs[i].update_player = segs[i].update_player;
+    hs[i].t_weight = segs[i].t_weight;
+    hs[i].first_tid = segs[i].first_tid;
+    const Tenant* t = get_tenant(c, segs[i].tenant);
+    hs[i].iteration = t ? t->next_iter : 1;
   }
-  c->traversals += n;
-  ++c->launches;
-  if (trace) {
-    HIPX(c, hipMemcpyAsync(trace, c->dtrace, (size_t)n * sizeof(mccfr_trace), hipMemcpyDeviceToHost, c->stream));
-    HIPX(c, hipStreamSynchronize(c->stream));
+  return issue_launch(c, hs.data(), n_segs, trace, nullptr);
+}
+
+int mccfr_train_tenants(mccfr_ctx* c, const uint32_t* tenants, const uint32_t* counts, uint32_t n, int64_t launches) {
+  if (!c || !tenants || !counts) return fail(c, MCCFR_E_ARG, "null argument");
+  if (launches <= 0) return MCCFR_OK;
+  std::vector<HostSeg> hs(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const Tenant* t = get_tenant(c, tenants[i]);
+    if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+    hs[i] = next_of(tenants[i], *t, counts[i]);
   }
+  int rc;
+  bool clamp = false;
+  for (uint32_t i = 0; i < n; ++i) clamp = clamp || c->tenants[tenants[i]].clamp_pending;
+  if (clamp) {  // the launch that follows an upload also clamps the whole table: issue it eagerly
+    if ((rc = issue_launch(c, hs.data(), n, nullptr, nullptr))) return rc;
+    for (uint32_t i = 0; i < n; ++i) advance(c->tenants[tenants[i]], hs[i]);
+    if (--launches == 0) goto done;
+  }
+  {
+    Prepared p;
+    if ((rc = prepare(c, hs.data(), n, p))) return rc;
+    for (uint32_t i = 0; i < n; ++i)
+      if ((rc = rm_sweep_if_dirty(c, c->tenants[tenants[i]]))) return rc;
+    std::vector<uint64_t> key = {(uint64_t)p.row,          (uint64_t)p.lc.lanes, (uint64_t)p.lc.n_segs,
+                                 (uint64_t)p.lc.slot_bits, (uint64_t)p.key_bits, (uint64_t)c->shape.wg,
+                                 (uint64_t)c->shape.tpl,   (uint64_t)c->sc.cap_lanes};
+    for (const DevSeg& d : p.segs) key.push_back(((uint64_t)d.tenant << 32) | d.n);
+    if ((rc = stage_segments(c, p.segs))) return rc;
+    if (!c->gexec || c->graph_key != key) {
+      drop_graph(c);
+      // one batched launch: [deal + walk -> records -> sort -> table update -> advance segments]
+      hipGraph_t g = nullptr;
+      HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape, nullptr);
+      launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+      launch_advance_segments(c->stream, c->d_segs, c->d_tenants, n);
+      HIPX(c, hipStreamEndCapture(c->stream, &g));
+      hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, MCCFR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      c->graph_key = key;
+    }
+    for (int64_t L = 0; L < launches; ++L) HIPX(c, hipGraphLaunch(c->gexec, c->stream));
+    uint64_t per = 0;
+    for (uint32_t i = 0; i < n; ++i) per += counts[i];
+    c->traversals += per * (uint64_t)launches;
+    c->launches += (uint64_t)launches;
+    for (int64_t L = 0; L < launches; ++L)
+      for (uint32_t i = 0; i < n; ++i) {
+        Tenant& t = c->tenants[tenants[i]];
+        t.traversals += counts[i];
+        ++t.launches;
+        advance(t, hs[i]);
+      }
+  }
+done:
+  for (uint32_t i = 0; i < n; ++i) {
+    Tenant& t = c->tenants[tenants[i]];
+    t.next_iter = hs[i].iteration;
+    t.upd_idx = hs[i].upd_idx;
+  }
+  HIPX(c, hipStreamSynchronize(c->stream));
   return MCCFR_OK;
 }
 
@@ -475,60 +778,30 @@ int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
   if (!c->hr) return fail(c, MCCFR_E_STATE, "no hand table loaded");
   if (iters <= 0) return MCCFR_OK;
   if (batch == 0 || batch > c->max_batch) return fail(c, MCCFR_E_ARG, "batch must be 1..max_batch");
-  int rc = rm_sweep_if_dirty(c);
-  if (rc) return rc;
-  if (c->gexec && c->graph_batch != batch) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
-  if (!c->gexec) {
-    // one player-update: [traverse -> touched-slot update -> advance launch state]
-    hipGraph_t g = nullptr;
-    HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_traverse(c->stream, c->row, c->dg, c->st, c->buckets, c->arith, c->lut, c->t, batch, c->walk_blocks,
-                    c->scatter_wg, nullptr);
-    launch_update(c->stream, c->row, c->dg, c->st, c->t, batch);
-    launch_advance_state(c->stream, c->st, c->cfg.num_players, c->cfg.use_linear_cfr, batch);
-    HIPX(c, hipStreamEndCapture(c->stream, &g));
-    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(c, MCCFR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    c->graph_batch = batch;
-  }
-  const int64_t it0 = c->next_iter;
-  launch_set_state(c->stream, c->st, 0, it0, c->cfg.use_linear_cfr ? it0 : 1, c->traversals, batch);
-  HIPX(c, hipGetLastError());
-  const int N = c->cfg.num_players;
-  for (int64_t it = 0; it < iters; ++it)
-    for (int p = 0; p < N; ++p) HIPX(c, hipGraphLaunch(c->gexec, c->stream));
-  c->traversals += (uint64_t)iters * N * batch;
-  c->launches += (uint64_t)iters * N;
-  c->next_iter += iters;
-  HIPX(c, hipStreamSynchronize(c->stream));
-  return MCCFR_OK;
+  Tenant& t = c->tenants[0];
+  t.upd_idx = 0;  // for iter = 1..iters, for p = 0..N-1 (cfr_gpu.cu:1286-1322)
+  const uint32_t id = 0;
+  return mccfr_train_tenants(c, &id, &batch, 1, iters * t.hg.n_upd);
 }
 
 int mccfr_set_max_batch(mccfr_ctx* c, uint32_t max_batch) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (max_batch == 0 || max_batch > (1u << 26)) return fail(c, MCCFR_E_ARG, "max_batch must be 1..2^26");
   HIPX(c, hipStreamSynchronize(c->stream));
-  if (max_batch != c->t.lanes) {
-    free_node_lists(c->t);
-    const hipError_t e = alloc_node_lists(c->t, max_batch);
+  drop_graph(c);  // the graph holds the old scratch pointers
+  const uint32_t cap = round_up(max_batch, kSegAlign);
+  if (cap != c->sc.cap_lanes) {
+    free_scratch(c->sc);
+    const hipError_t e = alloc_scratch(c->sc, cap);
     if (e != hipSuccess) {
-      free_node_lists(c->t);
-      c->t.lanes = 0;
+      free_scratch(c->sc);
       c->max_batch = 0;
-      return fail(c, MCCFR_E_NOMEM, std::string("node lists: ") + hipGetErrorString(e));
+      return fail(c, MCCFR_E_NOMEM, std::string("launch scratch: ") + hipGetErrorString(e));
     }
   }
   if (c->dtrace) {
     (void)hipFree(c->dtrace);
     c->dtrace = nullptr;
-  }
-  if (c->gexec) {  // the graph holds the old scratch pointers
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
   }
   c->max_batch = max_batch;
   return MCCFR_OK;
@@ -538,15 +811,16 @@ static constexpr uint64_t kCheckpointMagic = 0x43465247505500ull;  // cfr_gpu.cu
 
 int mccfr_save_checkpoint(mccfr_ctx* c, const char* path) {
   if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
-  if (c->slots > 0xFFFFFFFFull) return fail(c, MCCFR_E_ARG, "table too large for the .ckpt header");
-  const size_t n = (size_t)c->slots * c->row;
+  const Tenant& t = c->tenants[0];
+  if (t.slots > 0xFFFFFFFFull) return fail(c, MCCFR_E_ARG, "table too large for the .ckpt header");
+  const size_t n = t.elems();
   std::vector<float> R(n), S(n);
   int rc = mccfr_download_tables(c, R.data(), S.data(), nullptr, n, 1);
   if (rc) return rc;
   std::ofstream f(path, std::ios::binary);
   if (!f) return fail(c, MCCFR_E_IO, std::string("cannot open ") + path);
   const uint64_t magic = kCheckpointMagic;
-  const uint32_t ts = (uint32_t)c->slots, na = (uint32_t)c->row;
+  const uint32_t ts = (uint32_t)t.slots, na = (uint32_t)t.row;
   f.write(reinterpret_cast<const char*>(&magic), 8);
   f.write(reinterpret_cast<const char*>(&ts), 4);
   f.write(reinterpret_cast<const char*>(&na), 4);
@@ -558,6 +832,7 @@ int mccfr_save_checkpoint(mccfr_ctx* c, const char* path) {
 
 int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
   if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
+  const Tenant& t = c->tenants[0];
   std::ifstream f(path, std::ios::binary);
   if (!f) return fail(c, MCCFR_E_IO, std::string("cannot open ") + path);
   uint64_t magic = 0;
@@ -567,10 +842,10 @@ int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
   f.read(reinterpret_cast<char*>(&na), 4);
   if (!f) return fail(c, MCCFR_E_IO, std::string("cannot read header of ") + path);
   if (magic != kCheckpointMagic) return fail(c, MCCFR_E_FORMAT, std::string(path) + " is not a checkpoint");
-  if ((uint64_t)ts != c->slots || na != (uint32_t)c->row)
+  if ((uint64_t)ts != t.slots || na != (uint32_t)t.row)
     return fail(c, MCCFR_E_FORMAT, std::string(path) + " was saved with table_size=" + std::to_string(ts) +
                                        " num_actions=" + std::to_string(na) + ", this context has " +
-                                       std::to_string(c->slots) + " and " + std::to_string(c->row));
+                                       std::to_string(t.slots) + " and " + std::to_string(t.row));
   const size_t n = (size_t)ts * na;
   std::vector<float> R(n), S(n);
   f.read(reinterpret_cast<char*>(R.data()), (std::streamsize)(n * sizeof(float)));
@@ -579,39 +854,90 @@ int mccfr_load_checkpoint(mccfr_ctx* c, const char* path) {
   return mccfr_upload_tables(c, R.data(), S.data(), n, 1);
 }
 
-int mccfr_set_walk_blocks(mccfr_ctx* c, uint32_t blocks) {
+int mccfr_set_launch_shape(mccfr_ctx* c, uint32_t walk_wg, uint32_t tpl) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  c->walk_blocks = blocks;
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+  if (walk_wg != 64 && walk_wg != 128 && walk_wg != 256 && walk_wg != 512 && walk_wg != 1024)
+    return fail(c, MCCFR_E_ARG, "walk workgroup must be 64, 128, 256, 512 or 1024");
+  if (tpl != 1 && tpl != 2 && tpl != 4) return fail(c, MCCFR_E_ARG, "traversals per lane must be 1, 2 or 4");
+  if (c->shape.wg != walk_wg || c->shape.tpl != tpl) drop_graph(c);
+  c->shape.wg = walk_wg;
+  c->shape.tpl = tpl;
+  return MCCFR_OK;
+}
+
+int mccfr_get_launch_shape(const mccfr_ctx* c, uint32_t* walk_wg, uint32_t* tpl) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  if (walk_wg) *walk_wg = c->shape.wg;
+  if (tpl) *tpl = c->shape.tpl;
+  return MCCFR_OK;
+}
+
+int mccfr_autotune(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, int reps, uint32_t* walk_wg,
+                   uint32_t* tpl, float* ms_out) {
+  if (!c || !segs || reps < 1) return fail(c, MCCFR_E_ARG, "bad autotune arguments");
+  std::vector<HostSeg> hs(n_segs);
+  for (uint32_t i = 0; i < n_segs; ++i) {
+    hs[i] = HostSeg{};
+    hs[i].tenant = segs[i].tenant;
+    hs[i].n = segs[i].n;
+    hs[i].update_player = segs[i].update_player;
+    hs[i].t_weight = segs[i].t_weight;
+    hs[i].first_tid = segs[i].first_tid;
   }
+  Prepared p;
+  int rc = prepare(c, hs.data(), n_segs, p);
+  if (rc) return rc;
+  for (uint32_t i = 0; i < n_segs; ++i)
+    if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;
+  if ((rc = stage_segments(c, p.segs))) return rc;
+  hipEvent_t e0, e1;
+  HIPX(c, hipEventCreate(&e0));
+  HIPX(c, hipEventCreate(&e1));
+  const WalkShape keep = c->shape;
+  WalkShape best = keep;
+  float best_ms = 1e30f;
+  const uint32_t wgs[5] = {64, 128, 256, 512, 1024};
+  const uint32_t tpls[3] = {1, 2, 4};
+  hipError_t err = hipSuccess;
+  for (uint32_t wg : wgs)
+    for (uint32_t tp : tpls) {
+      WalkShape sh;
+      sh.wg = wg;
+      sh.tpl = tp;
+      launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, sh, nullptr);  // warm-up
+      if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps; ++r)
+        launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, sh, nullptr);
+      if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)reps;
+      if (err == hipSuccess && ms < best_ms) {
+        best_ms = ms;
+        best = sh;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) return fail(c, MCCFR_E_HIP, std::string("autotune: ") + hipGetErrorString(err));
+  if (best.wg != keep.wg || best.tpl != keep.tpl) drop_graph(c);
+  c->shape = best;
+  if (walk_wg) *walk_wg = best.wg;
+  if (tpl) *tpl = best.tpl;
+  if (ms_out) *ms_out = best_ms;
   return MCCFR_OK;
 }
 
-int mccfr_set_scatter_wg(mccfr_ctx* c, uint32_t wg) {
-  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  if (wg != 256 && wg != 512 && wg != 1024) return fail(c, MCCFR_E_ARG, "scatter workgroup must be 256, 512 or 1024");
-  c->scatter_wg = wg;
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
-  return MCCFR_OK;
-}
+int mccfr_randomize_tables(mccfr_ctx* c, uint64_t seed) { return mccfr_tenant_randomize(c, 0, seed); }
 
-int mccfr_get_walk_blocks(const mccfr_ctx* c, uint32_t* blocks) {
-  if (!c || !blocks) return fail(nullptr, MCCFR_E_ARG, "null argument");
-  *blocks = c->walk_blocks;
-  return MCCFR_OK;
-}
-
-int mccfr_randomize_tables(mccfr_ctx* c, uint64_t seed) {
-  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  launch_randomize(c->stream, c->row, c->cfg.num_actions, c->t, c->slots, seed);
+int mccfr_tenant_randomize(mccfr_ctx* c, uint32_t id, uint64_t seed) {
+  Tenant* t = get_tenant(c, id);
+  if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+  launch_randomize(c->stream, t->row, t->cfg.num_actions, t->hg.R, t->hg.S, t->slots, seed);
   HIPX(c, hipGetLastError());
-  c->sigma_dirty = true;
-  return rm_sweep_if_dirty(c);
+  t->sigma_dirty = true;
+  return rm_sweep_if_dirty(c, *t);
 }
 
 int mccfr_sync(mccfr_ctx* c) {
@@ -621,18 +947,28 @@ int mccfr_sync(mccfr_ctx* c) {
 }
 
 int mccfr_table_shape(const mccfr_ctx* c, uint64_t* slots, uint32_t* row) {
-  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  if (slots) *slots = c->slots;
-  if (row) *row = (uint32_t)c->row;
+  return mccfr_tenant_table_shape(c, 0, slots, row);
+}
+
+int mccfr_tenant_table_shape(const mccfr_ctx* c, uint32_t id, uint64_t* slots, uint32_t* row) {
+  if (!c || id >= c->tenants.size() || !c->tenants[id].used) return fail(nullptr, MCCFR_E_ARG, "no such tenant");
+  const Tenant& t = c->tenants[id];
+  if (slots) *slots = t.slots;
+  if (row) *row = (uint32_t)t.row;
   return MCCFR_OK;
 }
 
 int mccfr_download_tables(mccfr_ctx* c, float* R, float* S, uint16_t* sig, size_t elems, int layout) {
-  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  const size_t want = (size_t)c->slots * c->row;
+  return mccfr_tenant_download(c, 0, R, S, sig, elems, layout);
+}
+
+int mccfr_tenant_download(mccfr_ctx* c, uint32_t id, float* R, float* S, uint16_t* sig, size_t elems, int layout) {
+  Tenant* t = get_tenant(c, id);
+  if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+  const size_t want = t->elems();
   if (elems != want) return fail(c, MCCFR_E_ARG, "elems must equal slots * row_actions");
   if (layout != 0 && layout != 1) return fail(c, MCCFR_E_ARG, "layout must be 0 (rows) or 1 (planes)");
-  int rc = rm_sweep_if_dirty(c);
+  int rc = rm_sweep_if_dirty(c, *t);
   if (rc) return rc;
   HIPX(c, hipStreamSynchronize(c->stream));
   auto get = [&](auto* host, const auto* dev) -> int {
@@ -643,19 +979,24 @@ int mccfr_download_tables(mccfr_ctx* c, float* R, float* S, uint16_t* sig, size_
     } else {
       std::vector<T> tmp(want);
       HIPX(c, hipMemcpy(tmp.data(), dev, want * sizeof(T), hipMemcpyDeviceToHost));
-      to_soa(tmp.data(), host, c->slots, c->row);
+      to_soa(tmp.data(), host, t->slots, t->row);
     }
     return MCCFR_OK;
   };
-  if ((rc = get(R, c->t.R))) return rc;
-  if ((rc = get(S, c->t.S))) return rc;
-  if ((rc = get(sig, c->t.sig))) return rc;
+  if ((rc = get(R, t->hg.R))) return rc;
+  if ((rc = get(S, t->hg.S))) return rc;
+  if ((rc = get(sig, t->hg.sig))) return rc;
   return MCCFR_OK;
 }
 
 int mccfr_upload_tables(mccfr_ctx* c, const float* R, const float* S, size_t elems, int layout) {
-  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
-  const size_t want = (size_t)c->slots * c->row;
+  return mccfr_tenant_upload(c, 0, R, S, elems, layout);
+}
+
+int mccfr_tenant_upload(mccfr_ctx* c, uint32_t id, const float* R, const float* S, size_t elems, int layout) {
+  Tenant* t = get_tenant(c, id);
+  if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+  const size_t want = t->elems();
   if (elems != want) return fail(c, MCCFR_E_ARG, "elems must equal slots * row_actions");
   if (layout != 0 && layout != 1) return fail(c, MCCFR_E_ARG, "layout must be 0 (rows) or 1 (planes)");
   HIPX(c, hipStreamSynchronize(c->stream));
@@ -665,32 +1006,58 @@ int mccfr_upload_tables(mccfr_ctx* c, const float* R, const float* S, size_t ele
       HIPX(c, hipMemcpy(dev, host, want * sizeof(float), hipMemcpyHostToDevice));
     } else {
       std::vector<float> tmp(want);
-      to_aos(host, tmp.data(), c->slots, c->row);
+      to_aos(host, tmp.data(), t->slots, t->row);
       HIPX(c, hipMemcpy(dev, tmp.data(), want * sizeof(float), hipMemcpyHostToDevice));
     }
     return MCCFR_OK;
   };
   int rc;
-  if ((rc = put(R, c->t.R))) return rc;
-  if ((rc = put(S, c->t.S))) return rc;
-  c->sigma_dirty = true;  // the next launch regret-matches the whole table first (cfr_gpu.cu:1290)
+  if ((rc = put(R, t->hg.R))) return rc;
+  if ((rc = put(S, t->hg.S))) return rc;
+  t->sigma_dirty = true;  // the next launch regret-matches the whole table first (cfr_gpu.cu:1290)
+  if (R && t->cfg.use_cfr_plus) t->clamp_pending = true;
   return MCCFR_OK;
 }
 
 int mccfr_normalized_strategy(mccfr_ctx* c, float* out, size_t elems, int layout) {
-  if (!c || !out) return fail(c, MCCFR_E_ARG, "null argument");
-  const size_t want = (size_t)c->slots * c->row;
-  if (elems != want) return fail(c, MCCFR_E_ARG, "elems must equal slots * row_actions");
-  if (layout != 0 && layout != 1) return fail(c, MCCFR_E_ARG, "layout must be 0 (rows) or 1 (planes)");
-  float* d = nullptr;
-  HIPX(c, hipMalloc(&d, want * sizeof(float)));
-  launch_normalize(c->stream, c->row, c->t.S, d, c->slots);
-  std::vector<float> tmp(layout ? want : 0);
-  hipError_t e = hipMemcpyAsync(layout ? tmp.data() : out, d, want * sizeof(float), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(d);
-  if (e != hipSuccess) return fail(c, MCCFR_E_HIP, hipGetErrorString(e));
-  if (layout) to_soa(tmp.data(), out, c->slots, c->row);
+  return tenant_normalized(c, 0, out, elems, layout);
+}
+
+int mccfr_tenant_normalized_strategy(mccfr_ctx* c, uint32_t id, float* out, size_t elems, int layout) {
+  return tenant_normalized(c, id, out, elems, layout);
+}
+
+int mccfr_tenant_footprint(const mccfr_game_cfg* cfg, uint64_t* bytes) {
+  if (!cfg || !bytes) return fail(nullptr, MCCFR_E_ARG, "null argument");
+  DevGame g;
+  const char* why = nullptr;
+  if (!make_dev_game(*cfg, nullptr, g, &why)) return fail(nullptr, MCCFR_E_ARG, why);
+  *bytes = tenant_bytes(*cfg);
+  return MCCFR_OK;
+}
+
+int mccfr_tenant_create(mccfr_ctx* c, const mccfr_game_cfg* cfg, const mccfr_root_state* root, uint32_t* tenant) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  int rc = tenant_create(c, cfg, root, nullptr, tenant);
+  if (rc) return rc;
+  HIPX(c, hipStreamSynchronize(c->stream));
+  return MCCFR_OK;
+}
+
+int mccfr_tenant_destroy(mccfr_ctx* c, uint32_t tenant) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  if (tenant == 0) return fail(c, MCCFR_E_ARG, "tenant 0 is the context's own game");
+  return tenant_destroy(c, tenant);
+}
+
+int mccfr_tenant_counters(const mccfr_ctx* c, uint32_t id, uint64_t* traversals, uint64_t* launches,
+                          int64_t* next_iteration, int32_t* next_player) {
+  if (!c || id >= c->tenants.size() || !c->tenants[id].used) return fail(nullptr, MCCFR_E_ARG, "no such tenant");
+  const Tenant& t = c->tenants[id];
+  if (traversals) *traversals = t.traversals;
+  if (launches) *launches = t.launches;
+  if (next_iteration) *next_iteration = t.next_iter;
+  if (next_player) *next_player = t.hg.upd_seats[t.upd_idx];
   return MCCFR_OK;
 }
 
@@ -698,7 +1065,7 @@ int mccfr_counters(const mccfr_ctx* c, uint64_t* traversals, uint64_t* launches,
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (traversals) *traversals = c->traversals;
   if (launches) *launches = c->launches;
-  if (next_iteration) *next_iteration = c->next_iter;
+  if (next_iteration) *next_iteration = c->tenants[0].next_iter;
   return MCCFR_OK;
 }
 
@@ -742,7 +1109,7 @@ int mccfr_copy_probe(mccfr_ctx* c, size_t bytes, int reps, float* ms_per_copy) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  (void)hipMemsetAsync(a, 1, bytes, c->stream);
+  launch_zero(c->stream, a, bytes);
   launch_copy(c->stream, (const uint4*)a, (uint4*)b, bytes / 16);  // warm-up
   (void)hipEventRecord(e0, c->stream);
   for (int r = 0; r < reps; ++r) launch_copy(c->stream, (const uint4*)a, (uint4*)b, bytes / 16);

@@ -1,23 +1,33 @@
-// The traversal pipeline of one launch (the reference's simulate kernel,
-// cfr_gpu.cu:878-1011, and its clamp / regret-matching sweeps, :1018-1022 and
-// :318-351), as four gfx950 kernels:
+// The launch pipeline: the reference's simulate kernel (cfr_gpu.cu:878-1011)
+// with its regret-matching and clamp sweeps (:318-351, :1018-1022), for one or
+// more tenants at once.  The launch's lane space is a list of segments (one
+// tenant each, aligned to kSegAlign lanes, so every wave serves one tenant).
 //
-//   k_deal      one lane per traversal: Philox deal, hole and board buckets
-//               (precomputed bucket tables), showdown ranks, dealer -> a
-//               compact root record in HBM.
-//   k_walk      persistent lanes walk traversal after traversal from the root
-//               records: both node types follow exactly one child
-//               (cfr_gpu.cu:800, :844), so a traversal is one root-to-leaf
-//               path; lanes keep their game state in LDS and record only the
-//               update-player nodes (slot, sampled action, legal mask) and the
-//               terminal utility.
-//   k_scatter   every recorded node's regret and strategy-sum contributions
-//               (cfr_gpu.cu:784-837) added into FP64 per-launch deltas.
-//   k_update    each touched slot once: regrets += deltas (CFR+ clamp),
-//               strategy sums += deltas, BF16 snapshot re-matched, deltas
-//               cleared.  Untouched slots need no sweep: clamp and regret
-//               matching are per-slot and idempotent on them.
+//   k_deal        one lane per traversal: Philox deal (around the tenant's
+//                 fixed board), hole and board buckets, showdown ranks,
+//                 dealer -> compact root records.
+//   k_walk        each traversal's root-to-leaf path: both node types follow
+//                 exactly one child (cfr_gpu.cu:800, :844), so a traversal is
+//                 one path.  Lanes keep the game state in LDS and record the
+//                 update-player nodes (key, sampled action, legal mask) and
+//                 the terminal utility.
+//   k_emit_*      records in lane order -> (key, reference) arrays.
+//   k_rs_*        stable LSD radix sort of the records by key
+//                 (key = segment << slot_bits | slot), 8-bit digits.
+//   k_reduce      every run of equal keys is one table slot of one tenant:
+//                 its owner sums the run's contributions (cfr_gpu.cu:784-837)
+//                 in FP64 in sorted order and applies them: regrets += sum
+//                 (CFR+ clamp), strategy sums += sum, BF16 snapshot
+//                 re-matched.  Runs longer than kShortRun are cut into
+//                 kPiece-record pieces (k_reduce_piece, one workgroup each)
+//                 whose sums k_reduce_runs adds in piece order and applies.
+// No atomics touch the tables, and every sum has a fixed order, so a tenant's
+// result does not depend on which other tenants share its launches.
+// Untouched slots need no sweep: clamp and regret matching are per-slot and
+// idempotent on them.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernel_util.hpp"
 #include "kernels.hpp"
@@ -26,7 +36,19 @@ namespace mccfr {
 namespace {
 
 constexpr int kDealWG = 256;
-constexpr int kWalkWG = 256;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Segment holding `lane` (lane0 ascending; the caller passes a wave-uniform lane).
+__device__ __forceinline__ uint32_t find_seg(const DevSeg* __restrict__ segs, uint32_t n_segs, uint32_t lane) {
+  uint32_t lo = 0, hi = n_segs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].lane0 <= lane) lo = mid;
+    else hi = mid;
+  }
+  return uniform(lo);
+}
 
 // ---------------------------------------------------------------------------
 // k_deal.  Buckets come from precomputed tables instead of the reference's
@@ -35,55 +57,80 @@ constexpr int kWalkWG = 256;
 // once, indexed by colex rank (kernels.hip k_build_buckets); the river rank and
 // bucket come from the arithmetic evaluator.  Per player: three independent
 // loads instead of a chain of dependent gathers into the 130 MB table.
+// Preflop root (cfr_gpu.cu:931-937, 963): 2N+5 draws over the full deck, then
+// the dealer draw.  Postflop root: the fixed board cards leave the deck, the
+// 2N hole cards and the 5 - n_board missing board cards are drawn from the
+// remaining deck_size cards, and the dealer is the root's.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kDealWG) void k_deal(const DevGame* __restrict__ gp, const LaunchState* __restrict__ st,
-                                                  BucketTables bt, ArithTables at, const uint8_t* __restrict__ lut_g,
-                                                  DevTables T, uint32_t n_lanes) {
+__global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                  const DevGame* __restrict__ tenants, DealInputs in,
+                                                  DevScratch sc) {
   __shared__ uint32_t s_lut[2704 / 4];
   __shared__ uint16_t s_off[13 * 8 * 5];
   __shared__ uint8_t s_deck[kDeckStride * kDealWG];
-  const DevGame& G = *gp;
-  for (int i = threadIdx.x; i < 2704 / 4; i += kDealWG) s_lut[i] = reinterpret_cast<const uint32_t*>(lut_g)[i];
-  for (int i = threadIdx.x; i < 13 * 8 * 5; i += kDealWG) s_off[i] = at.offsets[i];
+  for (int i = threadIdx.x; i < 2704 / 4; i += kDealWG) s_lut[i] = reinterpret_cast<const uint32_t*>(in.lut)[i];
+  for (int i = threadIdx.x; i < 13 * 8 * 5; i += kDealWG) s_off[i] = in.at.offsets[i];
   __syncthreads();
-  uint32_t n = (uint32_t)st->n;
-  if (n > n_lanes) n = n_lanes;
   const uint32_t t = blockIdx.x * kDealWG + threadIdx.x;
-  if (t >= n) return;
+  if (t >= lc.lanes) return;
+  const uint32_t si = find_seg(segs, lc.n_segs, uniform(t & ~63u));
+  const DevSeg& S = segs[si];
+  const uint32_t i = t - S.lane0;
+  if (i >= S.n) {  // padding lane of the segment's last wave
+    sc.node_count[t] = 0;
+    return;
+  }
+  const DevGame& G = tenants[uniform(S.tenant)];
   const int N = G.N;
+  const int nb = G.n_board;
 
-  // partial Fisher-Yates over 2N+5 draws (cfr_gpu.cu:460-470, 931-937)
   DrawStream rng;
-  rng.init(G.seed_lo, G.seed_hi, st->first_tid + t, G.tag);
+  rng.init(G.seed_lo, G.seed_hi, S.first_tid + i, G.tag);
   uint8_t* dk = s_deck + threadIdx.x * kDeckStride;
-  for (int i = 0; i < 52; ++i) dk[i] = (uint8_t)i;
-  const int n_deal = 2 * N + 5;
-  for (int i = 0; i < n_deal; ++i) {
+  if (nb == 0) {
+    for (int k = 0; k < 52; ++k) dk[k] = (uint8_t)k;
+  } else {
+    const uint64_t bm = ((uint64_t)G.board_hi << 32) | G.board_lo;
+    int j = 0;
+    for (int k = 0; k < 52; ++k)
+      if (!(bm >> k & 1ull)) dk[j++] = (uint8_t)k;
+  }
+  const uint32_t M = G.deck_size;
+  const int n_deal = G.n_deal;
+  for (int k = 0; k < n_deal; ++k) {  // partial Fisher-Yates (cfr_gpu.cu:460-470)
     const uint32_t r = rng.next();
-    const int j = i + (int)(((uint64_t)r * (uint64_t)(52 - i)) >> 32);
-    const uint8_t a = dk[i], b = dk[j];
-    dk[i] = b;
+    const int j = k + (int)(((uint64_t)r * (uint64_t)(M - (uint32_t)k)) >> 32);
+    const uint8_t a = dk[k], b = dk[j];
+    dk[k] = b;
     dk[j] = a;
   }
-  const uint32_t f0 = dk[2 * N], f1 = dk[2 * N + 1], f2 = dk[2 * N + 2], c3 = dk[2 * N + 3], c4 = dk[2 * N + 4];
+  uint32_t bc[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    int idx = 2 * N + k - nb;
+    idx = idx < 0 ? 0 : idx;
+    bc[k] = k < nb ? (uint32_t)G.board[k] : (uint32_t)dk[idx];
+  }
   const uint8_t* lut = reinterpret_cast<const uint8_t*>(s_lut);
+  const size_t L = sc.cap_lanes;
 #pragma unroll
   for (int pl = 0; pl < kMaxPlayers; ++pl) {
     if (pl < N) {
       const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
       const uint32_t hb = lut[h0 * 52 + h1];
-      const uint32_t bf = bt.flop5[colex5(h0, h1, f0, f1, f2)];
-      const uint32_t btn = bt.turn6[colex6(h0, h1, f0, f1, f2, c3)];
-      const uint32_t c7[7] = {h0, h1, f0, f1, f2, c3, c4};
-      const uint32_t r7 = arith_rank7(c7, at.nonflush, at.flush, s_off);
-      T.root_bkt[(size_t)pl * T.lanes + t] = hb | (bf << 8) | (btn << 16) | (bucket_of_rank(r7) << 24);
-      T.root_rank[(size_t)pl * T.lanes + t] = (uint16_t)r7;
+      const uint32_t bf = in.bt.flop5[colex5(h0, h1, bc[0], bc[1], bc[2])];
+      const uint32_t btn = in.bt.turn6[colex6(h0, h1, bc[0], bc[1], bc[2], bc[3])];
+      const uint32_t c7[7] = {h0, h1, bc[0], bc[1], bc[2], bc[3], bc[4]};
+      const uint32_t r7 = arith_rank7(c7, in.at.nonflush, in.at.flush, s_off);
+      sc.root_bkt[(size_t)pl * L + t] = hb | (bf << 8) | (btn << 16) | (bucket_of_rank(r7) << 24);
+      sc.root_rank[(size_t)pl * L + t] = (uint16_t)r7;
     }
   }
-  T.root_dealer[t] = (uint8_t)(rng.next() % (uint32_t)N);  // draw 2N+5 (cfr_gpu.cu:963)
+  // draw 2N+5 of the preflop root (cfr_gpu.cu:963)
+  sc.root_dealer[t] = G.root_street == 0 ? (uint8_t)(rng.next() % (uint32_t)N) : (uint8_t)G.root_dealer;
 }
 
-// chips_for over the LDS copy of the menu (divergent action index).
+// chips_for over the wave's LDS copy of the menu (divergent action index).
 __device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, const uint8_t* num,
                                          const uint8_t* den, int a, int pot, int stack, int to_call,
                                          int current_bet, int lfr) {
@@ -108,89 +155,114 @@ __device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, 
 }
 
 // ---------------------------------------------------------------------------
-// k_walk
+// k_walk.  Wave w covers lanes [w*64*tpl, (w+1)*64*tpl); lane l walks the
+// traversals at lanes w*64*tpl + j*64 + l, j < tpl, one after another.
 // ---------------------------------------------------------------------------
-template <int ROW>
-__global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp, const LaunchState* __restrict__ st,
-                                                  DevTables T, uint32_t n_lanes, mccfr_trace* __restrict__ trace) {
+template <int ROW, int WG>
+__global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                             const DevGame* __restrict__ tenants, DevScratch sc, uint32_t tpl,
+                                             mccfr_trace* __restrict__ trace) {
   // Per-lane state, field-major ([field][lane]): any per-lane field index
   // maps consecutive lanes to consecutive banks.
-  __shared__ uint32_t s_sb[kMaxPlayers * kWalkWG];    // stack | bet << 16
-  __shared__ uint32_t s_bk[kMaxPlayers * kWalkWG];    // buckets: hole | flop << 8 | turn << 16 | river << 24
-  __shared__ uint16_t s_rk[kMaxPlayers * kWalkWG];    // 7-card rank at showdown
-  __shared__ uint32_t s_ps[kRegPending * kWalkWG];    // pending update nodes: slot
-  __shared__ uint32_t s_pa[kRegPending * kWalkWG];    //                       action | legal mask << 8
-  __shared__ uint8_t s_kind[MCCFR_MAX_ACTIONS], s_num[MCCFR_MAX_ACTIONS], s_den[MCCFR_MAX_ACTIONS];
-  const DevGame& G = *gp;
-  if (threadIdx.x < MCCFR_MAX_ACTIONS) {
-    s_kind[threadIdx.x] = G.kind[threadIdx.x];
-    s_num[threadIdx.x] = G.num[threadIdx.x];
-    s_den[threadIdx.x] = G.den[threadIdx.x];
-  }
-  __syncthreads();
+  __shared__ uint32_t s_sb[kMaxPlayers * WG];    // stack | bet << 16
+  __shared__ uint32_t s_bk[kMaxPlayers * WG];    // buckets: hole | flop << 8 | turn << 16 | river << 24
+  __shared__ uint16_t s_rk[kMaxPlayers * WG];    // 7-card rank at showdown
+  __shared__ uint32_t s_ps[kRegPending * WG];    // pending update nodes: key
+  __shared__ uint32_t s_pa[kRegPending * WG];    //                       action | legal mask << 8
+  __shared__ uint8_t s_menu[WG / 64][3][MCCFR_MAX_ACTIONS];  // kind, num, den of the wave's tenant
 
-  const int tx = threadIdx.x;
+  const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+  const uint32_t base = uniform((uint32_t)((blockIdx.x * WG + tx) >> 6) * 64u * tpl);
+  if (base >= lc.lanes) return;
+  const uint32_t si = find_seg(segs, lc.n_segs, base);
+  const DevSeg& Sg = segs[si];
+  const DevGame& G = tenants[uniform(Sg.tenant)];
+  uint8_t* m_kind = s_menu[wv][0];
+  uint8_t* m_num = s_menu[wv][1];
+  uint8_t* m_den = s_menu[wv][2];
+  if (lane < MCCFR_MAX_ACTIONS) {
+    m_kind[lane] = G.kind[lane];
+    m_num[lane] = G.num[lane];
+    m_den[lane] = G.den[lane];
+  }
+  __builtin_amdgcn_wave_barrier();
+
   uint32_t* sb = s_sb + tx;
   uint32_t* bk = s_bk + tx;
   uint16_t* rk = s_rk + tx;
   uint32_t* ps = s_ps + tx;
   uint32_t* pa = s_pa + tx;
 
-  uint32_t n = (uint32_t)st->n;
-  if (n > n_lanes) n = n_lanes;
-  const uint32_t stride = gridDim.x * kWalkWG;
-  const int up = st->update_player;
-  const uint64_t tid0 = st->first_tid;
+  const uint32_t seg_key = uniform((uint32_t)((uint64_t)si << lc.slot_bits));
+  const uint32_t lane0 = Sg.lane0, seg_n = Sg.n;
+  const int up = Sg.update_player;
+  const uint64_t tid0 = Sg.first_tid;
   const int N = G.N;
   const int A = G.num_actions;
   const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
   const uint32_t allmask = (1u << N) - 1u;
-  const size_t L = T.lanes;
+  const size_t L = sc.cap_lanes;
+  const uint16_t* __restrict__ sig = G.sig;
 
-  uint32_t t = blockIdx.x * kWalkWG + tx;
-  bool live = t < n;
+  uint32_t j = 0;
+  uint32_t t = base + (uint32_t)lane;
+  bool live = t - lane0 < seg_n;
 
   DrawStream rng;
   uint32_t folded = 0, all_in = 0, ab = 0, queue = 0, phash = 0;
   int pot = 0, current_bet = 0, lfr = 0, street = 0, raises = 0, depth = 0, nq = 0, dealer = 0;
   int npend = 0, decisions = 0, upd_legal = 0;
 
-  // Root state of traversal t (cfr_gpu.cu:954-981): stacks, blinds, preflop queue.
+  // Root state of traversal t: the reference's preflop root (cfr_gpu.cu:954-981:
+  // stacks, blinds, preflop queue from the seat after the big blind) or the
+  // tenant's postflop root (queue from the seat after the dealer, as at a
+  // street change, cfr_gpu.cu:676-686).
   auto init = [&]() {
 #pragma unroll
     for (int p = 0; p < kMaxPlayers; ++p) {
       if (p < N) {
-        sb[p * kWalkWG] = (uint32_t)G.stack[p];
-        bk[p * kWalkWG] = T.root_bkt[(size_t)p * L + t];
-        rk[p * kWalkWG] = T.root_rank[(size_t)p * L + t];
+        sb[p * WG] = (uint32_t)G.root_stack[p];
+        bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
+        rk[p * WG] = sc.root_rank[(size_t)p * L + t];
       }
     }
-    dealer = T.root_dealer[t];
-    const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
-    const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
-    folded = 0;
-    all_in = 0;
-    const int s_sb = (int)(sb[sb_pos * kWalkWG] & 0xFFFFu);
-    const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
-    sb[sb_pos * kWalkWG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 16);
-    if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
-    const int s_bb = (int)(sb[bb_pos * kWalkWG] & 0xFFFFu);
-    const int bb_amt = G.bb < s_bb ? G.bb : s_bb;
-    sb[bb_pos * kWalkWG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 16);
-    if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
-    pot = sb_amt + bb_amt;
-    current_bet = bb_amt;
-    lfr = G.bb;
-    street = 0;
-    raises = 0;
-    ab = 0;
-    depth = 0;
+    dealer = sc.root_dealer[t];
     queue = 0;
     nq = 0;
-    for (int i = 0, p = wrap(bb_pos + 1, N); i < N; ++i, p = wrap(p + 1, N))
-      if (!(all_in >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
-    rng.init(G.seed_lo, G.seed_hi, tid0 + t, G.tag);
-    rng.k = (uint32_t)(2 * N + 6);  // the deal and the dealer draw came first
+    if (G.root_street == 0) {
+      const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
+      const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
+      folded = 0;
+      all_in = 0;
+      const int s_sb = (int)(sb[sb_pos * WG] & 0xFFFFu);
+      const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
+      sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 16);
+      if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
+      const int s_bb = (int)(sb[bb_pos * WG] & 0xFFFFu);
+      const int bb_amt = G.bb < s_bb ? G.bb : s_bb;
+      sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 16);
+      if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
+      pot = sb_amt + bb_amt;
+      current_bet = bb_amt;
+      street = 0;
+      ab = 0;
+      for (int k = 0, p = wrap(bb_pos + 1, N); k < N; ++k, p = wrap(p + 1, N))
+        if (!(all_in >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
+    } else {
+      folded = G.root_folded;
+      all_in = G.root_allin;
+      pot = G.root_pot;
+      current_bet = 0;
+      street = G.root_street;
+      ab = G.root_ab;
+      for (int k = 0, p = wrap(dealer + 1, N); k < N; ++k, p = wrap(p + 1, N))
+        if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
+    }
+    lfr = G.bb;
+    raises = 0;
+    depth = 0;
+    rng.init(G.seed_lo, G.seed_hi, tid0 + (t - lane0), G.tag);
+    rng.k = G.walk_k0;  // the deal (and the preflop dealer draw) came first
     npend = 0;
     decisions = 0;
     upd_legal = 0;
@@ -209,15 +281,16 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
       r.reserved = 0;
       trace[t] = r;
     }
-    T.node_count[t] = (uint8_t)npend;
-    T.node_u[t] = u;
+    sc.node_count[t] = (uint8_t)npend;
+    sc.node_u[t] = u;
     const int kr = npend < kRegPending ? npend : kRegPending;
     for (int k = 0; k < kr; ++k) {
-      T.node_slot[(size_t)k * L + t] = ps[k * kWalkWG];
-      T.node_av[(size_t)k * L + t] = pa[k * kWalkWG];
+      sc.node_slot[(size_t)k * L + t] = ps[k * WG];
+      sc.node_av[(size_t)k * L + t] = pa[k * WG];
     }
-    t += stride;
-    live = t < n;
+    ++j;
+    t += 64;
+    live = j < tpl && t - lane0 < seg_n;
     if (live) init();
   };
 
@@ -231,18 +304,18 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
     }
     const uint32_t alive = allmask & ~folded;
     if (__popc(alive) == 1) {
-      const int inv_up = G.stack[up] - (int)(sb[up * kWalkWG] & 0xFFFFu);
+      const int inv_up = G.stack[up] - (int)(sb[up * WG] & 0xFFFFu);
       finish(((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up), 1);
       continue;
     }
     if (nq == 0) {
       if (street < 3) {
         ++street;
-        for (int p = 0; p < N; ++p) sb[p * kWalkWG] &= 0xFFFFu;
+        for (int p = 0; p < N; ++p) sb[p * WG] &= 0xFFFFu;
         current_bet = 0;
         lfr = G.bb;
         raises = 0;
-        for (int i = 0, p = wrap(dealer + 1, N); i < N; ++i, p = wrap(p + 1, N))
+        for (int k = 0, p = wrap(dealer + 1, N); k < N; ++k, p = wrap(p + 1, N))
           if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
         if (nq > 0) {
           ++depth;
@@ -250,31 +323,31 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
         }
       }
       // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending order
-      const int inv_up = G.stack[up] - (int)(sb[up * kWalkWG] & 0xFFFFu);
+      const int inv_up = G.stack[up] - (int)(sb[up * WG] & 0xFFFFu);
       float pay = -(float)inv_up;
       int prev = 0;
       for (;;) {
         int lvl = 0x7FFFFFFF;
         for (int p = 0; p < N; ++p) {
-          const int inv = G.stack[p] - (int)(sb[p * kWalkWG] & 0xFFFFu);
+          const int inv = G.stack[p] - (int)(sb[p * WG] & 0xFFFFu);
           if (inv > prev && inv < lvl) lvl = inv;
         }
         if (lvl == 0x7FFFFFFF) break;
         int layer_pot = 0;
         uint32_t best = 0;
         for (int p = 0; p < N; ++p) {
-          const int inv = G.stack[p] - (int)(sb[p * kWalkWG] & 0xFFFFu);
+          const int inv = G.stack[p] - (int)(sb[p * WG] & 0xFFFFu);
           const int c = (inv < lvl ? inv : lvl) - prev;
           layer_pot += c > 0 ? c : 0;
-          const uint32_t r = rk[p * kWalkWG];
+          const uint32_t r = rk[p * WG];
           if (!(folded >> p & 1u) && inv >= lvl && r > best) best = r;
         }
         if (best > 0) {
           int n_win = 0;
           for (int p = 0; p < N; ++p)
-            if (!(folded >> p & 1u) && G.stack[p] - (int)(sb[p * kWalkWG] & 0xFFFFu) >= lvl && rk[p * kWalkWG] == best)
+            if (!(folded >> p & 1u) && G.stack[p] - (int)(sb[p * WG] & 0xFFFFu) >= lvl && rk[p * WG] == best)
               ++n_win;
-          if (!(folded >> up & 1u) && inv_up >= lvl && rk[up * kWalkWG] == best) pay += (float)layer_pot / (float)n_win;
+          if (!(folded >> up & 1u) && inv_up >= lvl && rk[up * WG] == best) pay += (float)layer_pot / (float)n_win;
         }
         prev = lvl;
       }
@@ -284,15 +357,15 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
 
     // decision node
     const int player = (int)(queue & 7u);
-    const uint32_t sbv = sb[player * kWalkWG];
+    const uint32_t sbv = sb[player * WG];
     const int stk = (int)(sbv & 0xFFFFu), bet = (int)(sbv >> 16);
     const int to_call = current_bet - bet;
     const uint32_t vm = valid_mask(G, pot, stk, to_call, current_bet, lfr, raises);
-    const uint32_t Bp = bk[player * kWalkWG];
+    const uint32_t Bp = bk[player * WG];
     const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
     const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
     float sg[ROW];
-    load_bf16_row<ROW>(T.sig + (size_t)slot * ROW, sg);
+    load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
     // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
     float total = 0.0f;
 #pragma unroll
@@ -316,12 +389,13 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
     ++decisions;
     if (player == up) {
       const uint32_t av = (uint32_t)act | (vm << 8);
+      const uint32_t key = seg_key | slot;
       if (npend < kRegPending) {
-        ps[npend * kWalkWG] = slot;
-        pa[npend * kWalkWG] = av;
+        ps[npend * WG] = key;
+        pa[npend * WG] = av;
       } else {
-        T.node_slot[(size_t)npend * L + t] = slot;
-        T.node_av[(size_t)npend * L + t] = av;
+        sc.node_slot[(size_t)npend * L + t] = key;
+        sc.node_av[(size_t)npend * L + t] = av;
       }
       ++npend;
       upd_legal += __popc(vm);
@@ -329,13 +403,13 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
     // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
     queue >>= 3;
     --nq;
-    const int chips = chips_lds(G, s_kind, s_num, s_den, act, pot, stk, to_call, current_bet, lfr);
+    const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
     ab = ((ab << G.hist_bits) | (uint32_t)act) & kHistoryMask;
     if (G.fold_mask >> act & 1u) {
       folded |= 1u << player;
     } else {
       const int ns = stk - chips, nb = bet + chips;
-      sb[player * kWalkWG] = (uint32_t)ns | ((uint32_t)nb << 16);
+      sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 16);
       pot += chips;
       if (ns == 0) all_in |= 1u << player;
       if (nb > current_bet) {
@@ -345,7 +419,7 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
         ++raises;
         queue = 0;
         nq = 0;
-        for (int i = 1, p = wrap(player + 1, N); i < N; ++i, p = wrap(p + 1, N))
+        for (int k = 1, p = wrap(player + 1, N); k < N; ++k, p = wrap(p + 1, N))
           if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
       }
     }
@@ -354,276 +428,567 @@ __global__ __launch_bounds__(kWalkWG) void k_walk(const DevGame* __restrict__ gp
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: contributions of every recorded update-player node
-// (cfr_gpu.cu:784-837).  Every node on a path sees the terminal utility u;
-// sigma comes from the regret row, which no kernel of the launch has modified.
-//
-// A workgroup takes kScatterTile traversals, sorts their records by
-// (slot, record) in LDS, and sums each slot's run in FP64 in that order, so a
-// slot receives one set of global atomics per tile however hot it is (the
-// preflop info sets collect tens of thousands of records per launch).  The
-// run's slot is appended to the launch's update list.
+// Record emission: lane-ordered (key, lane << 5 | node) pairs.  Blocks of
+// kEmitLanes lanes, 4 lanes per thread.
 // ---------------------------------------------------------------------------
-// Workgroup sizes the scatter is built for; 4 traversals per thread, sort
-// capacity 2 records per traversal per pass.
-template <int WG>
-__device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, int n_pow2) {
-  for (int k = 2; k <= n_pow2; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n_pow2; i += WG) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = keys[i], b = keys[ixj];
-          const bool ascending = (i & k) == 0;
-          if ((a > b) == ascending) {
-            keys[i] = b;
-            keys[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-}
-
-template <int ROW, int kScatterWG>
-__global__ __launch_bounds__(kScatterWG) void k_scatter(const DevGame* __restrict__ gp,
-                                                        LaunchState* __restrict__ st, DevTables T,
-                                                        uint32_t n_lanes) {
-  constexpr int kScatterTile = 4 * kScatterWG;  // traversals per workgroup
-  constexpr int kScatterCap = 2 * kScatterTile; // records sorted per pass
-  __shared__ uint64_t s_keys[kScatterCap];
-  __shared__ uint32_t s_run[kScatterCap];
-  __shared__ uint32_t s_wsum[kScatterWG / 64];
-  __shared__ uint32_t s_runs, s_base;
-  const DevGame& G = *gp;
-  uint32_t n = (uint32_t)st->n;
-  if (n > n_lanes) n = n_lanes;
-  const uint32_t t0 = blockIdx.x * kScatterTile;
-  if (t0 >= n) return;
-  const float tw = (float)st->t_weight;
-  const int A = G.num_actions;
-  const int tx = threadIdx.x, lane = tx & 63, wave = tx >> 6;
-
-  // records of this thread's traversals: t0 + tx + j*WG, j < 4
-  constexpr int kPer = kScatterTile / kScatterWG;
-  int cnt[kPer];
-  uint32_t mine = 0;
-#pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const uint32_t t = t0 + tx + j * kScatterWG;
-    cnt[j] = t < n ? (int)T.node_count[t] : 0;
-    mine += (uint32_t)cnt[j];
-  }
-  // block exclusive scan of the per-thread record counts (waves, then wave totals)
-  uint32_t incl = mine;
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t v = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += v;
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
   }
-  if (lane == 63) s_wsum[wave] = incl;
+  if (lane == 63) s_w[w] = incl;
   __syncthreads();
-  uint32_t before = 0, total = 0;
-  for (int w = 0; w < kScatterWG / 64; ++w) {
-    const uint32_t v = s_wsum[w];
-    if (w < wave) before += v;
-    total += v;
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t x = s_w[k];
+    if (k < w) before += x;
+    tot += x;
   }
-  const uint32_t my_first = before + incl - mine;
+  __syncthreads();
+  if (total) *total = tot;
+  return before + incl - v;
+}
 
-  for (uint32_t c0 = 0; c0 < total; c0 += kScatterCap) {
-    const uint32_t cn = total - c0 < (uint32_t)kScatterCap ? total - c0 : (uint32_t)kScatterCap;
-    int np2 = 64;
-    while ((uint32_t)np2 < cn) np2 <<= 1;
-    for (int i = tx; i < np2; i += kScatterWG) s_keys[i] = ~0ull;
-    if (tx == 0) s_runs = 0;
-    __syncthreads();
-    // keys: slot << 32 | record reference (local traversal index << 5 | k)
-    uint32_t pos = my_first;
+__device__ __forceinline__ uint32_t count4(uint32_t w) {
+  return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
+}
+
+__global__ __launch_bounds__(256) void k_emit_count(LaunchCfg lc, DevScratch sc) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t l0 = blockIdx.x * kEmitLanes + threadIdx.x * 4;
+  const uint32_t c = l0 < lc.lanes ? count4(*reinterpret_cast<const uint32_t*>(sc.node_count + l0)) : 0u;
+  uint32_t total = 0;
+  (void)block_excl_scan_256(c, s_w, &total);
+  if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block counts in place (one workgroup); resets the
+// launch counters.
+__global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc) {
+  __shared__ uint32_t s_w[16];
+  const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
+  const uint32_t per = (nblk + 1023) / 1024;
+  const uint32_t b0 = min(threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += sc.blk_cnt[b];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = sum;
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const uint32_t tl = (uint32_t)(tx + j * kScatterWG);
-      for (int k = 0; k < cnt[j]; ++k, ++pos) {
-        if (pos < c0 || pos >= c0 + cn) continue;
-        const uint32_t slot = T.node_slot[(size_t)k * T.lanes + t0 + tl];
-        s_keys[pos - c0] = ((uint64_t)slot << 32) | (tl << 5) | (uint32_t)k;
-      }
-    }
-    __syncthreads();
-    block_bitonic_sort<kScatterWG>(s_keys, np2);
-    // each run of equal slots is summed by the thread holding its first record
-    for (uint32_t i = tx; i < cn; i += kScatterWG) {
-      const uint32_t slot = (uint32_t)(s_keys[i] >> 32);
-      if (i > 0 && (uint32_t)(s_keys[i - 1] >> 32) == slot) continue;
-      const float* rrow = T.R + (size_t)slot * ROW;
-      float rr[ROW];
-      load_f32_row<ROW>(rrow, rr);
-      double dr[ROW], ds[ROW];
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) dr[a] = ds[a] = 0.0;
-      uint32_t legal_any = 0;
-      // sigma of the run's rows, recomputed only when the legal mask changes
-      // (hash collisions can put info sets with different masks in one slot)
-      uint32_t cached_vm = 0;
-      float sig_c[ROW];
-      float q_of_act[ROW];
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) sig_c[a] = q_of_act[a] = 0.0f;
-      uint32_t r = i;
-      while (r < cn && (uint32_t)(s_keys[r] >> 32) == slot) {
-        // gather up to 4 records of the run at once so their loads overlap
-        constexpr int kBatch = 4;
-        uint32_t av[kBatch];
-        float uu[kBatch];
-        int m = 0;
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-          av[b] = 0;
-          uu[b] = 0.0f;
-          if (r + b < cn && (uint32_t)(s_keys[r + b] >> 32) == slot) {
-            const uint32_t ref = (uint32_t)s_keys[r + b];
-            const uint32_t t = t0 + (ref >> 5), k = ref & 31u;
-            av[b] = T.node_av[(size_t)k * T.lanes + t];
-            uu[b] = T.node_u[t];
-            m = b + 1;
-          }
-        }
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-          if (b >= m) break;
-          const uint32_t act = av[b] & 0xFFu, vm = av[b] >> 8;
-          const float u = uu[b];
-          if (vm != cached_vm) {
-            cached_vm = vm;
-            float sum_r = 0.0f;
-            float lr[ROW];
-#pragma unroll
-            for (int a = 0; a < ROW; ++a) {
-              lr[a] = (a < A && (vm >> a & 1u)) ? fmaxf(0.0f, rr[a]) : 0.0f;
-              sum_r += lr[a];
-            }
-            const float uni = 1.0f / (float)__popc(vm);
-#pragma unroll
-            for (int a = 0; a < ROW; ++a) {
-              sig_c[a] = (sum_r > 1e-7f) ? lr[a] / sum_r : ((a < A && (vm >> a & 1u)) ? uni : 0.0f);
-              q_of_act[a] = fmaxf((sum_r > 1e-7f) ? lr[a] / sum_r : uni, 0.05f);
-            }
-          }
-          legal_any |= vm;
-          float q = q_of_act[0];
-#pragma unroll
-          for (int a = 1; a < ROW; ++a) q = ((uint32_t)a == act) ? q_of_act[a] : q;
-          const float rd_sampled = u / q - u;
-          const float rd_other = 0.0f - u;
-#pragma unroll
-          for (int a = 0; a < ROW; ++a)
-            if (a < A && (vm >> a & 1u)) {
-              dr[a] += (double)((uint32_t)a == act ? rd_sampled : rd_other);
-              ds[a] += (double)(tw * sig_c[a]);
-            }
-        }
-        r += (uint32_t)m;
-      }
-      double* drow = T.D + (size_t)slot * (2 * ROW);
-#pragma unroll
-      for (int a = 0; a < ROW; ++a)
-        if (legal_any >> a & 1u) {
-          unsafeAtomicAdd(drow + a, dr[a]);
-          unsafeAtomicAdd(drow + ROW + a, ds[a]);
-        }
-      s_run[atomicAdd(&s_runs, 1u)] = slot;
-    }
-    __syncthreads();
-    if (tx == 0) s_base = atomicAdd(&st->runs, s_runs);
-    __syncthreads();
-    for (uint32_t i = tx; i < s_runs; i += kScatterWG) T.upd_list[s_base + i] = s_run[i];
-    __syncthreads();  // s_keys and s_run are reused by the next chunk
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t x = s_w[k];
+    if (k < w) before += x;
+    tot += x;
+  }
+  uint32_t run = before + incl - sum;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t c = sc.blk_cnt[b];
+    sc.blk_cnt[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) {
+    sc.cnt->n_rec = tot;
+    sc.cnt->n_long = 0;
+    sc.cnt->n_pieces = 0;
   }
 }
+
+__global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t l0 = blockIdx.x * kEmitLanes + threadIdx.x * 4;
+  const uint32_t w4 = l0 < lc.lanes ? *reinterpret_cast<const uint32_t*>(sc.node_count + l0) : 0u;
+  uint32_t pos = sc.blk_cnt[blockIdx.x] + block_excl_scan_256(count4(w4), s_w, nullptr);
+  if (l0 >= lc.lanes) return;
+  const size_t L = sc.cap_lanes;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t t = l0 + (uint32_t)q;
+    const uint32_t c = (w4 >> (8 * q)) & 0xFFu;
+    for (uint32_t k = 0; k < c; ++k, ++pos) {
+      sc.key[0][pos] = sc.node_slot[(size_t)k * L + t];
+      sc.val[0][pos] = (t << 5) | k;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// k_update: the scatter's run list holds every touched slot (once per tile);
-// the epoch stamp lets exactly one thread handle each slot.
-// R <- clamp(R + dR), S <- S + dS, sigma <- RM(R), deltas cleared.
+// Stable LSD radix sort, one 8-bit digit per pass: tile histograms, a scan of
+// the (digit, tile) counts, and a scatter that ranks each key within its tile
+// by wave-level digit matching (8 ballots) so equal digits keep their order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t n_tiles_of(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+// Lanes of the wave whose digit equals this lane's, among `valid` lanes.
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t bal = __ballot((d >> b) & 1u);
+    peers &= ((d >> b) & 1u) ? bal : ~bal;
+  }
+  return peers;
+}
+
+__global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict__ keys,
+                                                     const LaunchCounters* __restrict__ cnt, uint32_t shift,
+                                                     uint32_t* __restrict__ hist, uint32_t max_tiles) {
+  __shared__ uint32_t s_h[kSortBins];
+  const uint32_t n = cnt->n_rec;
+  const uint32_t nt = n_tiles_of(n);
+  const int tx = threadIdx.x, lane = tx & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    s_h[tx] = 0;
+    __syncthreads();
+#pragma unroll 4
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
+      const bool valid = idx < n;
+      const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
+      const uint64_t peers = digit_peers(d, valid);  // one LDS add per distinct digit of the wave
+      if (valid && (peers & lt) == 0) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(size_t)tx * max_tiles + tile] = s_h[tx];
+    __syncthreads();
+  }
+}
+
+// One workgroup per digit: exclusive prefix of the digit's tile counts in
+// place (coalesced chunks of 1024 tiles) and the digit's total.
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ bin_total,
+                                                 const LaunchCounters* __restrict__ cnt, uint32_t max_tiles) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t nt = n_tiles_of(cnt->n_rec);
+  uint32_t* row = hist + (size_t)blockIdx.x * max_tiles;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nt; c0 += 1024) {
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t idx = c0 + threadIdx.x * 4 + (uint32_t)k;
+      v[k] = idx < nt ? row[idx] : 0u;
+      sum += v[k];
+    }
+    uint32_t tot = 0;
+    uint32_t run = carry + block_excl_scan_256(sum, s_w, &tot);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t idx = c0 + threadIdx.x * 4 + (uint32_t)k;
+      if (idx < nt) row[idx] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bin_total[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                        const LaunchCounters* __restrict__ cnt, uint32_t shift,
+                                                        const uint32_t* __restrict__ hist,
+                                                        const uint32_t* __restrict__ bin_total, uint32_t max_tiles) {
+  static_assert(kSortWG == 256 && kSortBins == 256, "one thread per digit");
+  constexpr int kWaves = kSortWG / 64;
+  __shared__ uint32_t s_wc[kWaves][kSortBins];  // per-wave digit counts, then per-wave bases
+  __shared__ uint32_t s_gb[kSortBins];          // global offset of (digit, tile)
+  __shared__ uint32_t s_w[4];
+  const uint32_t n = cnt->n_rec;
+  const uint32_t nt = n_tiles_of(n);
+  const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  // first output position of each digit: exclusive scan of the digit totals
+  const uint32_t digit_base = block_excl_scan_256(bin_total[tx], s_w, nullptr);
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) s_wc[k][tx] = 0;
+    __syncthreads();
+    // wave w holds keys [wbase, wbase + 64 * kSortItems) of the tile, item-major
+    uint32_t kk[kSortItems], vv[kSortItems], off[kSortItems];
+    const uint32_t wbase = tile * kSortTile + (uint32_t)(w * 64 * kSortItems);
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
+      const bool valid = idx < n;
+      kk[i] = valid ? kin[idx] : 0u;
+      vv[i] = valid ? vin[idx] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
+      const bool valid = idx < n;
+      const uint32_t d = (kk[i] >> shift) & 0xFFu;
+      const uint64_t peers = digit_peers(d, valid);
+      // all lanes read the wave's count before the group's first lane bumps it
+      const uint32_t before = s_wc[w][d];
+      off[i] = before + (uint32_t)__popcll(peers & lt);
+      __builtin_amdgcn_wave_barrier();
+      if (valid && (peers & lt) == 0) s_wc[w][d] = before + (uint32_t)__popcll(peers);
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+      uint32_t run = 0;
+#pragma unroll
+      for (int k = 0; k < kWaves; ++k) {
+        const uint32_t c = s_wc[k][tx];
+        s_wc[k][tx] = run;
+        run += c;
+      }
+      s_gb[tx] = digit_base + hist[(size_t)tx * max_tiles + tile];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
+      if (idx < n) {
+        const uint32_t d = (kk[i] >> shift) & 0xFFu;
+        const uint32_t pos = s_gb[d] + s_wc[w][d] + off[i];
+        kout[pos] = kk[i];
+        vout[pos] = vv[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Owner-computes table update of one run of records (cfr_gpu.cu:784-837 for
+// the contributions, :1018-1022 and :318-351 for clamp and re-matching).
+// sigma comes from the pre-launch regret row, which nothing has modified yet:
+// each slot is written only by its run's owner.
 // ---------------------------------------------------------------------------
 template <int ROW>
-__global__ __launch_bounds__(256) void k_update(const DevGame* __restrict__ gp, const LaunchState* __restrict__ st,
-                                                DevTables T) {
-  const DevGame& G = *gp;
-  const bool cfr_plus = G.use_cfr_plus != 0;
-  const uint32_t epoch = st->epoch;
-  const uint32_t runs = st->runs;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < runs; i += gridDim.x * blockDim.x) {
-    const uint32_t slot = T.upd_list[i];
-    if (atomicExch(T.stamp + slot, epoch) == epoch) continue;
-    float* R = T.R + (size_t)slot * ROW;
-    float* S = T.S + (size_t)slot * ROW;
-    double* D = T.D + (size_t)slot * (2 * ROW);
-    float r[ROW], s[ROW];
-    load_f32_row<ROW>(R, r);
-    load_f32_row<ROW>(S, s);
-    float sum = 0.0f;
+struct RunAcc {
+  double dr[ROW], ds[ROW];
+  uint32_t cached_vm;
+  float sig_c[ROW], q_of_act[ROW];
+
+  __device__ __forceinline__ void reset() {
 #pragma unroll
     for (int a = 0; a < ROW; ++a) {
-      float nr = (float)((double)r[a] + D[a]);
-      if (cfr_plus) nr = fmaxf(0.0f, nr);
-      r[a] = nr;
-      s[a] = (float)((double)s[a] + D[ROW + a]);
-      sum += fmaxf(0.0f, nr);
+      dr[a] = ds[a] = 0.0;
+      sig_c[a] = q_of_act[a] = 0.0f;
     }
-    uint32_t packed[ROW / 2];
+    cached_vm = 0;
+  }
+  // sigma of the slot's row, recomputed only when the legal mask changes
+  // (hash collisions can put info sets with different masks in one slot)
+  __device__ __forceinline__ void add(const float (&rr)[ROW], int A, float tw, uint32_t av, float u) {
+    const uint32_t act = av & 0xFFu, vm = av >> 8;
+    if (vm != cached_vm) {
+      cached_vm = vm;
+      float sum_r = 0.0f;
+      float lr[ROW];
 #pragma unroll
-    for (int a = 0; a < ROW; a += 2) {
-      const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, r[a]) / sum : 0.0f;
-      const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, r[a + 1]) / sum : 0.0f;
-      packed[a / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+      for (int a = 0; a < ROW; ++a) {
+        lr[a] = (a < A && (vm >> a & 1u)) ? fmaxf(0.0f, rr[a]) : 0.0f;
+        sum_r += lr[a];
+      }
+      const float uni = 1.0f / (float)__popc(vm);
+#pragma unroll
+      for (int a = 0; a < ROW; ++a) {
+        sig_c[a] = (sum_r > 1e-7f) ? lr[a] / sum_r : ((a < A && (vm >> a & 1u)) ? uni : 0.0f);
+        q_of_act[a] = fmaxf((sum_r > 1e-7f) ? lr[a] / sum_r : uni, 0.05f);
+      }
+    }
+    float q = q_of_act[0];
+#pragma unroll
+    for (int a = 1; a < ROW; ++a) q = ((uint32_t)a == act) ? q_of_act[a] : q;
+    const float rd_sampled = u / q - u;
+    const float rd_other = 0.0f - u;
+#pragma unroll
+    for (int a = 0; a < ROW; ++a)
+      if (a < A && (vm >> a & 1u)) {
+        dr[a] += (double)((uint32_t)a == act ? rd_sampled : rd_other);
+        ds[a] += (double)(tw * sig_c[a]);
+      }
+  }
+};
+
+template <int ROW>
+__device__ __forceinline__ void apply_row(float* __restrict__ R, float* __restrict__ S, uint16_t* __restrict__ sig,
+                                          const float (&rr)[ROW], const double (&dr)[ROW], const double (&ds)[ROW],
+                                          bool cfr_plus) {
+  float s[ROW];
+  load_f32_row<ROW>(S, s);
+  float r[ROW];
+  float sum = 0.0f;
+#pragma unroll
+  for (int a = 0; a < ROW; ++a) {
+    float nr = (float)((double)rr[a] + dr[a]);
+    if (cfr_plus) nr = fmaxf(0.0f, nr);
+    r[a] = nr;
+    s[a] = (float)((double)s[a] + ds[a]);
+    sum += fmaxf(0.0f, nr);
+  }
+  uint32_t packed[ROW / 2];
+#pragma unroll
+  for (int a = 0; a < ROW; a += 2) {
+    const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, r[a]) / sum : 0.0f;
+    const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, r[a + 1]) / sum : 0.0f;
+    packed[a / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+  }
+#pragma unroll
+  for (int v = 0; v < ROW / 4; ++v) {
+    reinterpret_cast<float4*>(R)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
+    reinterpret_cast<float4*>(S)[v] = make_float4(s[4 * v], s[4 * v + 1], s[4 * v + 2], s[4 * v + 3]);
+  }
+#pragma unroll
+  for (int v = 0; v < ROW / 8; ++v)
+    reinterpret_cast<uint4*>(sig)[v] = make_uint4(packed[4 * v], packed[4 * v + 1], packed[4 * v + 2], packed[4 * v + 3]);
+}
+
+struct RunCtx {
+  float* R;
+  float* S;
+  uint16_t* sig;
+  int A;
+  bool cfr_plus;
+  float tw;
+};
+
+__device__ __forceinline__ RunCtx run_ctx(uint32_t key, uint32_t slot_bits, const DevSeg* __restrict__ segs,
+                                          const DevGame* __restrict__ tenants, int row) {
+  const uint32_t si = (uint32_t)((uint64_t)key >> slot_bits);
+  const uint32_t slot = key & (uint32_t)((1ull << slot_bits) - 1ull);
+  const DevSeg& S = segs[si];
+  const DevGame& G = tenants[S.tenant];
+  RunCtx c;
+  c.R = G.R + (size_t)slot * row;
+  c.S = G.S + (size_t)slot * row;
+  c.sig = G.sig + (size_t)slot * row;
+  c.A = G.num_actions;
+  c.cfr_plus = G.use_cfr_plus != 0;
+  c.tw = (float)S.t_weight;
+  return c;
+}
+
+// Every record is looked at by one lane; the lane holding a run's first
+// record owns the run.  The run's end comes from the wave's head flags
+// (ballot), else from a short forward scan.  Runs of at most kShortRun
+// records are reduced and applied by their owner; longer runs are split into
+// kPiece-record pieces for k_reduce_piece and finished by k_reduce_runs.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                const DevGame* __restrict__ tenants, DevScratch sc,
+                                                const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals) {
+  const uint32_t n = sc.cnt->n_rec;
+  const size_t L = sc.cap_lanes;
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
+    const uint32_t i = base + (uint32_t)lane;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys[i] : 0u;
+    const bool head = valid && (i == 0 || keys[i - 1] != key);
+    const uint64_t heads = __ballot(head);
+    if (!head) continue;
+    const uint64_t later = heads & ~((2ull << lane) - 1ull);  // heads of higher lanes
+    uint32_t e;
+    if (later) {
+      e = base + (uint32_t)__builtin_ctzll(later);
+    } else {
+      e = min(base + 64u, n);
+      while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
+    }
+    if (e - i > kShortRun) {
+      // exact end: keys[e - 1] == key and the keys ascend
+      uint32_t lo = e, hi = n;
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] == key) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
+      const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
+      const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
+      sc.long_runs[r] = make_uint4(i, len, p0, np);
+      for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
+      continue;
+    }
+    const RunCtx c = run_ctx(key, lc.slot_bits, segs, tenants, ROW);
+    float rr[ROW];
+    load_f32_row<ROW>(c.R, rr);
+    RunAcc<ROW> acc;
+    acc.reset();
+    for (uint32_t r = i; r < e;) {
+      // gather up to 4 records at once so their loads overlap
+      constexpr int kBatch = 4;
+      uint32_t av[kBatch];
+      float uu[kBatch];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        av[b] = 0;
+        uu[b] = 0.0f;
+        if (r + b < e) {
+          const uint32_t ref = vals[r + b];
+          const uint32_t t = ref >> 5, k = ref & 31u;
+          av[b] = sc.node_av[(size_t)k * L + t];
+          uu[b] = sc.node_u[t];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b)
+        if (r + b < e) acc.add(rr, c.A, c.tw, av[b], uu[b]);
+      r += kBatch;
+    }
+    apply_row<ROW>(c.R, c.S, c.sig, rr, acc.dr, acc.ds, c.cfr_plus);
+  }
+}
+
+// One workgroup per piece of a long run: records split over the threads in a
+// fixed pattern, partial sums combined in a fixed tree, FP64 sums stored.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                      const DevGame* __restrict__ tenants, DevScratch sc,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ vals) {
+  __shared__ double s_red[4][2 * ROW];
+  const uint32_t n_pieces = sc.cnt->n_pieces;
+  const size_t L = sc.cap_lanes;
+  const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
+  for (uint32_t p = blockIdx.x; p < n_pieces; p += gridDim.x) {
+    const uint4 run = sc.long_runs[sc.piece_run[p]];
+    const uint32_t start = run.x + (p - run.z) * kPiece;
+    const uint32_t end = min(run.x + run.y, start + kPiece);
+    const RunCtx c = run_ctx(keys[start], lc.slot_bits, segs, tenants, ROW);
+    float rr[ROW];
+    load_f32_row<ROW>(c.R, rr);
+    RunAcc<ROW> acc;
+    acc.reset();
+#pragma unroll
+    for (uint32_t k = 0; k < kPiece / 256; ++k) {
+      const uint32_t idx = start + (uint32_t)tx + 256u * k;
+      if (idx < end) {
+        const uint32_t ref = vals[idx];
+        const uint32_t t = ref >> 5, kk = ref & 31u;
+        acc.add(rr, c.A, c.tw, sc.node_av[(size_t)kk * L + t], sc.node_u[t]);
+      }
     }
 #pragma unroll
-    for (int v = 0; v < ROW / 4; ++v) {
-      reinterpret_cast<float4*>(R)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
-      reinterpret_cast<float4*>(S)[v] = make_float4(s[4 * v], s[4 * v + 1], s[4 * v + 2], s[4 * v + 3]);
+    for (int a = 0; a < ROW; ++a) {
+      double x = acc.dr[a], y = acc.ds[a];
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) {
+        x += __shfl_xor(x, d, 64);
+        y += __shfl_xor(y, d, 64);
+      }
+      acc.dr[a] = x;
+      acc.ds[a] = y;
     }
+    if (lane == 0) {
 #pragma unroll
-    for (int v = 0; v < ROW / 8; ++v)
-      reinterpret_cast<uint4*>(T.sig + (size_t)slot * ROW)[v] =
-          make_uint4(packed[4 * v], packed[4 * v + 1], packed[4 * v + 2], packed[4 * v + 3]);
+      for (int a = 0; a < ROW; ++a) {
+        s_red[w][a] = acc.dr[a];
+        s_red[w][ROW + a] = acc.ds[a];
+      }
+    }
+    __syncthreads();
+    if (tx < 2 * ROW) sc.partial[(size_t)p * 32 + tx] = ((s_red[0][tx] + s_red[1][tx]) + s_red[2][tx]) + s_red[3][tx];
+    __syncthreads();
+  }
+}
+
+// One thread per long run: its pieces' sums in piece order, then the update.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                     const DevGame* __restrict__ tenants, DevScratch sc,
+                                                     const uint32_t* __restrict__ keys) {
+  const uint32_t n_long = sc.cnt->n_long;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_long; r += gridDim.x * blockDim.x) {
+    const uint4 run = sc.long_runs[r];
+    const RunCtx c = run_ctx(keys[run.x], lc.slot_bits, segs, tenants, ROW);
+    float rr[ROW];
+    load_f32_row<ROW>(c.R, rr);
+    double dr[ROW], ds[ROW];
 #pragma unroll
-    for (int v = 0; v < ROW / 2; ++v) reinterpret_cast<double4*>(D)[v] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int a = 0; a < ROW; ++a) dr[a] = ds[a] = 0.0;
+    for (uint32_t j = 0; j < run.w; ++j) {
+      const double* q = sc.partial + (size_t)(run.z + j) * 32;
+#pragma unroll
+      for (int a = 0; a < ROW; ++a) {
+        dr[a] += q[a];
+        ds[a] += q[ROW + a];
+      }
+    }
+    apply_row<ROW>(c.R, c.S, c.sig, rr, dr, ds, c.cfr_plus);
+  }
+}
+
+template <int ROW, int WG>
+void walk_launch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                 const DevScratch& sc, uint32_t tpl, mccfr_trace* trace) {
+  const uint32_t waves = lc.lanes / (64u * tpl);
+  const dim3 grid((waves * 64u + WG - 1) / WG);
+  hipLaunchKernelGGL((k_walk<ROW, WG>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
+}
+
+template <int ROW>
+void walk_dispatch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                   const DevScratch& sc, WalkShape sh, mccfr_trace* trace) {
+  switch (sh.wg) {
+    case 64: walk_launch<ROW, 64>(s, lc, segs, tenants, sc, sh.tpl, trace); break;
+    case 128: walk_launch<ROW, 128>(s, lc, segs, tenants, sc, sh.tpl, trace); break;
+    case 512: walk_launch<ROW, 512>(s, lc, segs, tenants, sc, sh.tpl, trace); break;
+    case 1024: walk_launch<ROW, 1024>(s, lc, segs, tenants, sc, sh.tpl, trace); break;
+    default: walk_launch<ROW, 256>(s, lc, segs, tenants, sc, sh.tpl, trace); break;
   }
 }
 
 }  // namespace
 
-void launch_traverse(hipStream_t s, int row, const DevGame* g, LaunchState* st, BucketTables bt,
-                     ArithTables at, const uint8_t* lut, DevTables t, uint32_t n_lanes, uint32_t walk_blocks,
-                     uint32_t scatter_wg, mccfr_trace* trace) {
-  hipLaunchKernelGGL(k_deal, dim3(grid_for(n_lanes, kDealWG, 1u << 30)), dim3(kDealWG), 0, s, g, st, bt, at, lut, t,
-                     n_lanes);
-  const dim3 wgrid(grid_for(n_lanes, kWalkWG, walk_blocks ? walk_blocks : 1u << 30));
+void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace) {
+  hipLaunchKernelGGL(k_deal, dim3(lc.lanes / kDealWG), dim3(kDealWG), 0, s, lc, segs, tenants, in, sc);
   if (row == 8)
-    hipLaunchKernelGGL(k_walk<8>, wgrid, dim3(kWalkWG), 0, s, g, st, t, n_lanes, trace);
+    walk_dispatch<8>(s, lc, segs, tenants, sc, shape, trace);
   else
-    hipLaunchKernelGGL(k_walk<16>, wgrid, dim3(kWalkWG), 0, s, g, st, t, n_lanes, trace);
-  const unsigned wg = scatter_wg == 256 || scatter_wg == 512 ? scatter_wg : 1024;
-  const dim3 sgrid(grid_for(n_lanes, 4 * wg, 1u << 30));
-#define MCCFR_SCATTER(R, W) hipLaunchKernelGGL((k_scatter<R, W>), sgrid, dim3(W), 0, s, g, st, t, n_lanes)
-  if (row == 8) {
-    if (wg == 256) MCCFR_SCATTER(8, 256);
-    else if (wg == 512) MCCFR_SCATTER(8, 512);
-    else MCCFR_SCATTER(8, 1024);
-  } else {
-    if (wg == 256) MCCFR_SCATTER(16, 256);
-    else if (wg == 512) MCCFR_SCATTER(16, 512);
-    else MCCFR_SCATTER(16, 1024);
-  }
-#undef MCCFR_SCATTER
+    walk_dispatch<16>(s, lc, segs, tenants, sc, shape, trace);
 }
 
-void launch_update(hipStream_t s, int row, const DevGame* g, LaunchState* st, DevTables t, uint32_t n_lanes) {
-  const unsigned grid = grid_for((size_t)n_lanes * 2, 256, 8192);
-  if (row == 8)
-    hipLaunchKernelGGL(k_update<8>, dim3(grid), dim3(256), 0, s, g, st, t);
-  else
-    hipLaunchKernelGGL(k_update<16>, dim3(grid), dim3(256), 0, s, g, st, t);
+void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
+                   const DevScratch& sc, uint32_t key_bits) {
+  const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
+  hipLaunchKernelGGL(k_emit_count, dim3(nblk), dim3(256), 0, s, lc, sc);
+  hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, sc);
+  hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
+  // at most lanes * kMaxDepth records; the sort kernels loop over the tiles
+  const uint64_t max_rec = (uint64_t)lc.lanes * kMaxDepth;
+  const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
+  const uint32_t sgrid = std::min<uint32_t>(std::max<uint32_t>(max_rec_tiles, 1u), 2048u);
+  const uint32_t passes = key_bits == 0 ? 1u : (key_bits + 7u) / 8u;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t shift = 8u * p;
+    const uint32_t* kin = sc.key[p & 1];
+    const uint32_t* vin = sc.val[p & 1];
+    hipLaunchKernelGGL(k_rs_hist, dim3(sgrid), dim3(kSortWG), 0, s, kin, sc.cnt, shift, sc.hist, sc.max_tiles);
+    hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(256), 0, s, sc.hist, sc.bin_total, sc.cnt, sc.max_tiles);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(sgrid), dim3(kSortWG), 0, s, kin, vin, sc.key[(p + 1) & 1],
+                       sc.val[(p + 1) & 1], sc.cnt, shift, sc.hist, sc.bin_total, sc.max_tiles);
+  }
+  const uint32_t* keys = sc.key[passes & 1];
+  const uint32_t* vals = sc.val[passes & 1];
+  const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
+  if (row == 8) {
+    hipLaunchKernelGGL(k_reduce<8>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
+    hipLaunchKernelGGL(k_reduce_piece<8>, dim3(2048), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
+    hipLaunchKernelGGL(k_reduce_runs<8>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys);
+  } else {
+    hipLaunchKernelGGL(k_reduce<16>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
+    hipLaunchKernelGGL(k_reduce_piece<16>, dim3(2048), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
+    hipLaunchKernelGGL(k_reduce_runs<16>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys);
+  }
 }
 
 }  // namespace mccfr

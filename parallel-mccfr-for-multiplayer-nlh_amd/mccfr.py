@@ -57,6 +57,47 @@ TRACE_DTYPE = np.dtype(
 )
 
 
+class RootState(C.Structure):
+    """mccfr_root_state: street 0 = the reference's preflop root; 1/2/3 = a
+    public flop/turn/river state (fixed board, stacks, investments, active seats)."""
+    _fields_ = [
+        ("street", C.c_int32),
+        ("dealer", C.c_int32),
+        ("active_mask", C.c_uint32),
+        ("history", C.c_uint32),
+        ("stack", C.c_int32 * 6),
+        ("invested", C.c_int32 * 6),
+        ("board", C.c_uint8 * 5),
+        ("reserved", C.c_uint8 * 3),
+    ]
+
+
+def root_state(street, dealer, active, stacks, invested, board, history=0) -> RootState:
+    r = RootState()
+    r.street = street
+    r.dealer = dealer
+    r.active_mask = active if isinstance(active, int) else sum(1 << p for p in active)
+    r.history = history
+    for i, v in enumerate(stacks):
+        r.stack[i] = int(v)
+    for i, v in enumerate(invested):
+        r.invested[i] = int(v)
+    for i, c in enumerate(board):
+        r.board[i] = int(c)
+    return r
+
+
+class Segment(C.Structure):
+    _fields_ = [
+        ("tenant", C.c_uint32),
+        ("n", C.c_uint32),
+        ("update_player", C.c_int32),
+        ("reserved", C.c_uint32),
+        ("t_weight", C.c_int64),
+        ("first_tid", C.c_uint64),
+    ]
+
+
 class Trace(C.Structure):
     _fields_ = [
         ("utility", C.c_float),
@@ -90,9 +131,8 @@ _SIGS = {
     "mccfr_sync": (C.c_int, [C.c_void_p]),
     "mccfr_set_max_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mccfr_randomize_tables": (C.c_int, [C.c_void_p, C.c_uint64]),
-    "mccfr_set_walk_blocks": (C.c_int, [C.c_void_p, C.c_uint32]),
-    "mccfr_get_walk_blocks": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
-    "mccfr_set_scatter_wg": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mccfr_set_launch_shape": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "mccfr_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mccfr_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_table_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
@@ -107,6 +147,30 @@ _SIGS = {
     "mccfr_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "mccfr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mccfr_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_tenant_footprint": (C.c_int, [C.POINTER(GameCfg), C.POINTER(C.c_uint64)]),
+    "mccfr_tenant_create": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.POINTER(RootState), C.POINTER(C.c_uint32)]),
+    "mccfr_tenant_destroy": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mccfr_run_segments": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mccfr_train_tenants": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int64]),
+    "mccfr_tenant_counters": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "mccfr_tenant_table_shape": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "mccfr_tenant_download": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                        C.c_int]),
+    "mccfr_tenant_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_tenant_normalized_strategy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_tenant_randomize": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "mccfr_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "mccfr_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mccfr_server_destroy": (None, [C.c_void_p]),
+    "mccfr_server_submit": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.POINTER(RootState), C.c_int64, C.c_uint32,
+                                      C.POINTER(C.c_uint64)]),
+    "mccfr_server_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mccfr_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "mccfr_server_tenant": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32)]),
+    "mccfr_server_result": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
 }
 
 
@@ -287,16 +351,13 @@ class Context:
         self.check(lib().mccfr_set_max_batch(self.h, n))
         self.max_batch = n
 
-    def set_walk_blocks(self, blocks: int):
-        self.check(lib().mccfr_set_walk_blocks(self.h, blocks))
+    def set_launch_shape(self, walk_wg: int, traversals_per_lane: int):
+        self.check(lib().mccfr_set_launch_shape(self.h, walk_wg, traversals_per_lane))
 
-    def set_scatter_wg(self, wg: int):
-        self.check(lib().mccfr_set_scatter_wg(self.h, wg))
-
-    def walk_blocks(self) -> int:
-        b = C.c_uint32()
-        self.check(lib().mccfr_get_walk_blocks(self.h, C.byref(b)))
-        return int(b.value)
+    def launch_shape(self):
+        w, t = C.c_uint32(), C.c_uint32()
+        self.check(lib().mccfr_get_launch_shape(self.h, C.byref(w), C.byref(t)))
+        return int(w.value), int(t.value)
 
     def randomize_tables(self, seed: int):
         self.check(lib().mccfr_randomize_tables(self.h, seed))
@@ -370,3 +431,193 @@ class Context:
 
     def eval7_device(self, dcards: int, stride: int, dranks: int, n: int, evaluator: int):
         self.check(lib().mccfr_eval7_device(self.h, C.c_void_p(dcards), stride, C.c_void_p(dranks), n, evaluator))
+
+    # tenants and batched launches ---------------------------------------------
+    def tenant_create(self, cfg: GameCfg, root: Optional[RootState] = None) -> int:
+        t = C.c_uint32()
+        self.check(lib().mccfr_tenant_create(self.h, C.byref(cfg), C.byref(root) if root is not None else None,
+                                             C.byref(t)))
+        return int(t.value)
+
+    def tenant_destroy(self, tenant: int):
+        self.check(lib().mccfr_tenant_destroy(self.h, tenant))
+
+    def tenant_shape(self, tenant: int):
+        s, r = C.c_uint64(), C.c_uint32()
+        self.check(lib().mccfr_tenant_table_shape(self.h, tenant, C.byref(s), C.byref(r)))
+        return int(s.value), int(r.value)
+
+    def run_segments(self, segs, trace: bool = False):
+        """segs: list of (tenant, n, update_player, t_weight, first_tid)."""
+        arr = (Segment * len(segs))()
+        for i, (t, n, up, w, tid) in enumerate(segs):
+            arr[i] = Segment(t, n, up, 0, w, tid)
+        total = sum(s[1] for s in segs)
+        tr = np.zeros(total, dtype=TRACE_DTYPE) if trace else None
+        self.check(lib().mccfr_run_segments(self.h, C.cast(arr, C.c_void_p), len(segs), _ptr(tr) if trace else None))
+        return tr
+
+    def train_tenants(self, tenants, counts, launches: int):
+        t = np.ascontiguousarray(tenants, np.uint32)
+        c = np.ascontiguousarray(counts, np.uint32)
+        self.check(lib().mccfr_train_tenants(self.h, _ptr(t), _ptr(c), t.size, launches))
+
+    def tenant_counters(self, tenant: int):
+        a, b, i, p = C.c_uint64(), C.c_uint64(), C.c_int64(), C.c_int32()
+        self.check(lib().mccfr_tenant_counters(self.h, tenant, C.byref(a), C.byref(b), C.byref(i), C.byref(p)))
+        return int(a.value), int(b.value), int(i.value), int(p.value)
+
+    def tenant_download(self, tenant: int, layout: int = 0):
+        slots, row = self.tenant_shape(tenant)
+        n = slots * row
+        R = np.empty(n, np.float32)
+        S = np.empty(n, np.float32)
+        sig = np.empty(n, np.uint16)
+        self.check(lib().mccfr_tenant_download(self.h, tenant, _ptr(R), _ptr(S), _ptr(sig), n, layout))
+        return R, S, sig
+
+    def tenant_upload(self, tenant: int, R, S, layout: int = 0):
+        slots, row = self.tenant_shape(tenant)
+        n = slots * row
+        Rc = None if R is None else np.ascontiguousarray(R, dtype=np.float32)
+        Sc = None if S is None else np.ascontiguousarray(S, dtype=np.float32)
+        self.check(lib().mccfr_tenant_upload(self.h, tenant, None if Rc is None else _ptr(Rc),
+                                             None if Sc is None else _ptr(Sc), n, layout))
+
+    def tenant_normalized_strategy(self, tenant: int, layout: int = 0) -> np.ndarray:
+        slots, row = self.tenant_shape(tenant)
+        out = np.empty(slots * row, np.float32)
+        self.check(lib().mccfr_tenant_normalized_strategy(self.h, tenant, _ptr(out), out.size, layout))
+        return out
+
+    def tenant_randomize(self, tenant: int, seed: int):
+        self.check(lib().mccfr_tenant_randomize(self.h, tenant, seed))
+
+    def autotune(self, segs, reps: int = 3):
+        arr = (Segment * len(segs))()
+        for i, (t, n, up, w, tid) in enumerate(segs):
+            arr[i] = Segment(t, n, up, 0, w, tid)
+        wg, tpl, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+        self.check(lib().mccfr_autotune(self.h, C.cast(arr, C.c_void_p), len(segs), reps, C.byref(wg),
+                                        C.byref(tpl), C.byref(ms)))
+        return int(wg.value), int(tpl.value), float(ms.value)
+
+
+def tenant_footprint(cfg: GameCfg) -> int:
+    b = C.c_uint64()
+    _check(lib().mccfr_tenant_footprint(C.byref(cfg), C.byref(b)))
+    return int(b.value)
+
+
+WAITING, RUNNING, DONE, RELEASED = range(4)
+
+
+class Server:
+    """mccfr_server: continuous batching of subgame re-solves on a Context."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_running: int = 256):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().mccfr_server_create(ctx.h, hbm_budget, max_running, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mccfr_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, cfg: GameCfg, root: Optional[RootState], iterations: int, batch: int) -> int:
+        r = C.c_uint64()
+        self.ctx.check(lib().mccfr_server_submit(self.h, C.byref(cfg), C.byref(root) if root is not None else None,
+                                                 iterations, batch, C.byref(r)))
+        return int(r.value)
+
+    def step(self, wait: bool = False):
+        a, b = C.c_uint32(), C.c_uint32()
+        self.ctx.check(lib().mccfr_server_step(self.h, int(wait), C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
+
+    def poll(self, req: int):
+        st, tr = C.c_int(), C.c_uint64()
+        self.ctx.check(lib().mccfr_server_poll(self.h, req, C.byref(st), C.byref(tr)))
+        return int(st.value), int(tr.value)
+
+    def tenant(self, req: int) -> int:
+        t = C.c_uint32()
+        self.ctx.check(lib().mccfr_server_tenant(self.h, req, C.byref(t)))
+        return int(t.value)
+
+    def result(self, req: int, layout: int = 0) -> np.ndarray:
+        slots, row = self.ctx.tenant_shape(self.tenant(req))
+        out = np.empty(slots * row, np.float32)
+        self.ctx.check(lib().mccfr_server_result(self.h, req, _ptr(out), out.size, layout))
+        return out
+
+    def release(self, req: int):
+        self.ctx.check(lib().mccfr_server_release(self.h, req))
+
+
+class Trainer:
+    """GPUCFRTrainer (poker_cuda/src/cfr_gpu.cuh:48-132) over the C ABI:
+    the reference's constructor arguments and defaults, load_hand_table,
+    train, get_strategy, export/import_tables, num_info_sets_active."""
+
+    def __init__(self, num_players=2, starting_stack=1000, small_blind=10, big_blind=20, use_cfr_plus=True,
+                 use_linear_cfr=True, table_log2=26, batch_cap=65536):
+        self.cfg = reference_cfg(num_players, starting_stack, small_blind, big_blind, use_cfr_plus, use_linear_cfr,
+                                 table_log2=table_log2)
+        self.ctx = Context(self.cfg, batch_cap)
+
+    def load_hand_table(self, path: Optional[str] = None) -> bool:
+        try:
+            if path is None:
+                self.ctx.use_generated_hand_table()
+            else:
+                self.ctx.load_hand_table(path)
+        except MccfrError:
+            return False
+        return True
+
+    def train(self, total_iterations: int, batch_size: int = 65536):
+        if batch_size > self.ctx.max_batch:
+            self.ctx.set_max_batch(batch_size)
+        self.ctx.train(total_iterations, batch_size)
+
+    def get_strategy(self):
+        """Rows whose strategy sum is >= 1e-6 (cfr_gpu.cu:1383-1412): {slot: probs}."""
+        _, S, _ = self.ctx.download()
+        S = S.reshape(-1, self.ctx.row)
+        P = self.ctx.normalized_strategy().reshape(-1, self.ctx.row)
+        keep = np.nonzero(S.sum(axis=1) >= 1e-6)[0]
+        return {int(i): P[i] for i in keep}
+
+    def num_info_sets_active(self) -> int:
+        _, S, _ = self.ctx.download()
+        return int(np.count_nonzero(S.reshape(-1, self.ctx.row).sum(axis=1) > 0))
+
+    def export_tables(self):
+        R, S, _ = self.ctx.download(layout=1)
+        return R, S
+
+    def import_tables(self, R, S):
+        self.ctx.upload(R, S, layout=1)
+
+    def save_checkpoint(self, path: str) -> bool:
+        try:
+            self.ctx.save_checkpoint(path)
+        except MccfrError:
+            return False
+        return True
+
+    def load_checkpoint(self, path: str) -> bool:
+        try:
+            self.ctx.load_checkpoint(path)
+        except MccfrError:
+            return False
+        return True
