@@ -192,6 +192,10 @@ int mccfr_counters(const mccfr_ctx* ctx, uint64_t* traversals, uint64_t* launche
 int mccfr_set_timing(mccfr_ctx* ctx, int enabled);
 int mccfr_kernel_times(mccfr_ctx* ctx, float* traverse_ms, float* update_ms, uint32_t n);
 
+/* Work of the last launch (waits for it): update-player records sorted,
+ * runs (table slots) longer than 16 records, and their 1024-record pieces. */
+int mccfr_last_launch_stats(mccfr_ctx* ctx, uint32_t* records, uint32_t* long_runs, uint32_t* pieces);
+
 /* Device copy of `bytes` bytes between two fresh buffers, `reps` times
  * (calibration of the HBM byte counters; returns ms per copy). */
 int mccfr_copy_probe(mccfr_ctx* ctx, size_t bytes, int reps, float* ms_per_copy);

@@ -95,7 +95,7 @@ struct LaunchCounters {
   uint32_t n_rec;        // update-player records emitted
   uint32_t n_long;       // runs handed to the long-run reduction
   uint32_t n_pieces;     // pieces of the long runs
-  uint32_t pad;
+  uint32_t n_cand;       // runs whose end the per-wave pass could not see
 };
 
 // ---------------------------------------------------------------------------
@@ -232,12 +232,15 @@ __host__ __device__ inline uint32_t valid_mask(const DevGame& g, int pot, int st
   const int mrt = min_raise_to(current_bet, lfr, g.bb);
   const int maxrt = current_bet + head;
   if (maxrt >= mrt) {
+    // Given mrt <= maxrt, a raise's target max(current_bet + max(1, num*pot/den), mrt)
+    // fits iff max(1, floor(num*pot/den)) <= head, i.e. num*pot < (head + 1)*den
+    // (head >= 1 here): no division needed.
     uint32_t rm = g.raise_mask;
     while (rm) {
       const int a = __builtin_ctz(rm);
       rm &= rm - 1;
-      const int target = g.kind[a] == MCCFR_ACT_RAISE_MIN ? mrt : frac_target(g, a, pot, current_bet, mrt);
-      if (target <= maxrt) mask |= 1u << a;
+      const bool fits = g.kind[a] == MCCFR_ACT_RAISE_MIN || (int)g.num[a] * pot < (head + 1) * (int)g.den[a];
+      if (fits) mask |= 1u << a;
     }
   }
   return mask | g.allin_mask;

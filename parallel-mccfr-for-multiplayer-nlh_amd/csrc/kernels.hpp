@@ -30,10 +30,11 @@ struct DevScratch {
   uint8_t* root_dealer;  // [lanes]
   uint32_t* blk_cnt;     // [lanes / kEmitLanes] records per emission block, then their offsets
   uint32_t* key[2];      // [kMaxDepth * lanes] record keys (segment << slot_bits | slot), ping-pong
-  uint32_t* val[2];      // [kMaxDepth * lanes] record references (lane << 5 | node index)
+  uint2* val[2];         // [kMaxDepth * lanes] record payloads: (action | legal mask << 8, utility bits)
   uint32_t* hist;        // [kSortBins][max_tiles] tile digit counts, then per-digit prefixes over tiles
   uint32_t* bin_total;   // [kSortBins] keys per digit in the current pass
-  uint4* long_runs;      // [max_long] long runs: first record, records, first piece
+  uint32_t* cand;        // [max_cand] first records of runs k_reduce leaves to k_reduce_cand
+  uint4* long_runs;      // [max_long] long runs: first record, records, first piece, pieces
   uint32_t* piece_run;   // [max_pieces] long run of each piece
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
   LaunchCounters* cnt;

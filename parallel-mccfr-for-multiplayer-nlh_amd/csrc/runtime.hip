@@ -82,12 +82,13 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   if (e == hipSuccess) e = hipMalloc(&s.blk_cnt, ((L + kEmitLanes - 1) / kEmitLanes) * sizeof(uint32_t));
   for (int k = 0; k < 2; ++k) {
     if (e == hipSuccess) e = hipMalloc(&s.key[k], recs * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&s.val[k], recs * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s.val[k], recs * sizeof(uint2));
   }
   if (e == hipSuccess) e = hipMalloc(&s.hist, (size_t)kSortBins * s.max_tiles * sizeof(uint32_t));
   const size_t max_long = recs / (kShortRun + 1) + 1;
   const size_t max_pieces = recs / kPiece + max_long;
   if (e == hipSuccess) e = hipMalloc(&s.bin_total, kSortBins * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.cand, (recs / 16 + 64) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.long_runs, max_long * sizeof(uint4));
   if (e == hipSuccess) e = hipMalloc(&s.piece_run, max_pieces * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.partial, max_pieces * 32 * sizeof(double));
@@ -98,7 +99,7 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
 
 void free_scratch(DevScratch& s) {
   void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_dealer,
-                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.long_runs,
+                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.long_runs,
                   s.piece_run, s.partial, s.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1093,6 +1094,17 @@ int mccfr_kernel_times(mccfr_ctx* c, float* trav_ms, float* upd_ms, uint32_t n) 
     if (trav_ms) trav_ms[i] = a;
     if (upd_ms) upd_ms[i] = b;
   }
+  return MCCFR_OK;
+}
+
+int mccfr_last_launch_stats(mccfr_ctx* c, uint32_t* records, uint32_t* long_runs, uint32_t* pieces) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  LaunchCounters h{};
+  HIPX(c, hipStreamSynchronize(c->stream));
+  HIPX(c, hipMemcpy(&h, c->sc.cnt, sizeof h, hipMemcpyDeviceToHost));
+  if (records) *records = h.n_rec;
+  if (long_runs) *long_runs = h.n_long;
+  if (pieces) *pieces = h.n_pieces;
   return MCCFR_OK;
 }
 

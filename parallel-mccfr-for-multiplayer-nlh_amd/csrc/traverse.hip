@@ -11,7 +11,9 @@
 //                 one path.  Lanes keep the game state in LDS and record the
 //                 update-player nodes (key, sampled action, legal mask) and
 //                 the terminal utility.
-//   k_emit_*      records in lane order -> (key, reference) arrays.
+//   k_emit_*      records in lane order -> (key, payload) arrays; the payload
+//                 carries the record's action word and utility, so the table
+//                 update reads records in sorted order without gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
 //                 (key = segment << slot_bits | slot), 8-bit digits.
 //   k_reduce      every run of equal keys is one table slot of one tenant:
@@ -155,16 +157,60 @@ __device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, 
 }
 
 // ---------------------------------------------------------------------------
-// k_walk.  Wave w covers lanes [w*64*tpl, (w+1)*64*tpl); lane l walks the
-// traversals at lanes w*64*tpl + j*64 + l, j < tpl, one after another.
+// k_walk.  Wave w covers lanes [w*64*tpl, (w+1)*64*tpl); in round j its lane
+// l walks the traversal at lane w*64*tpl + j*64 + l.
+//
+// Per-lane state: seat stacks and bets in LDS (field-major: lane-consecutive
+// banks), a bet counts only while its 2-bit street tag matches the street, so
+// a street change writes nothing; the action queue is a seat mask plus the
+// seat to search from (the reference's queue, cfr_gpu.cu:575-585, lists the
+// same seats in the same cyclic order).  The loop only runs decision nodes
+// and street changes; showdowns and the result stores run once per round for
+// the whole wave.
 // ---------------------------------------------------------------------------
-template <int ROW, int WG>
+constexpr uint32_t kStackMask = 0x7FFFu;  // chips behind: bits 0..14; bet: 15..29; street tag: 30..31
+
+// Betting rules of one tenant held in registers (valid_mask restated over the
+// wave's LDS menu; same legality as device_common.hpp valid_mask).
+struct Rules {
+  uint32_t check_mask, call_mask, fold_mask, allin_mask, raise_mask;
+  int max_raises, bb;
+};
+
+__device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint8_t* kind, const uint8_t* num,
+                                               const uint8_t* den, int pot, int stack, int to_call, int current_bet,
+                                               int lfr, int raises) {
+  if (pot <= 0) pot = 1;
+  uint32_t mask;
+  if (to_call == 0) {
+    mask = g.check_mask;
+  } else if (to_call >= stack) {
+    return g.fold_mask | g.allin_mask;
+  } else {
+    mask = g.fold_mask | g.call_mask;
+  }
+  const int head = stack - to_call;
+  if (head <= 0) return mask;
+  if (g.max_raises > 0 && raises >= g.max_raises) return mask;
+  const int mrt = min_raise_to(current_bet, lfr, g.bb);
+  if (current_bet + head >= mrt) {
+    // max(current_bet + max(1, num*pot/den), mrt) <= current_bet + head
+    //   <=> num*pot < (head + 1)*den, given mrt <= current_bet + head
+    uint32_t rm = g.raise_mask;
+    while (rm) {
+      const int a = __builtin_ctz(rm);
+      rm &= rm - 1;
+      if (kind[a] == MCCFR_ACT_RAISE_MIN || (int)num[a] * pot < (head + 1) * (int)den[a]) mask |= 1u << a;
+    }
+  }
+  return mask | g.allin_mask;
+}
+
+template <int ROW, int WG, bool TRACE>
 __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                              const DevGame* __restrict__ tenants, DevScratch sc, uint32_t tpl,
                                              mccfr_trace* __restrict__ trace) {
-  // Per-lane state, field-major ([field][lane]): any per-lane field index
-  // maps consecutive lanes to consecutive banks.
-  __shared__ uint32_t s_sb[kMaxPlayers * WG];    // stack | bet << 16
+  __shared__ uint32_t s_sb[kMaxPlayers * WG];    // stack | bet << 15 | street << 30
   __shared__ uint32_t s_bk[kMaxPlayers * WG];    // buckets: hole | flop << 8 | turn << 16 | river << 24
   __shared__ uint16_t s_rk[kMaxPlayers * WG];    // 7-card rank at showdown
   __shared__ uint32_t s_ps[kRegPending * WG];    // pending update nodes: key
@@ -193,243 +239,271 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   uint32_t* ps = s_ps + tx;
   uint32_t* pa = s_pa + tx;
 
+  // the tenant's constants, once, in registers
+  Rules rules;
+  rules.check_mask = G.check_mask;
+  rules.call_mask = G.call_mask;
+  rules.fold_mask = G.fold_mask;
+  rules.allin_mask = G.allin_mask;
+  rules.raise_mask = G.raise_mask;
+  rules.max_raises = G.max_raises;
+  rules.bb = G.bb;
+  const int N = G.N;
+  const int A = G.num_actions;
+  const int small_blind = G.sb;
+  const int hist_bits = G.hist_bits;
+  const int root_street = G.root_street;
+  const uint32_t root_folded = G.root_folded, root_allin = G.root_allin, root_ab = G.root_ab;
+  const int root_pot = G.root_pot;
+  const uint32_t walk_k0 = G.walk_k0;
+  const uint32_t seed_lo = G.seed_lo, seed_hi = G.seed_hi, tag = G.tag;
+  const int n_passive = G.n_passive;
+  uint64_t passive = 0;  // fallback preference list, 4 bits per entry
+  for (int i = 0; i < n_passive; ++i) passive |= (uint64_t)(G.passive_order[i] & 15u) << (4 * i);
+  const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
+  const uint16_t* __restrict__ sig = G.sig;
   const uint32_t seg_key = uniform((uint32_t)((uint64_t)si << lc.slot_bits));
   const uint32_t lane0 = Sg.lane0, seg_n = Sg.n;
   const int up = Sg.update_player;
+  const int up_stack0 = G.stack[up];
   const uint64_t tid0 = Sg.first_tid;
-  const int N = G.N;
-  const int A = G.num_actions;
-  const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
-  const uint16_t* __restrict__ sig = G.sig;
 
-  uint32_t j = 0;
-  uint32_t t = base + (uint32_t)lane;
-  bool live = t - lane0 < seg_n;
+  for (uint32_t j = 0; j < tpl; ++j) {
+    const uint32_t t = base + j * 64u + (uint32_t)lane;
+    const bool active = t - lane0 < seg_n;
+    if (!__any(active)) break;
 
-  DrawStream rng;
-  uint32_t folded = 0, all_in = 0, ab = 0, queue = 0, phash = 0;
-  int pot = 0, current_bet = 0, lfr = 0, street = 0, raises = 0, depth = 0, nq = 0, dealer = 0;
-  int npend = 0, decisions = 0, upd_legal = 0;
+    DrawStream rng;
+    uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
+    int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
+    int npend = 0, decisions = 0, upd_legal = 0;
+    int status = 4;  // 0 walking, 1 fold, 2 showdown, 3 depth limit, 4 no traversal
+    float u = 0.0f;
 
-  // Root state of traversal t: the reference's preflop root (cfr_gpu.cu:954-981:
-  // stacks, blinds, preflop queue from the seat after the big blind) or the
-  // tenant's postflop root (queue from the seat after the dealer, as at a
-  // street change, cfr_gpu.cu:676-686).
-  auto init = [&]() {
+    // Root state (cfr_gpu.cu:954-981: stacks, blinds, preflop queue from the
+    // seat after the big blind; or the tenant's postflop root with the queue
+    // from the seat after the dealer, as at a street change, cfr_gpu.cu:676-686).
+    if (active) {
 #pragma unroll
-    for (int p = 0; p < kMaxPlayers; ++p) {
-      if (p < N) {
-        sb[p * WG] = (uint32_t)G.root_stack[p];
-        bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
-        rk[p * WG] = sc.root_rank[(size_t)p * L + t];
-      }
-    }
-    dealer = sc.root_dealer[t];
-    queue = 0;
-    nq = 0;
-    if (G.root_street == 0) {
-      const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
-      const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
-      folded = 0;
-      all_in = 0;
-      const int s_sb = (int)(sb[sb_pos * WG] & 0xFFFFu);
-      const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
-      sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 16);
-      if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
-      const int s_bb = (int)(sb[bb_pos * WG] & 0xFFFFu);
-      const int bb_amt = G.bb < s_bb ? G.bb : s_bb;
-      sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 16);
-      if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
-      pot = sb_amt + bb_amt;
-      current_bet = bb_amt;
-      street = 0;
-      ab = 0;
-      for (int k = 0, p = wrap(bb_pos + 1, N); k < N; ++k, p = wrap(p + 1, N))
-        if (!(all_in >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
-    } else {
-      folded = G.root_folded;
-      all_in = G.root_allin;
-      pot = G.root_pot;
-      current_bet = 0;
-      street = G.root_street;
-      ab = G.root_ab;
-      for (int k = 0, p = wrap(dealer + 1, N); k < N; ++k, p = wrap(p + 1, N))
-        if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
-    }
-    lfr = G.bb;
-    raises = 0;
-    depth = 0;
-    rng.init(G.seed_lo, G.seed_hi, tid0 + (t - lane0), G.tag);
-    rng.k = G.walk_k0;  // the deal (and the preflop dealer draw) came first
-    npend = 0;
-    decisions = 0;
-    upd_legal = 0;
-    phash = 2166136261u;
-  };
-
-  auto finish = [&](float u, int term) {
-    if (trace) {
-      mccfr_trace r;
-      r.utility = u;
-      r.path_hash = phash;
-      r.decisions = (uint16_t)decisions;
-      r.updates = (uint8_t)npend;
-      r.terminal = (uint8_t)term;
-      r.update_legal = (uint16_t)upd_legal;
-      r.reserved = 0;
-      trace[t] = r;
-    }
-    sc.node_count[t] = (uint8_t)npend;
-    sc.node_u[t] = u;
-    const int kr = npend < kRegPending ? npend : kRegPending;
-    for (int k = 0; k < kr; ++k) {
-      sc.node_slot[(size_t)k * L + t] = ps[k * WG];
-      sc.node_av[(size_t)k * L + t] = pa[k * WG];
-    }
-    ++j;
-    t += 64;
-    live = j < tpl && t - lane0 < seg_n;
-    if (live) init();
-  };
-
-  if (live) init();
-  while (!__all(!live)) {
-    if (!live) continue;
-    // one recursion level of traverse_gpu (cfr_gpu.cu:644-860)
-    if (depth >= kMaxDepth) {
-      finish(0.0f, 3);
-      continue;
-    }
-    const uint32_t alive = allmask & ~folded;
-    if (__popc(alive) == 1) {
-      const int inv_up = G.stack[up] - (int)(sb[up * WG] & 0xFFFFu);
-      finish(((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up), 1);
-      continue;
-    }
-    if (nq == 0) {
-      if (street < 3) {
-        ++street;
-        for (int p = 0; p < N; ++p) sb[p * WG] &= 0xFFFFu;
-        current_bet = 0;
-        lfr = G.bb;
-        raises = 0;
-        for (int k = 0, p = wrap(dealer + 1, N); k < N; ++k, p = wrap(p + 1, N))
-          if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
-        if (nq > 0) {
-          ++depth;
-          continue;
+      for (int p = 0; p < kMaxPlayers; ++p) {
+        if (p < N) {
+          sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
+          bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
+          rk[p * WG] = sc.root_rank[(size_t)p * L + t];
         }
       }
-      // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending order
-      const int inv_up = G.stack[up] - (int)(sb[up * WG] & 0xFFFFu);
+      dealer = sc.root_dealer[t];
+      if (root_street == 0) {
+        const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
+        const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
+        const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
+        const int sb_amt = small_blind < s_sb ? small_blind : s_sb;
+        sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
+        if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
+        const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
+        const int bb_amt = rules.bb < s_bb ? rules.bb : s_bb;
+        sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 15);
+        if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
+        pot = sb_amt + bb_amt;
+        current_bet = bb_amt;
+        qmask = allmask & ~all_in;
+        qpos = wrap(bb_pos + 1, N);
+      } else {
+        folded = root_folded;
+        all_in = root_allin;
+        pot = root_pot;
+        street = root_street;
+        ab = root_ab;
+        qmask = allmask & ~(folded | all_in);
+        qpos = wrap(dealer + 1, N);
+      }
+      rng.init(seed_lo, seed_hi, tid0 + (t - lane0), tag);
+      rng.k = walk_k0;  // the deal (and the preflop dealer draw) came first
+      status = 0;
+    }
+
+    while (__any(status == 0)) {
+      if (status != 0) continue;
+      // one recursion level of traverse_gpu (cfr_gpu.cu:644-860)
+      if (depth >= kMaxDepth) {
+        status = 3;
+        continue;
+      }
+      const uint32_t alive = allmask & ~folded;
+      if (__popc(alive) == 1) {
+        const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
+        u = ((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
+        status = 1;
+        continue;
+      }
+      if (qmask == 0) {
+        if (street < 3) {  // street change (cfr_gpu.cu:668-692)
+          ++street;
+          current_bet = 0;
+          lfr = rules.bb;
+          raises = 0;
+          qmask = allmask & ~(folded | all_in);
+          qpos = wrap(dealer + 1, N);
+          if (qmask) {
+            ++depth;
+            continue;
+          }
+        }
+        status = 2;  // showdown (nobody left to act: straight there, cfr_gpu.cu:688-694)
+        continue;
+      }
+
+      // decision node: the first queued seat at or after qpos
+      const uint32_t rot = ((qmask >> qpos) | (qmask << (N - qpos))) & allmask;
+      const int player = wrap(qpos + (int)__builtin_ctz(rot), N);
+      const uint32_t sbv = sb[player * WG];
+      const int stk = (int)(sbv & kStackMask);
+      const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+      const int to_call = current_bet - bet;
+      const uint32_t vm = rules_mask(rules, m_kind, m_num, m_den, pot, stk, to_call, current_bet, lfr, raises);
+      const uint32_t Bp = bk[player * WG];
+      const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
+      const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
+      float sg[ROW];
+      load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
+      // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
+      float total = 0.0f;
+#pragma unroll
+      for (int a = 0; a < ROW; ++a)
+        if (a < A && (vm >> a & 1u)) total += sg[a];
+      int act = -1;
+      if (total >= 1e-7f) {
+        const float r = u01(rng.next()) * total;
+        float cum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < ROW; ++a) {
+          const bool valid = a < A && (vm >> a & 1u);
+          if (valid) cum += sg[a];
+          if (valid && act < 0 && r <= cum) act = a;
+        }
+      }
+      if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508, 520-521)
+        for (int i = 0; i < n_passive; ++i) {
+          const int a = (int)((passive >> (4 * i)) & 15u);
+          if (vm >> a & 1u) {
+            act = a;
+            break;
+          }
+        }
+        if (act < 0) act = 0;
+      }
+      if (TRACE) {
+        phash = fnv_path(phash, slot, (uint32_t)act);
+        ++decisions;
+      }
+      if (player == up) {
+        const uint32_t av = (uint32_t)act | (vm << 8);
+        const uint32_t key = seg_key | slot;
+        if (npend < kRegPending) {
+          ps[npend * WG] = key;
+          pa[npend * WG] = av;
+        } else {
+          sc.node_slot[(size_t)npend * L + t] = key;
+          sc.node_av[(size_t)npend * L + t] = av;
+        }
+        ++npend;
+        if (TRACE) upd_legal += __popc(vm);
+      }
+      // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
+      qmask &= ~(1u << player);
+      qpos = wrap(player + 1, N);
+      const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
+      ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
+      if (rules.fold_mask >> act & 1u) {
+        folded |= 1u << player;
+      } else {
+        const int ns = stk - chips, nb = bet + chips;
+        sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 15) | ((uint32_t)street << 30);
+        pot += chips;
+        if (ns == 0) all_in |= 1u << player;
+        if (nb > current_bet) {
+          const int raise = nb - current_bet;
+          if (raise >= lfr || current_bet == 0) lfr = raise;
+          current_bet = nb;
+          ++raises;
+          qmask = allmask & ~(folded | all_in) & ~(1u << player);
+        }
+      }
+      ++depth;
+    }
+
+    // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending
+    // order, for every lane of the wave that reached one
+    if (__any(status == 2) && status == 2) {
+      int inv[kMaxPlayers];
+      uint32_t rkv[kMaxPlayers];
+#pragma unroll
+      for (int p = 0; p < kMaxPlayers; ++p) {
+        inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
+        rkv[p] = p < N ? (uint32_t)rk[p * WG] : 0u;
+      }
+      const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
+      const uint32_t rk_up = rk[up * WG];
+      const bool up_in = !(folded >> up & 1u);
       float pay = -(float)inv_up;
       int prev = 0;
-      for (;;) {
+#pragma unroll
+      for (int it = 0; it < kMaxPlayers; ++it) {
         int lvl = 0x7FFFFFFF;
-        for (int p = 0; p < N; ++p) {
-          const int inv = G.stack[p] - (int)(sb[p * WG] & 0xFFFFu);
-          if (inv > prev && inv < lvl) lvl = inv;
+#pragma unroll
+        for (int p = 0; p < kMaxPlayers; ++p)
+          if (inv[p] > prev && inv[p] < lvl) lvl = inv[p];
+        if (lvl != 0x7FFFFFFF) {
+          int layer_pot = 0;
+          uint32_t best = 0;
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p) {
+            const int c = (inv[p] < lvl ? inv[p] : lvl) - prev;
+            layer_pot += c > 0 ? c : 0;
+            if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] > best) best = rkv[p];
+          }
+          if (best > 0) {
+            int n_win = 0;
+#pragma unroll
+            for (int p = 0; p < kMaxPlayers; ++p)
+              if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] == best) ++n_win;
+            if (up_in && inv_up >= lvl && rk_up == best) pay += (float)layer_pot / (float)n_win;
+          }
+          prev = lvl;
         }
-        if (lvl == 0x7FFFFFFF) break;
-        int layer_pot = 0;
-        uint32_t best = 0;
-        for (int p = 0; p < N; ++p) {
-          const int inv = G.stack[p] - (int)(sb[p * WG] & 0xFFFFu);
-          const int c = (inv < lvl ? inv : lvl) - prev;
-          layer_pot += c > 0 ? c : 0;
-          const uint32_t r = rk[p * WG];
-          if (!(folded >> p & 1u) && inv >= lvl && r > best) best = r;
-        }
-        if (best > 0) {
-          int n_win = 0;
-          for (int p = 0; p < N; ++p)
-            if (!(folded >> p & 1u) && G.stack[p] - (int)(sb[p * WG] & 0xFFFFu) >= lvl && rk[p * WG] == best)
-              ++n_win;
-          if (!(folded >> up & 1u) && inv_up >= lvl && rk[up * WG] == best) pay += (float)layer_pot / (float)n_win;
-        }
-        prev = lvl;
       }
-      finish(pay, 2);
-      continue;
+      u = pay;
     }
 
-    // decision node
-    const int player = (int)(queue & 7u);
-    const uint32_t sbv = sb[player * WG];
-    const int stk = (int)(sbv & 0xFFFFu), bet = (int)(sbv >> 16);
-    const int to_call = current_bet - bet;
-    const uint32_t vm = valid_mask(G, pot, stk, to_call, current_bet, lfr, raises);
-    const uint32_t Bp = bk[player * WG];
-    const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
-    const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
-    float sg[ROW];
-    load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
-    // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
-    float total = 0.0f;
-#pragma unroll
-    for (int a = 0; a < ROW; ++a)
-      if (a < A && (vm >> a & 1u)) total += sg[a];
-    int act = -1;
-    if (total < 1e-7f) {
-      act = passive_fallback(G, vm);
-    } else {
-      const float r = u01(rng.next()) * total;
-      float cum = 0.0f;
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        const bool valid = a < A && (vm >> a & 1u);
-        if (valid) cum += sg[a];
-        if (valid && act < 0 && r <= cum) act = a;
+    if (active) {
+      if (status == 3) u = 0.0f;  // depth guard (cfr_gpu.cu:661)
+      if (TRACE) {
+        mccfr_trace r;
+        r.utility = u;
+        r.path_hash = phash;
+        r.decisions = (uint16_t)decisions;
+        r.updates = (uint8_t)npend;
+        r.terminal = (uint8_t)status;
+        r.update_legal = (uint16_t)upd_legal;
+        r.reserved = 0;
+        trace[t] = r;
       }
-      if (act < 0) act = passive_fallback(G, vm);
-    }
-    phash = fnv_path(phash, slot, (uint32_t)act);
-    ++decisions;
-    if (player == up) {
-      const uint32_t av = (uint32_t)act | (vm << 8);
-      const uint32_t key = seg_key | slot;
-      if (npend < kRegPending) {
-        ps[npend * WG] = key;
-        pa[npend * WG] = av;
-      } else {
-        sc.node_slot[(size_t)npend * L + t] = key;
-        sc.node_av[(size_t)npend * L + t] = av;
-      }
-      ++npend;
-      upd_legal += __popc(vm);
-    }
-    // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
-    queue >>= 3;
-    --nq;
-    const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
-    ab = ((ab << G.hist_bits) | (uint32_t)act) & kHistoryMask;
-    if (G.fold_mask >> act & 1u) {
-      folded |= 1u << player;
-    } else {
-      const int ns = stk - chips, nb = bet + chips;
-      sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 16);
-      pot += chips;
-      if (ns == 0) all_in |= 1u << player;
-      if (nb > current_bet) {
-        const int raise = nb - current_bet;
-        if (raise >= lfr || current_bet == 0) lfr = raise;
-        current_bet = nb;
-        ++raises;
-        queue = 0;
-        nq = 0;
-        for (int k = 1, p = wrap(player + 1, N); k < N; ++k, p = wrap(p + 1, N))
-          if (!((folded | all_in) >> p & 1u)) { queue |= (uint32_t)p << (3 * nq); ++nq; }
+      sc.node_count[t] = (uint8_t)npend;
+      sc.node_u[t] = u;
+      const int kr = npend < kRegPending ? npend : kRegPending;
+      for (int k = 0; k < kr; ++k) {
+        sc.node_slot[(size_t)k * L + t] = ps[k * WG];
+        sc.node_av[(size_t)k * L + t] = pa[k * WG];
       }
     }
-    ++depth;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Record emission: lane-ordered (key, lane << 5 | node) pairs.  Blocks of
-// kEmitLanes lanes, 4 lanes per thread.
+// Record emission: lane-ordered (key, (action | mask << 8, utility)) pairs.
+// Blocks of kEmitLanes lanes, 4 lanes per thread.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -500,6 +574,7 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc)
     sc.cnt->n_rec = tot;
     sc.cnt->n_long = 0;
     sc.cnt->n_pieces = 0;
+    sc.cnt->n_cand = 0;
   }
 }
 
@@ -514,9 +589,10 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
   for (int q = 0; q < 4; ++q) {
     const uint32_t t = l0 + (uint32_t)q;
     const uint32_t c = (w4 >> (8 * q)) & 0xFFu;
+    const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
     for (uint32_t k = 0; k < c; ++k, ++pos) {
       sc.key[0][pos] = sc.node_slot[(size_t)k * L + t];
-      sc.val[0][pos] = (t << 5) | k;
+      sc.val[0][pos] = make_uint2(sc.node_av[(size_t)k * L + t], ub);
     }
   }
 }
@@ -593,8 +669,8 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, ui
   if (threadIdx.x == 0) bin_total[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+__global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint2* __restrict__ vin,
+                                                        uint32_t* __restrict__ kout, uint2* __restrict__ vout,
                                                         const LaunchCounters* __restrict__ cnt, uint32_t shift,
                                                         const uint32_t* __restrict__ hist,
                                                         const uint32_t* __restrict__ bin_total, uint32_t max_tiles) {
@@ -614,14 +690,15 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
     for (int k = 0; k < kWaves; ++k) s_wc[k][tx] = 0;
     __syncthreads();
     // wave w holds keys [wbase, wbase + 64 * kSortItems) of the tile, item-major
-    uint32_t kk[kSortItems], vv[kSortItems], off[kSortItems];
+    uint32_t kk[kSortItems], off[kSortItems];
+    uint2 vv[kSortItems];
     const uint32_t wbase = tile * kSortTile + (uint32_t)(w * 64 * kSortItems);
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       const bool valid = idx < n;
       kk[i] = valid ? kin[idx] : 0u;
-      vv[i] = valid ? vin[idx] : 0u;
+      vv[i] = valid ? vin[idx] : make_uint2(0u, 0u);
     }
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
@@ -718,10 +795,11 @@ struct RunAcc {
 
 template <int ROW>
 __device__ __forceinline__ void apply_row(float* __restrict__ R, float* __restrict__ S, uint16_t* __restrict__ sig,
-                                          const float (&rr)[ROW], const double (&dr)[ROW], const double (&ds)[ROW],
-                                          bool cfr_plus) {
+                                          const float (&rr)[ROW], const float (&ss)[ROW], const double (&dr)[ROW],
+                                          const double (&ds)[ROW], bool cfr_plus) {
   float s[ROW];
-  load_f32_row<ROW>(S, s);
+#pragma unroll
+  for (int a = 0; a < ROW; ++a) s[a] = ss[a];
   float r[ROW];
   float sum = 0.0f;
 #pragma unroll
@@ -774,17 +852,41 @@ __device__ __forceinline__ RunCtx run_ctx(uint32_t key, uint32_t slot_bits, cons
   return c;
 }
 
+// Reduces a run [i, e) of at most kShortRun records and applies it.
+template <int ROW>
+__device__ __forceinline__ void short_run(uint32_t key, uint32_t i, uint32_t e, const LaunchCfg& lc,
+                                          const DevSeg* __restrict__ segs, const DevGame* __restrict__ tenants,
+                                          const uint2* __restrict__ vals) {
+  const RunCtx c = run_ctx(key, lc.slot_bits, segs, tenants, ROW);
+  float rr[ROW], ss[ROW];
+  load_f32_row<ROW>(c.R, rr);
+  load_f32_row<ROW>(c.S, ss);  // issued together with the regret row
+  RunAcc<ROW> acc;
+  acc.reset();
+  for (uint32_t r = i; r < e;) {
+    // load up to 4 records at once so their loads overlap
+    constexpr int kBatch = 4;
+    uint2 v[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) v[b] = r + b < e ? vals[r + b] : make_uint2(0u, 0u);
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b)
+      if (r + b < e) acc.add(rr, c.A, c.tw, v[b].x, __uint_as_float(v[b].y));
+    r += kBatch;
+  }
+  apply_row<ROW>(c.R, c.S, c.sig, rr, ss, acc.dr, acc.ds, c.cfr_plus);
+}
+
 // Every record is looked at by one lane; the lane holding a run's first
-// record owns the run.  The run's end comes from the wave's head flags
-// (ballot), else from a short forward scan.  Runs of at most kShortRun
-// records are reduced and applied by their owner; longer runs are split into
-// kPiece-record pieces for k_reduce_piece and finished by k_reduce_runs.
+// record owns the run.  A run that ends inside the wave (the wave's head
+// flags, one ballot) with at most kShortRun records is reduced and applied
+// here; the others -- longer, or running past the wave -- are listed for
+// k_reduce_cand, so no lane here waits on a search.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                 const DevGame* __restrict__ tenants, DevScratch sc,
-                                                const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals) {
+                                                const uint32_t* __restrict__ keys, const uint2* __restrict__ vals) {
   const uint32_t n = sc.cnt->n_rec;
-  const size_t L = sc.cap_lanes;
   const int lane = threadIdx.x & 63;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
@@ -793,57 +895,57 @@ __global__ __launch_bounds__(256) void k_reduce(LaunchCfg lc, const DevSeg* __re
     const uint32_t key = valid ? keys[i] : 0u;
     const bool head = valid && (i == 0 || keys[i - 1] != key);
     const uint64_t heads = __ballot(head);
-    if (!head) continue;
     const uint64_t later = heads & ~((2ull << lane) - 1ull);  // heads of higher lanes
-    uint32_t e;
-    if (later) {
-      e = base + (uint32_t)__builtin_ctzll(later);
-    } else {
-      e = min(base + 64u, n);
-      while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
+    const uint32_t e = later ? base + (uint32_t)__builtin_ctzll(later) : min(base + 64u, n);
+    const bool closed = later != 0 || e == n;
+    const bool defer = head && (!closed || e - i > kShortRun);
+    const uint64_t dm = __ballot(defer);
+    if (dm) {  // one atomic per wave for its deferred runs
+      uint32_t b0 = 0;
+      if (lane == (int)__builtin_ctzll(dm)) b0 = atomicAdd(&sc.cnt->n_cand, (uint32_t)__popcll(dm));
+      b0 = __shfl(b0, (int)__builtin_ctzll(dm), 64);
+      if (defer) sc.cand[b0 + (uint32_t)__popcll(dm & ((1ull << lane) - 1ull))] = i;
     }
-    if (e - i > kShortRun) {
-      // exact end: keys[e - 1] == key and the keys ascend
-      uint32_t lo = e, hi = n;
-      while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        if (keys[mid] == key) lo = mid + 1;
-        else hi = mid;
-      }
-      const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
-      const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
-      const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
-      sc.long_runs[r] = make_uint4(i, len, p0, np);
-      for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
+    if (head && !defer) short_run<ROW>(key, i, e, lc, segs, tenants, vals);
+  }
+}
+
+// One thread per deferred run: its end by a short forward scan, then
+// exponential and binary search (the keys ascend).  Short runs are reduced
+// here; long ones are cut into pieces for k_reduce_piece.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce_cand(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                     const DevGame* __restrict__ tenants, DevScratch sc,
+                                                     const uint32_t* __restrict__ keys,
+                                                     const uint2* __restrict__ vals) {
+  const uint32_t n = sc.cnt->n_rec;
+  const uint32_t nc = sc.cnt->n_cand;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    const uint32_t i = sc.cand[c];
+    const uint32_t key = keys[i];
+    uint32_t e = i + 1;
+    while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
+    if (e - i <= kShortRun) {
+      short_run<ROW>(key, i, e, lc, segs, tenants, vals);
       continue;
     }
-    const RunCtx c = run_ctx(key, lc.slot_bits, segs, tenants, ROW);
-    float rr[ROW];
-    load_f32_row<ROW>(c.R, rr);
-    RunAcc<ROW> acc;
-    acc.reset();
-    for (uint32_t r = i; r < e;) {
-      // gather up to 4 records at once so their loads overlap
-      constexpr int kBatch = 4;
-      uint32_t av[kBatch];
-      float uu[kBatch];
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b) {
-        av[b] = 0;
-        uu[b] = 0.0f;
-        if (r + b < e) {
-          const uint32_t ref = vals[r + b];
-          const uint32_t t = ref >> 5, k = ref & 31u;
-          av[b] = sc.node_av[(size_t)k * L + t];
-          uu[b] = sc.node_u[t];
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b)
-        if (r + b < e) acc.add(rr, c.A, c.tw, av[b], uu[b]);
-      r += kBatch;
+    // keys[e - 1] == key: bracket the end, then bisect
+    uint32_t lo = e, step = 32;
+    while (lo + step < n && keys[lo + step - 1] == key) {
+      lo += step;
+      step <<= 1;
     }
-    apply_row<ROW>(c.R, c.S, c.sig, rr, acc.dr, acc.ds, c.cfr_plus);
+    uint32_t hi = min(lo + step, n);
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (keys[mid] == key) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
+    const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
+    const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
+    sc.long_runs[r] = make_uint4(i, len, p0, np);
+    for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
   }
 }
 
@@ -853,10 +955,9 @@ template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                       const DevGame* __restrict__ tenants, DevScratch sc,
                                                       const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ vals) {
+                                                      const uint2* __restrict__ vals) {
   __shared__ double s_red[4][2 * ROW];
   const uint32_t n_pieces = sc.cnt->n_pieces;
-  const size_t L = sc.cap_lanes;
   const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
   for (uint32_t p = blockIdx.x; p < n_pieces; p += gridDim.x) {
     const uint4 run = sc.long_runs[sc.piece_run[p]];
@@ -871,9 +972,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, const DevSeg
     for (uint32_t k = 0; k < kPiece / 256; ++k) {
       const uint32_t idx = start + (uint32_t)tx + 256u * k;
       if (idx < end) {
-        const uint32_t ref = vals[idx];
-        const uint32_t t = ref >> 5, kk = ref & 31u;
-        acc.add(rr, c.A, c.tw, sc.node_av[(size_t)kk * L + t], sc.node_u[t]);
+        const uint2 v = vals[idx];
+        acc.add(rr, c.A, c.tw, v.x, __uint_as_float(v.y));
       }
     }
 #pragma unroll
@@ -909,8 +1009,9 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, const DevSeg*
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_long; r += gridDim.x * blockDim.x) {
     const uint4 run = sc.long_runs[r];
     const RunCtx c = run_ctx(keys[run.x], lc.slot_bits, segs, tenants, ROW);
-    float rr[ROW];
+    float rr[ROW], ss[ROW];
     load_f32_row<ROW>(c.R, rr);
+    load_f32_row<ROW>(c.S, ss);
     double dr[ROW], ds[ROW];
 #pragma unroll
     for (int a = 0; a < ROW; ++a) dr[a] = ds[a] = 0.0;
@@ -922,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, const DevSeg*
         ds[a] += q[ROW + a];
       }
     }
-    apply_row<ROW>(c.R, c.S, c.sig, rr, dr, ds, c.cfr_plus);
+    apply_row<ROW>(c.R, c.S, c.sig, rr, ss, dr, ds, c.cfr_plus);
   }
 }
 
@@ -931,7 +1032,10 @@ void walk_launch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const D
                  const DevScratch& sc, uint32_t tpl, mccfr_trace* trace) {
   const uint32_t waves = lc.lanes / (64u * tpl);
   const dim3 grid((waves * 64u + WG - 1) / WG);
-  hipLaunchKernelGGL((k_walk<ROW, WG>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
+  if (trace)
+    hipLaunchKernelGGL((k_walk<ROW, WG, true>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
+  else
+    hipLaunchKernelGGL((k_walk<ROW, WG, false>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
 }
 
 template <int ROW>
@@ -971,24 +1075,26 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = 8u * p;
     const uint32_t* kin = sc.key[p & 1];
-    const uint32_t* vin = sc.val[p & 1];
+    const uint2* vin = sc.val[p & 1];
     hipLaunchKernelGGL(k_rs_hist, dim3(sgrid), dim3(kSortWG), 0, s, kin, sc.cnt, shift, sc.hist, sc.max_tiles);
     hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(256), 0, s, sc.hist, sc.bin_total, sc.cnt, sc.max_tiles);
     hipLaunchKernelGGL(k_rs_scatter, dim3(sgrid), dim3(kSortWG), 0, s, kin, vin, sc.key[(p + 1) & 1],
                        sc.val[(p + 1) & 1], sc.cnt, shift, sc.hist, sc.bin_total, sc.max_tiles);
   }
   const uint32_t* keys = sc.key[passes & 1];
-  const uint32_t* vals = sc.val[passes & 1];
+  const uint2* vals = sc.val[passes & 1];
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
+#define MCCFR_REDUCE(R)                                                                                       \
+  hipLaunchKernelGGL(k_reduce<R>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);           \
+  hipLaunchKernelGGL(k_reduce_cand<R>, dim3(256), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);        \
+  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);      \
+  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys)
   if (row == 8) {
-    hipLaunchKernelGGL(k_reduce<8>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
-    hipLaunchKernelGGL(k_reduce_piece<8>, dim3(2048), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
-    hipLaunchKernelGGL(k_reduce_runs<8>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys);
+    MCCFR_REDUCE(8);
   } else {
-    hipLaunchKernelGGL(k_reduce<16>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
-    hipLaunchKernelGGL(k_reduce_piece<16>, dim3(2048), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);
-    hipLaunchKernelGGL(k_reduce_runs<16>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys);
+    MCCFR_REDUCE(16);
   }
+#undef MCCFR_REDUCE
 }
 
 }  // namespace mccfr

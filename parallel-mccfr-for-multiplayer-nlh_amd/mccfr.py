@@ -147,6 +147,8 @@ _SIGS = {
     "mccfr_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "mccfr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mccfr_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "mccfr_last_launch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint32)]),
     "mccfr_tenant_footprint": (C.c_int, [C.POINTER(GameCfg), C.POINTER(C.c_uint64)]),
     "mccfr_tenant_create": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.POINTER(RootState), C.POINTER(C.c_uint32)]),
     "mccfr_tenant_destroy": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -405,6 +407,12 @@ class Context:
         b = np.empty(n, np.float32)
         self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), n))
         return a, b
+
+    def last_launch_stats(self):
+        """(records, long runs, pieces) of the last launch."""
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.check(lib().mccfr_last_launch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
 
     def copy_probe(self, nbytes: int, reps: int) -> float:
         ms = C.c_float()
