@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the MCCFR hot path on one MI355X.
 
-Workload (default): 6-max 100bb NLH -- 6 players, 2,000-chip stacks at
-blinds 10/20, menu {fold, check/call, 1/2 pot, pot, 2 pot, all-in}, at most 3
-raises per street; FP32 regret and strategy-sum tables of 2^26 slots; one step =
-one launch of 2^20 traversals (deal, buckets, betting-tree walk, delta scatter,
-touched-slot CFR+ update), update player cycling over the seats as in the
-training loop.  Inputs (tables, hand table) are resident in HBM before timing.
+Headline workload (`value`): 6-max 100bb NLH -- 6 players, 2,000-chip stacks
+at blinds 10/20, menu {fold, check/call, 1/2 pot, pot, 2 pot, all-in}, at most
+3 raises per street; FP32 regret and strategy-sum tables of 2^26 slots; one
+step = one launch of 2^20 traversals (deal, buckets, betting-tree walk, record
+sort, owner-computes CFR+ table update), the update player cycling over the
+seats as in the training loop.  Tables and the hand table are resident in HBM
+before timing.
 
-Prints one JSON line with traversals/s, the traversal kernel's HBM roofline
-fraction (HIP events on the library stream around every timed launch), the
-hand-evaluation microbenchmark (2^28 hands, both evaluators) and the CPU
-baseline (the oracle's FP32 restatement of the reference launch, timed on a
-bounded sample on this host).
+The same JSON line carries:
+  roofline      the walk kernel (the launch's dominant kernel): algorithmic
+                bytes per launch / its mean duration (HIP events around it on
+                the library stream), HBM bytes per launch from the committed
+                rocprofv3 FETCH_SIZE/WRITE_SIZE summary;
+  batched       32 uneven games (2-6 players, 10-200bb, seed 2) sharing
+                2^21-traversal launches from one captured graph (autotuned
+                shape) vs the same games each launched alone;
+  serving       re-solve requests arriving Poisson at 200/s (seeds 3 and 4):
+                p50/p99 latency from arrival to final tables, aggregate
+                traversals/s; and all requests at once, served vs one at a time;
+  hand_eval     2^28 uniform 7-card hands through both evaluators;
+  cpu_baseline  the oracle's FP32 restatement of the reference launch on a
+                bounded sample, on this host's cores.
 """
 import argparse
 import ctypes as C
@@ -35,19 +45,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traversal_bytes(tr, num_players, row):
-    """Algorithmic bytes of a launch (per-traversal figure x traversals):
-    72*N hand-table words (5+6+7 words x 4 B per player), 2*A B per decision
-    node (BF16 strategy row), 4*A B per update node (FP32 regret row) and 16 B
-    per legal action at an update node (read-modify-write of regret and
-    strategy sum, 4+4 B each)."""
+def walk_bytes(tr, num_players, row):
+    """Algorithmic bytes of the walk kernel for one launch (per-traversal figure
+    x traversals): root records in (4 B buckets + 2 B rank per seat, 1 B
+    dealer), one BF16 strategy row per decision node (2 B x padded row), out:
+    one 8-byte record (key, action word) per update-player node and the
+    traversal's 1-byte node count and 4-byte utility."""
+    n = len(tr)
+    d = tr["decisions"].astype(np.float64).sum()
+    u = tr["updates"].astype(np.float64).sum()
+    return (6.0 * num_players + 1.0) * n + 2.0 * row * d + 8.0 * u + 5.0 * n
+
+
+def path_bytes(tr, num_players, row):
+    """The reference path's algorithmic bytes per launch (72 N hand-table
+    words per traversal, a BF16 row per decision, an FP32 regret row and a
+    regret/strategy-sum read-modify-write per legal action at update nodes)."""
     d = tr["decisions"].astype(np.float64).sum()
     u = tr["updates"].astype(np.float64).sum()
     v = tr["update_legal"].astype(np.float64).sum()
     return 72.0 * num_players * len(tr) + 2.0 * row * d + 4.0 * row * u + 16.0 * v
 
 
-def hand_bench(mccfr, ctx, log2n, reps=3):
+def hand_bench(ctx, log2n, reps=3):
     n = 1 << log2n
     dc = ctx.dalloc(n * 8)
     dr = ctx.dalloc(n * 2)
@@ -63,12 +83,12 @@ def hand_bench(mccfr, ctx, log2n, reps=3):
                 ctx.eval7_device(dc, 8, dr, n, ev)
             ctx.sync()
             dt = (time.perf_counter() - t0) / reps
-            out[name] = {"hands_per_s": n / dt, "ms": dt * 1e3,
-                         "gbs_37B_per_hand": 37.0 * n / dt / 1e9}
+            out[name] = {"hands_per_s": n / dt, "ms": dt * 1e3, "gbs_37B_per_hand": 37.0 * n / dt / 1e9}
     finally:
         ctx.dfree(dc)
         ctx.dfree(dr)
     out["hands"] = n
+    out["seed"] = 1
     return out
 
 
@@ -91,6 +111,94 @@ def cpu_baseline(mccfr, ctx, cfg, launch, tid, batch, launches, threads):
                       f"{secs:.1f} s"}
 
 
+def batched_bench(mccfr, workloads, hr, launches, n_games=32, table_log2=20, total=1 << 21):
+    """Uneven games in shared launches vs each game in its own context."""
+    games, counts = workloads.uneven_mix(seed=2, n_games=n_games, table_log2=table_log2, traversals=total)
+    lanes = int(sum((int(c) + 255) // 256 * 256 for c in counts))
+    small = mccfr.reference_cfg(num_players=2, starting_stack=1000, table_log2=4)
+    out = {"games": n_games, "traversals_per_launch": total, "table_slots_per_game": 1 << table_log2,
+           "seed": 2, "launches": launches}
+    with mccfr.Context(small, lanes) as ctx:
+        ctx.set_hand_table(hr)
+        ids = [ctx.tenant_create(g.cfg) for g in games]
+        segs = [(t, int(c), 0, 1, 0) for t, c in zip(ids, counts)]
+        wg, tpl, ms = ctx.autotune(segs, reps=2)
+        out["autotuned_shape"] = {"walk_wg": wg, "traversals_per_lane": tpl, "walk_ms": ms}
+        ctx.train_tenants(ids, counts, 2)  # capture + warm-up
+        t0 = time.perf_counter()
+        ctx.train_tenants(ids, counts, launches)
+        dt = time.perf_counter() - t0
+    out["batched_traversals_per_s"] = launches * total / dt
+    ctxs = []
+    try:
+        for g, c in zip(games, counts):
+            cx = mccfr.Context(g.cfg, int(c))
+            cx.set_hand_table(hr)
+            cx.train_tenants([0], [int(c)], 1)  # capture + warm-up
+            ctxs.append(cx)
+        t0 = time.perf_counter()
+        for cx, c in zip(ctxs, counts):
+            cx.train_tenants([0], [int(c)], launches)
+        dt = time.perf_counter() - t0
+    finally:
+        for cx in ctxs:
+            cx.close()
+    out["plain_traversals_per_s"] = launches * total / dt
+    out["speedup"] = out["batched_traversals_per_s"] / out["plain_traversals_per_s"]
+    return out
+
+
+def serve_run(mccfr, hr, reqs, budget, max_running, poisson):
+    """Runs requests through a server; with poisson=True each is submitted at
+    its arrival time, else all at once.  Returns (latencies s, traversals, wall s)."""
+    small = mccfr.reference_cfg(num_players=2, starting_stack=1000, table_log2=4)
+    lat, trav = [], 0
+    with mccfr.Context(small, 1 << 20) as ctx:
+        ctx.set_hand_table(hr)
+        with mccfr.Server(ctx, budget, max_running) as srv:
+            n, i = len(reqs), 0
+            pending = {}
+            t0 = time.perf_counter()
+            while len(lat) < n:
+                now = time.perf_counter() - t0
+                while i < n and (not poisson or reqs[i].arrival <= now):
+                    r = reqs[i]
+                    rid = srv.submit(r.cfg, r.root, r.iterations, r.batch)
+                    pending[rid] = r.arrival if poisson else 0.0
+                    i += 1
+                running, waiting = srv.step(wait=True)
+                now = time.perf_counter() - t0
+                for rid in list(pending):
+                    st, tr = srv.poll(rid)
+                    if st == mccfr.DONE:
+                        lat.append(now - pending.pop(rid))
+                        trav += tr
+                        srv.release(rid)
+                if running == 0 and waiting == 0 and i < n and poisson:
+                    gap = reqs[i].arrival - (time.perf_counter() - t0)
+                    if gap > 0:
+                        time.sleep(min(gap, 0.002))
+            wall = time.perf_counter() - t0
+    return np.array(lat), trav, wall
+
+
+def serving_bench(mccfr, workloads, hr, n_req, sat_req, budget, max_running):
+    reqs = workloads.resolve_requests(n_req, rate=200.0, arrival_seed=3, state_seed=4)
+    lat, trav, wall = serve_run(mccfr, hr, reqs, budget, max_running, poisson=True)
+    out = {"requests": n_req, "arrival_rate_per_s": 200.0, "seeds": {"arrivals": 3, "states": 4},
+           "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "max_ms": float(lat.max() * 1e3), "traversals_per_s": trav / wall, "wall_s": wall,
+           "iterations_total": int(sum(r.iterations for r in reqs)),
+           "hbm_budget_bytes": budget, "max_running": max_running}
+    sat = reqs[:sat_req]
+    _, trav_b, wall_b = serve_run(mccfr, hr, sat, budget, max_running, poisson=False)
+    _, trav_p, wall_p = serve_run(mccfr, hr, sat, budget, 1, poisson=False)
+    out["saturated"] = {"requests": sat_req, "served_traversals_per_s": trav_b / wall_b,
+                        "one_at_a_time_traversals_per_s": trav_p / wall_p,
+                        "speedup": (trav_b / wall_b) / (trav_p / wall_p)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,9 +209,15 @@ def main():
     ap.add_argument("--hand-log2", type=int, default=28)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cpus)")
     ap.add_argument("--cpu-launches", type=int, default=24)
+    ap.add_argument("--batched-launches", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-saturated", type=int, default=120)
+    ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hand-bench", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="timed launches only (for rocprofv3 runs)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="headline launches only (for rocprofv3 runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="with --profile: also run a plain device copy of this size (byte-counter calibration)")
     ap.add_argument("--table-seed", type=int, default=2026,
@@ -116,6 +230,7 @@ def main():
         sys.exit(2)
 
     import mccfr
+    import workloads
     cfg = mccfr.cfg_6p_100bb(table_log2=args.table_log2, seed=0)
     B = 1 << args.batch_log2
     N = cfg.num_players
@@ -123,6 +238,7 @@ def main():
         f"steps {args.steps}")
     ctx = mccfr.Context(cfg, B)
     ctx.use_generated_hand_table()
+    hr = mccfr.generate_hand_table()
     state = {"launch": 0, "tid": 0}
 
     def step(trace=False):
@@ -162,11 +278,11 @@ def main():
         step()
     ctx.sync()
     elapsed = time.perf_counter() - t0
-    trav_ms, upd_ms = ctx.kernel_times(args.steps)
+    deal_ms, walk_ms, upd_ms = ctx.kernel_times(args.steps)
     ctx.set_timing(False)
     value = args.steps * B / elapsed
-    log(f"bench: {value / 1e6:.1f} M traversals/s; traverse kernel {trav_ms.mean():.3f} ms, "
-        f"update kernel {upd_ms.mean():.3f} ms per launch")
+    log(f"bench: {value / 1e6:.1f} M traversals/s; deal {deal_ms.mean():.3f} ms, walk {walk_ms.mean():.3f} ms, "
+        f"table update {upd_ms.mean():.3f} ms per launch")
 
     result = {
         "metric": METRIC, "value": value, "unit": "traversals/s", "n_gpus": 1, "steps": args.steps,
@@ -175,7 +291,8 @@ def main():
         "config": {"workload": "6max_100bb: 6 players, 2000-chip stacks, blinds 10/20, menu "
                                "{F, C/C, 1/2P, 1P, 2P, AI}, <=3 raises/street",
                    "table_slots": 1 << args.table_log2, "traversals_per_launch": B, "parallelism": "single GPU"},
-        "kernel_ms": {"traverse": float(trav_ms.mean()), "update": float(upd_ms.mean())},
+        "kernel_ms": {"deal": float(deal_ms.mean()), "walk": float(walk_ms.mean()),
+                      "table_update": float(upd_ms.mean())},
         "table_state": f"seeded random regrets (seed {args.table_seed}) + {args.warmup} warm-up launches",
         "fresh_tables_traversals_per_s": fresh,
     }
@@ -187,23 +304,28 @@ def main():
         return
 
     tr = step(trace=True)  # path statistics of a launch like the timed ones
-    nbytes = traversal_bytes(tr, N, ctx.row)
-    achieved = nbytes / (float(trav_ms.mean()) * 1e-3) / 1e9
+    recs, long_runs, pieces = ctx.last_launch_stats()
+    wbytes = walk_bytes(tr, N, ctx.row)
+    achieved = wbytes / (float(walk_ms.mean()) * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traverse_6max_100bb_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "walk_6max_100bb_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                          "algorithmic_bytes_per_launch": nbytes,
+                          "kernel": "k_walk (betting-tree walk)",
+                          "algorithmic_bytes_per_launch": wbytes,
                           "per_traversal": {"decisions": float(tr["decisions"].mean()),
                                             "update_nodes": float(tr["updates"].mean()),
                                             "update_legal_actions": float(tr["update_legal"].mean())}}
+    result["launch"] = {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
+                        "path_bytes_per_launch": path_bytes(tr, N, ctx.row),
+                        "path_gbs": path_bytes(tr, N, ctx.row) / (elapsed / args.steps) / 1e9}
 
     if not args.no_hand_bench:
         log("bench: hand-evaluation microbenchmark")
-        result["hand_eval"] = hand_bench(mccfr, ctx, args.hand_log2)
+        result["hand_eval"] = hand_bench(ctx, args.hand_log2)
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -211,6 +333,21 @@ def main():
         result["cpu_baseline"] = cpu_baseline(mccfr, ctx, cfg, state["launch"], state["tid"], B,
                                               args.cpu_launches, threads)
     ctx.close()
+
+    if not args.no_batched:
+        log("bench: batched uneven games")
+        result["batched"] = batched_bench(mccfr, workloads, hr, args.batched_launches)
+        log(f"bench: batched {result['batched']['batched_traversals_per_s'] / 1e6:.1f} M/s vs plain "
+            f"{result['batched']['plain_traversals_per_s'] / 1e6:.1f} M/s")
+
+    if not args.no_serving:
+        log("bench: re-solve serving")
+        sv = serving_bench(mccfr, workloads, hr, args.serve_requests, args.serve_saturated,
+                           args.serve_budget_gib << 30, 256)
+        result["serving"] = sv
+        result["p99_resolve_latency_ms"] = sv["p99_ms"]
+        log(f"bench: serving p50 {sv['p50_ms']:.1f} ms p99 {sv['p99_ms']:.1f} ms; saturated speedup "
+            f"{sv['saturated']['speedup']:.2f}x")
     print(json.dumps(result), flush=True)
 
 

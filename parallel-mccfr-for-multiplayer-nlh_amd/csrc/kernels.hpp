@@ -82,7 +82,8 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 // (launch_walk), then records -> sort -> owner-computes table update
 // (launch_reduce).  `trace` (device, indexed by lane) may be null.
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace);
+                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
+                 hipEvent_t after_deal = nullptr);
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits);
 

@@ -215,10 +215,13 @@ int stage_segments(mccfr_ctx* c, const std::vector<DevSeg>& segs) {
   return MCCFR_OK;
 }
 
+hipEvent_t timing_event(mccfr_ctx* c, int k) {
+  return c->timing ? c->ev[(size_t)(c->ev_next % kTimingRing) * 4 + k] : nullptr;
+}
+
 int record_event(mccfr_ctx* c, int k) {
   if (!c->timing) return MCCFR_OK;
-  hipEvent_t e = c->ev[(size_t)(c->ev_next % kTimingRing) * 3 + k];
-  HIPX(c, hipEventRecord(e, c->stream));
+  HIPX(c, hipEventRecord(timing_event(c, k), c->stream));
   return MCCFR_OK;
 }
 
@@ -435,10 +438,10 @@ int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* hos
   if ((rc = stage_segments(c, p.segs))) return rc;
   if ((rc = record_event(c, 0))) return rc;
   launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape,
-              host_trace ? c->dtrace : nullptr);
-  if ((rc = record_event(c, 1))) return rc;
-  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+              host_trace ? c->dtrace : nullptr, timing_event(c, 1));
   if ((rc = record_event(c, 2))) return rc;
+  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+  if ((rc = record_event(c, 3))) return rc;
   uint64_t total = 0;
   for (uint32_t i = 0; i < n; ++i) {
     Tenant& t = c->tenants[segs[i].tenant];
@@ -1073,7 +1076,7 @@ int mccfr_counters(const mccfr_ctx* c, uint64_t* traversals, uint64_t* launches,
 int mccfr_set_timing(mccfr_ctx* c, int enabled) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (enabled && c->ev.empty()) {
-    c->ev.resize((size_t)kTimingRing * 3, nullptr);
+    c->ev.resize((size_t)kTimingRing * 4, nullptr);
     for (auto& e : c->ev) HIPX(c, hipEventCreate(&e));
   }
   c->timing = enabled != 0;
@@ -1082,17 +1085,19 @@ int mccfr_set_timing(mccfr_ctx* c, int enabled) {
   return MCCFR_OK;
 }
 
-int mccfr_kernel_times(mccfr_ctx* c, float* trav_ms, float* upd_ms, uint32_t n) {
+int mccfr_kernel_times(mccfr_ctx* c, float* deal_ms, float* walk_ms, float* upd_ms, uint32_t n) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (n > c->ev_used) return fail(c, MCCFR_E_ARG, "fewer timed launches recorded than requested");
   HIPX(c, hipStreamSynchronize(c->stream));
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t k = (c->ev_next - n + i) % kTimingRing;
-    float a = 0.f, b = 0.f;
-    HIPX(c, hipEventElapsedTime(&a, c->ev[k * 3 + 0], c->ev[k * 3 + 1]));
-    HIPX(c, hipEventElapsedTime(&b, c->ev[k * 3 + 1], c->ev[k * 3 + 2]));
-    if (trav_ms) trav_ms[i] = a;
-    if (upd_ms) upd_ms[i] = b;
+    float a = 0.f, b = 0.f, d = 0.f;
+    HIPX(c, hipEventElapsedTime(&a, c->ev[k * 4 + 0], c->ev[k * 4 + 1]));
+    HIPX(c, hipEventElapsedTime(&b, c->ev[k * 4 + 1], c->ev[k * 4 + 2]));
+    HIPX(c, hipEventElapsedTime(&d, c->ev[k * 4 + 2], c->ev[k * 4 + 3]));
+    if (deal_ms) deal_ms[i] = a;
+    if (walk_ms) walk_ms[i] = b;
+    if (upd_ms) upd_ms[i] = d;
   }
   return MCCFR_OK;
 }

@@ -77,7 +77,7 @@ struct mccfr_ctx {
   uint64_t traversals = 0, launches = 0;
 
   bool timing = false;
-  std::vector<hipEvent_t> ev;  // 3 per timed launch: start, after walk, after table update
+  std::vector<hipEvent_t> ev;  // 4 per timed launch: start, after deal, after walk, after table update
   uint32_t ev_next = 0, ev_used = 0;
 
   std::string err;

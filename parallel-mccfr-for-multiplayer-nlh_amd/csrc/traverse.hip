@@ -1053,8 +1053,10 @@ void walk_dispatch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const
 }  // namespace
 
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace) {
+                 const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
+                 hipEvent_t after_deal) {
   hipLaunchKernelGGL(k_deal, dim3(lc.lanes / kDealWG), dim3(kDealWG), 0, s, lc, segs, tenants, in, sc);
+  if (after_deal) (void)hipEventRecord(after_deal, s);
   if (row == 8)
     walk_dispatch<8>(s, lc, segs, tenants, sc, shape, trace);
   else

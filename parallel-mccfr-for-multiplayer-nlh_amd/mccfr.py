@@ -141,7 +141,7 @@ _SIGS = {
     "mccfr_normalized_strategy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
     "mccfr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
-    "mccfr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mccfr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "mccfr_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "mccfr_random_hands_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_size_t]),
     "mccfr_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -403,10 +403,12 @@ class Context:
         self.check(lib().mccfr_set_timing(self.h, int(on)))
 
     def kernel_times(self, n: int):
+        """(deal_ms, walk_ms, update_ms) arrays of the last n timed launches."""
         a = np.empty(n, np.float32)
         b = np.empty(n, np.float32)
-        self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), n))
-        return a, b
+        d = np.empty(n, np.float32)
+        self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), _ptr(d), n))
+        return a, b, d
 
     def last_launch_stats(self):
         """(records, long runs, pieces) of the last launch."""
