@@ -96,6 +96,18 @@ struct LaunchCounters {
   uint32_t n_long;       // runs handed to the long-run reduction
   uint32_t n_pieces;     // pieces of the long runs
   uint32_t n_cand;       // runs whose end the per-wave pass could not see
+  uint32_t n_runs;       // short runs listed for the row-parallel reduction
+  uint32_t pad[3];
+};
+
+// What the table update needs of a segment, gathered once per launch.
+struct SegCtx {
+  float* R;
+  float* S;
+  uint16_t* sig;
+  float tw;              // strategy-sum weight as FP32 (cfr_gpu.cu:828)
+  int16_t num_actions;
+  int16_t cfr_plus;
 };
 
 // ---------------------------------------------------------------------------

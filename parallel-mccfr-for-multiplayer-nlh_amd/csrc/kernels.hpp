@@ -11,12 +11,13 @@ namespace mccfr {
 // written to the lane's node list in HBM when they are reached.
 constexpr int kRegPending = 4;
 
-// Radix sort of the launch's records: 8-bit digits, tiles of
-// kSortWG threads x kSortItems keys.
+// Radix sort of the launch's records: 10-bit digits (26-30-bit keys in three
+// passes), tiles of kSortWG threads x kSortItems keys.
 constexpr int kSortWG = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr uint32_t kSortTile = kSortWG * kSortItems;
-constexpr uint32_t kSortBins = 256;
+constexpr uint32_t kDigitBits = 10;
+constexpr uint32_t kSortBins = 1u << kDigitBits;
 
 // Scratch of one launch, sized for `cap_lanes` lanes.  Node lists are
 // lane-strided: entry k of lane l at [k * cap_lanes + l].
@@ -33,7 +34,13 @@ struct DevScratch {
   uint2* val[2];         // [kMaxDepth * lanes] record payloads: (action | legal mask << 8, utility bits)
   uint32_t* hist;        // [kSortBins][max_tiles] tile digit counts, then per-digit prefixes over tiles
   uint32_t* bin_total;   // [kSortBins] keys per digit in the current pass
-  uint32_t* cand;        // [max_cand] first records of runs k_reduce leaves to k_reduce_cand
+  // Per-wave regions of the sorted records (64 records each): region w lists
+  // its short runs at runs[64 w ..] and the runs it defers at cand[4 w ..].
+  uint2* runs;           // [max_waves][64] short runs: first record, records
+  uint32_t* n_runs;      // [max_waves] short runs of each region
+  uint32_t* cand;        // [max_waves][4] first records of deferred runs
+  uint32_t* n_cand;      // [max_waves] deferred runs of each region
+  SegCtx* seg_ctx;       // [kMaxSegs] table pointers and weights of the launch's segments
   uint4* long_runs;      // [max_long] long runs: first record, records, first piece, pieces
   uint32_t* piece_run;   // [max_pieces] long run of each piece
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece

@@ -15,14 +15,16 @@
 //                 carries the record's action word and utility, so the table
 //                 update reads records in sorted order without gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
-//                 (key = segment << slot_bits | slot), 8-bit digits.
-//   k_reduce      every run of equal keys is one table slot of one tenant:
-//                 its owner sums the run's contributions (cfr_gpu.cu:784-837)
-//                 in FP64 in sorted order and applies them: regrets += sum
-//                 (CFR+ clamp), strategy sums += sum, BF16 snapshot
-//                 re-matched.  Runs longer than kShortRun are cut into
-//                 kPiece-record pieces (k_reduce_piece, one workgroup each)
-//                 whose sums k_reduce_runs adds in piece order and applies.
+//                 (key = segment << slot_bits | slot), 10-bit digits.
+//   k_runs        every run of equal keys is one table slot of one tenant;
+//                 runs are listed (k_reduce_cand finds the ends the per-wave
+//                 pass cannot see).  k_reduce_rows: a group of ROW lanes owns
+//                 a short run, lane a summing action a's contributions
+//                 (cfr_gpu.cu:784-837) in FP64 in record order, then applies
+//                 them: regrets += sum (CFR+ clamp), strategy sums += sum, BF16
+//                 snapshot re-matched.  Runs longer than kShortRun are cut
+//                 into kPiece-record pieces (k_reduce_piece, one workgroup
+//                 each) whose sums k_reduce_runs adds in piece order.
 // No atomics touch the tables, and every sum has a fixed order, so a tenant's
 // result does not depend on which other tenants share its launches.
 // Untouched slots need no sweep: clamp and regret matching are per-slot and
@@ -575,6 +577,7 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc)
     sc.cnt->n_long = 0;
     sc.cnt->n_pieces = 0;
     sc.cnt->n_cand = 0;
+    sc.cnt->n_runs = 0;
   }
 }
 
@@ -598,9 +601,10 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, one 8-bit digit per pass: tile histograms, a scan of
-// the (digit, tile) counts, and a scatter that ranks each key within its tile
-// by wave-level digit matching (8 ballots) so equal digits keep their order.
+// Stable LSD radix sort, one kDigitBits-bit digit per pass: tile histograms, a
+// scan of the (digit, tile) counts, and a scatter that ranks each key within
+// its tile by wave-level digit matching (one ballot per digit bit) so equal
+// digits keep their order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t n_tiles_of(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 
@@ -608,7 +612,7 @@ __device__ __forceinline__ uint32_t n_tiles_of(uint32_t n) { return (n + kSortTi
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
   uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (int b = 0; b < (int)kDigitBits; ++b) {
     const uint64_t bal = __ballot((d >> b) & 1u);
     peers &= ((d >> b) & 1u) ? bal : ~bal;
   }
@@ -624,18 +628,18 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
   const int tx = threadIdx.x, lane = tx & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    s_h[tx] = 0;
+    for (uint32_t b = tx; b < kSortBins; b += kSortWG) s_h[b] = 0;
     __syncthreads();
 #pragma unroll 4
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
       const bool valid = idx < n;
-      const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
+      const uint32_t d = valid ? (keys[idx] >> shift) & (kSortBins - 1u) : 0u;
       const uint64_t peers = digit_peers(d, valid);  // one LDS add per distinct digit of the wave
       if (valid && (peers & lt) == 0) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    hist[(size_t)tx * max_tiles + tile] = s_h[tx];
+    for (uint32_t b = tx; b < kSortBins; b += kSortWG) hist[(size_t)b * max_tiles + tile] = s_h[b];
     __syncthreads();
   }
 }
@@ -674,8 +678,9 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
                                                         const LaunchCounters* __restrict__ cnt, uint32_t shift,
                                                         const uint32_t* __restrict__ hist,
                                                         const uint32_t* __restrict__ bin_total, uint32_t max_tiles) {
-  static_assert(kSortWG == 256 && kSortBins == 256, "one thread per digit");
+  static_assert(kSortWG == 256 && kSortBins % kSortWG == 0, "whole digits per thread");
   constexpr int kWaves = kSortWG / 64;
+  constexpr int kPerThread = (int)(kSortBins / kSortWG);  // digits owned by each thread
   __shared__ uint32_t s_wc[kWaves][kSortBins];  // per-wave digit counts, then per-wave bases
   __shared__ uint32_t s_gb[kSortBins];          // global offset of (digit, tile)
   __shared__ uint32_t s_w[4];
@@ -684,10 +689,24 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
   const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
   // first output position of each digit: exclusive scan of the digit totals
-  const uint32_t digit_base = block_excl_scan_256(bin_total[tx], s_w, nullptr);
+  // (thread tx owns digits tx*kPerThread .. +kPerThread-1)
+  uint32_t digit_base[kPerThread];
+  {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      digit_base[q] = sum;
+      sum += bin_total[tx * kPerThread + q];
+    }
+    const uint32_t b0 = block_excl_scan_256(sum, s_w, nullptr);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) digit_base[q] += b0;
+  }
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) s_wc[k][tx] = 0;
+    for (int k = 0; k < kWaves; ++k)
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) s_wc[k][tx * kPerThread + q] = 0;
     __syncthreads();
     // wave w holds keys [wbase, wbase + 64 * kSortItems) of the tile, item-major
     uint32_t kk[kSortItems], off[kSortItems];
@@ -704,7 +723,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       const bool valid = idx < n;
-      const uint32_t d = (kk[i] >> shift) & 0xFFu;
+      const uint32_t d = (kk[i] >> shift) & (kSortBins - 1u);
       const uint64_t peers = digit_peers(d, valid);
       // all lanes read the wave's count before the group's first lane bumps it
       const uint32_t before = s_wc[w][d];
@@ -714,22 +733,24 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    {
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const uint32_t dd = (uint32_t)(tx * kPerThread + q);
       uint32_t run = 0;
 #pragma unroll
       for (int k = 0; k < kWaves; ++k) {
-        const uint32_t c = s_wc[k][tx];
-        s_wc[k][tx] = run;
+        const uint32_t c = s_wc[k][dd];
+        s_wc[k][dd] = run;
         run += c;
       }
-      s_gb[tx] = digit_base + hist[(size_t)tx * max_tiles + tile];
+      s_gb[dd] = digit_base[q] + hist[(size_t)dd * max_tiles + tile];
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       if (idx < n) {
-        const uint32_t d = (kk[i] >> shift) & 0xFFu;
+        const uint32_t d = (kk[i] >> shift) & (kSortBins - 1u);
         const uint32_t pos = s_gb[d] + s_wc[w][d] + off[i];
         kout[pos] = kk[i];
         vout[pos] = vv[i];
@@ -740,156 +761,40 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Owner-computes table update of one run of records (cfr_gpu.cu:784-837 for
-// the contributions, :1018-1022 and :318-351 for clamp and re-matching).
-// sigma comes from the pre-launch regret row, which nothing has modified yet:
-// each slot is written only by its run's owner.
+// Owner-computes table update (cfr_gpu.cu:784-837 for the contributions,
+// :1018-1022 and :318-351 for clamp and re-matching).  Every run of equal
+// keys is one slot of one tenant with exactly one owner: a group of ROW lanes,
+// lane a holding action a's FP64 sums.  sigma comes from the pre-launch
+// regret row, which nothing has modified yet: only the owner writes a slot.
 // ---------------------------------------------------------------------------
-template <int ROW>
-struct RunAcc {
-  double dr[ROW], ds[ROW];
-  uint32_t cached_vm;
-  float sig_c[ROW], q_of_act[ROW];
-
-  __device__ __forceinline__ void reset() {
-#pragma unroll
-    for (int a = 0; a < ROW; ++a) {
-      dr[a] = ds[a] = 0.0;
-      sig_c[a] = q_of_act[a] = 0.0f;
-    }
-    cached_vm = 0;
+__global__ __launch_bounds__(256) void k_seg_ctx(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                 const DevGame* __restrict__ tenants, SegCtx* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lc.n_segs; i += gridDim.x * blockDim.x) {
+    const DevSeg& S = segs[i];
+    const DevGame& G = tenants[S.tenant];
+    SegCtx c;
+    c.R = G.R;
+    c.S = G.S;
+    c.sig = G.sig;
+    c.tw = (float)S.t_weight;
+    c.num_actions = (int16_t)G.num_actions;
+    c.cfr_plus = (int16_t)(G.use_cfr_plus != 0);
+    out[i] = c;
   }
-  // sigma of the slot's row, recomputed only when the legal mask changes
-  // (hash collisions can put info sets with different masks in one slot)
-  __device__ __forceinline__ void add(const float (&rr)[ROW], int A, float tw, uint32_t av, float u) {
-    const uint32_t act = av & 0xFFu, vm = av >> 8;
-    if (vm != cached_vm) {
-      cached_vm = vm;
-      float sum_r = 0.0f;
-      float lr[ROW];
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        lr[a] = (a < A && (vm >> a & 1u)) ? fmaxf(0.0f, rr[a]) : 0.0f;
-        sum_r += lr[a];
-      }
-      const float uni = 1.0f / (float)__popc(vm);
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        sig_c[a] = (sum_r > 1e-7f) ? lr[a] / sum_r : ((a < A && (vm >> a & 1u)) ? uni : 0.0f);
-        q_of_act[a] = fmaxf((sum_r > 1e-7f) ? lr[a] / sum_r : uni, 0.05f);
-      }
-    }
-    float q = q_of_act[0];
-#pragma unroll
-    for (int a = 1; a < ROW; ++a) q = ((uint32_t)a == act) ? q_of_act[a] : q;
-    const float rd_sampled = u / q - u;
-    const float rd_other = 0.0f - u;
-#pragma unroll
-    for (int a = 0; a < ROW; ++a)
-      if (a < A && (vm >> a & 1u)) {
-        dr[a] += (double)((uint32_t)a == act ? rd_sampled : rd_other);
-        ds[a] += (double)(tw * sig_c[a]);
-      }
-  }
-};
-
-template <int ROW>
-__device__ __forceinline__ void apply_row(float* __restrict__ R, float* __restrict__ S, uint16_t* __restrict__ sig,
-                                          const float (&rr)[ROW], const float (&ss)[ROW], const double (&dr)[ROW],
-                                          const double (&ds)[ROW], bool cfr_plus) {
-  float s[ROW];
-#pragma unroll
-  for (int a = 0; a < ROW; ++a) s[a] = ss[a];
-  float r[ROW];
-  float sum = 0.0f;
-#pragma unroll
-  for (int a = 0; a < ROW; ++a) {
-    float nr = (float)((double)rr[a] + dr[a]);
-    if (cfr_plus) nr = fmaxf(0.0f, nr);
-    r[a] = nr;
-    s[a] = (float)((double)s[a] + ds[a]);
-    sum += fmaxf(0.0f, nr);
-  }
-  uint32_t packed[ROW / 2];
-#pragma unroll
-  for (int a = 0; a < ROW; a += 2) {
-    const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, r[a]) / sum : 0.0f;
-    const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, r[a + 1]) / sum : 0.0f;
-    packed[a / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
-  }
-#pragma unroll
-  for (int v = 0; v < ROW / 4; ++v) {
-    reinterpret_cast<float4*>(R)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
-    reinterpret_cast<float4*>(S)[v] = make_float4(s[4 * v], s[4 * v + 1], s[4 * v + 2], s[4 * v + 3]);
-  }
-#pragma unroll
-  for (int v = 0; v < ROW / 8; ++v)
-    reinterpret_cast<uint4*>(sig)[v] = make_uint4(packed[4 * v], packed[4 * v + 1], packed[4 * v + 2], packed[4 * v + 3]);
 }
 
-struct RunCtx {
-  float* R;
-  float* S;
-  uint16_t* sig;
-  int A;
-  bool cfr_plus;
-  float tw;
-};
-
-__device__ __forceinline__ RunCtx run_ctx(uint32_t key, uint32_t slot_bits, const DevSeg* __restrict__ segs,
-                                          const DevGame* __restrict__ tenants, int row) {
-  const uint32_t si = (uint32_t)((uint64_t)key >> slot_bits);
-  const uint32_t slot = key & (uint32_t)((1ull << slot_bits) - 1ull);
-  const DevSeg& S = segs[si];
-  const DevGame& G = tenants[S.tenant];
-  RunCtx c;
-  c.R = G.R + (size_t)slot * row;
-  c.S = G.S + (size_t)slot * row;
-  c.sig = G.sig + (size_t)slot * row;
-  c.A = G.num_actions;
-  c.cfr_plus = G.use_cfr_plus != 0;
-  c.tw = (float)S.t_weight;
-  return c;
-}
-
-// Reduces a run [i, e) of at most kShortRun records and applies it.
-template <int ROW>
-__device__ __forceinline__ void short_run(uint32_t key, uint32_t i, uint32_t e, const LaunchCfg& lc,
-                                          const DevSeg* __restrict__ segs, const DevGame* __restrict__ tenants,
-                                          const uint2* __restrict__ vals) {
-  const RunCtx c = run_ctx(key, lc.slot_bits, segs, tenants, ROW);
-  float rr[ROW], ss[ROW];
-  load_f32_row<ROW>(c.R, rr);
-  load_f32_row<ROW>(c.S, ss);  // issued together with the regret row
-  RunAcc<ROW> acc;
-  acc.reset();
-  for (uint32_t r = i; r < e;) {
-    // load up to 4 records at once so their loads overlap
-    constexpr int kBatch = 4;
-    uint2 v[kBatch];
-#pragma unroll
-    for (int b = 0; b < kBatch; ++b) v[b] = r + b < e ? vals[r + b] : make_uint2(0u, 0u);
-#pragma unroll
-    for (int b = 0; b < kBatch; ++b)
-      if (r + b < e) acc.add(rr, c.A, c.tw, v[b].x, __uint_as_float(v[b].y));
-    r += kBatch;
-  }
-  apply_row<ROW>(c.R, c.S, c.sig, rr, ss, acc.dr, acc.ds, c.cfr_plus);
-}
-
-// Every record is looked at by one lane; the lane holding a run's first
-// record owns the run.  A run that ends inside the wave (the wave's head
-// flags, one ballot) with at most kShortRun records is reduced and applied
-// here; the others -- longer, or running past the wave -- are listed for
-// k_reduce_cand, so no lane here waits on a search.
-template <int ROW>
-__global__ __launch_bounds__(256) void k_reduce(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                const DevGame* __restrict__ tenants, DevScratch sc,
-                                                const uint32_t* __restrict__ keys, const uint2* __restrict__ vals) {
+// Finds the runs: one wave per 64 sorted records, head flags by ballot.  A
+// run that ends inside its wave with at most kShortRun records is listed in
+// the wave's region for k_reduce_rows; the others (longer, or running past the
+// wave: at most 4 per wave) in the wave's region for k_reduce_cand.  Plain
+// stores, no atomics: every region has one writer.
+__global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const int lane = threadIdx.x & 63;
   const uint32_t stride = gridDim.x * blockDim.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
+    const uint32_t w = base >> 6;
     const uint32_t i = base + (uint32_t)lane;
     const bool valid = i < n;
     const uint32_t key = valid ? keys[i] : 0u;
@@ -898,132 +803,214 @@ __global__ __launch_bounds__(256) void k_reduce(LaunchCfg lc, const DevSeg* __re
     const uint64_t later = heads & ~((2ull << lane) - 1ull);  // heads of higher lanes
     const uint32_t e = later ? base + (uint32_t)__builtin_ctzll(later) : min(base + 64u, n);
     const bool closed = later != 0 || e == n;
-    const bool defer = head && (!closed || e - i > kShortRun);
+    const bool is_short = head && closed && e - i <= kShortRun;
+    const bool defer = head && !is_short;
+    const uint64_t sm = __ballot(is_short);
     const uint64_t dm = __ballot(defer);
-    if (dm) {  // one atomic per wave for its deferred runs
-      uint32_t b0 = 0;
-      if (lane == (int)__builtin_ctzll(dm)) b0 = atomicAdd(&sc.cnt->n_cand, (uint32_t)__popcll(dm));
-      b0 = __shfl(b0, (int)__builtin_ctzll(dm), 64);
-      if (defer) sc.cand[b0 + (uint32_t)__popcll(dm & ((1ull << lane) - 1ull))] = i;
+    if (is_short) sc.runs[(size_t)w * 64 + (uint32_t)__popcll(sm & lt)] = make_uint2(i, e - i);
+    if (defer) sc.cand[(size_t)w * 4 + (uint32_t)__popcll(dm & lt)] = i;
+    if (lane == 0) {
+      sc.n_runs[w] = (uint32_t)__popcll(sm);
+      sc.n_cand[w] = (uint32_t)__popcll(dm);
     }
-    if (head && !defer) short_run<ROW>(key, i, e, lc, segs, tenants, vals);
   }
 }
 
-// One thread per deferred run: its end by a short forward scan, then
-// exponential and binary search (the keys ascend).  Short runs are reduced
-// here; long ones are cut into pieces for k_reduce_piece.
-template <int ROW>
-__global__ __launch_bounds__(256) void k_reduce_cand(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                     const DevGame* __restrict__ tenants, DevScratch sc,
-                                                     const uint32_t* __restrict__ keys,
-                                                     const uint2* __restrict__ vals) {
+// One thread per wave region's deferred runs: each run's end by a short
+// forward scan, then exponential and binary search (the keys ascend).  Short
+// runs join the region's run list; long ones are cut into kPiece-record
+// pieces.
+__global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
-  const uint32_t nc = sc.cnt->n_cand;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-    const uint32_t i = sc.cand[c];
-    const uint32_t key = keys[i];
-    uint32_t e = i + 1;
-    while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
-    if (e - i <= kShortRun) {
-      short_run<ROW>(key, i, e, lc, segs, tenants, vals);
-      continue;
+  const uint32_t n_waves = (n + 63) / 64;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_waves; w += gridDim.x * blockDim.x) {
+    const uint32_t nc = sc.n_cand[w];
+    uint32_t nr = sc.n_runs[w];
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t i = sc.cand[(size_t)w * 4 + c];
+      const uint32_t key = keys[i];
+      uint32_t e = i + 1;
+      while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
+      if (e - i <= kShortRun) {
+        sc.runs[(size_t)w * 64 + nr++] = make_uint2(i, e - i);
+        continue;
+      }
+      // keys[e - 1] == key: bracket the end, then bisect
+      uint32_t lo = e, step = 32;
+      while (lo + step < n && keys[lo + step - 1] == key) {
+        lo += step;
+        step <<= 1;
+      }
+      uint32_t hi = min(lo + step, n);
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] == key) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
+      const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
+      const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
+      sc.long_runs[r] = make_uint4(i, len, p0, np);
+      for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
     }
-    // keys[e - 1] == key: bracket the end, then bisect
-    uint32_t lo = e, step = 32;
-    while (lo + step < n && keys[lo + step - 1] == key) {
-      lo += step;
-      step <<= 1;
-    }
-    uint32_t hi = min(lo + step, n);
-    while (lo < hi) {
-      const uint32_t mid = lo + ((hi - lo) >> 1);
-      if (keys[mid] == key) lo = mid + 1;
-      else hi = mid;
-    }
-    const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
-    const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
-    const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
-    sc.long_runs[r] = make_uint4(i, len, p0, np);
-    for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
+    if (nc) sc.n_runs[w] = nr;
   }
 }
 
-// One workgroup per piece of a long run: records split over the threads in a
-// fixed pattern, partial sums combined in a fixed tree, FP64 sums stored.
+// Contributions of one slot's records for one action (lane a of the group):
+// sigma and q from the pre-launch regret row rr (cfr_gpu.cu:784-801),
+// recomputed only when the legal mask changes (hash collisions can put info
+// sets with different masks in one slot).
 template <int ROW>
-__global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                      const DevGame* __restrict__ tenants, DevScratch sc,
-                                                      const uint32_t* __restrict__ keys,
+struct RowAcc {
+  double dr = 0.0, ds = 0.0;
+  uint32_t cached_vm = 0;
+  float sum_r = 0.0f, uni = 0.0f;
+
+  __device__ __forceinline__ void add(const float (&rr)[ROW], float r_a, int a, int A, float tw, uint32_t av,
+                                      float u) {
+    const uint32_t act = av & 0xFFu, vm = av >> 8;
+    if (vm != cached_vm) {
+      cached_vm = vm;
+      float s = 0.0f;
+#pragma unroll
+      for (int b = 0; b < ROW; ++b) s += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
+      sum_r = s;
+      uni = 1.0f / (float)__popc(vm);
+    }
+    const bool pos = sum_r > 1e-7f;
+    float r_act = rr[0];
+#pragma unroll
+    for (int b = 1; b < ROW; ++b) r_act = ((uint32_t)b == act) ? rr[b] : r_act;
+    const float q = fmaxf(pos ? fmaxf(0.0f, r_act) / sum_r : uni, 0.05f);
+    const bool legal = a < A && (vm >> a & 1u);
+    const float sig_a = pos ? (legal ? fmaxf(0.0f, r_a) : 0.0f) / sum_r : (legal ? uni : 0.0f);
+    const float rd_sampled = u / q - u;
+    const float rd_other = 0.0f - u;
+    if (legal) {
+      dr += (double)((uint32_t)a == act ? rd_sampled : rd_other);
+      ds += (double)(tw * sig_a);
+    }
+  }
+};
+
+// Lane a of a ROW-lane group: R <- R + dR (CFR+ clamp), S <- S + dS, and the
+// slot's BF16 strategy re-matched from the new row (sum in action order).
+template <int ROW>
+__device__ __forceinline__ void apply_lane(const SegCtx& c, size_t row_off, int a, int group_lane0, float r_a,
+                                           float s_a, double dr, double ds) {
+  float nr = (float)((double)r_a + dr);
+  if (c.cfr_plus) nr = fmaxf(0.0f, nr);
+  const float ns = (float)((double)s_a + ds);
+  float sum = 0.0f;
+#pragma unroll
+  for (int b = 0; b < ROW; ++b) sum += fmaxf(0.0f, __shfl(nr, group_lane0 + b, 64));
+  const float x = (sum > 1e-7f) ? fmaxf(0.0f, nr) / sum : 0.0f;
+  c.R[row_off + a] = nr;
+  c.S[row_off + a] = ns;
+  c.sig[row_off + a] = f32_to_bf16(x);
+}
+
+__device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, uint32_t key, uint32_t slot_bits,
+                                             uint32_t* slot) {
+  *slot = key & (uint32_t)((1ull << slot_bits) - 1ull);
+  return sctx[(uint32_t)((uint64_t)key >> slot_bits)];
+}
+
+// Short runs, one ROW-lane group each; a workgroup takes wave regions in turn
+// and spreads each region's runs over its groups.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
+                                                     const uint2* __restrict__ vals) {
+  constexpr uint32_t kGroups = 256 / ROW;
+  const uint32_t n_waves = (sc.cnt->n_rec + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int a = lane & (ROW - 1);
+  const int g_lane0 = lane & ~(ROW - 1);
+  const uint32_t g = threadIdx.x / ROW;
+  for (uint32_t w = blockIdx.x; w < n_waves; w += gridDim.x)
+  for (uint32_t j = g, nr = sc.n_runs[w]; j < nr; j += kGroups) {
+    const uint2 run = sc.runs[(size_t)w * 64 + j];
+    uint32_t slot;
+    const SegCtx c = ctx_of_key(sc.seg_ctx, keys[run.x], lc.slot_bits, &slot);
+    const size_t row_off = (size_t)slot * ROW;
+    float rr[ROW];
+    load_f32_row<ROW>(c.R + row_off, rr);
+    const float r_a = c.R[row_off + a];
+    const float s_a = c.S[row_off + a];
+    RowAcc<ROW> acc;
+    for (uint32_t k = 0; k < run.y; ++k) {
+      const uint2 v = vals[run.x + k];
+      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
+    }
+    apply_lane<ROW>(c, row_off, a, g_lane0, r_a, s_a, acc.dr, acc.ds);
+  }
+}
+
+// One workgroup per piece of a long run: its groups take records in a fixed
+// stride, their sums are added in group order, and the piece's FP64 sums
+// stored.
+template <int ROW>
+__global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ vals) {
-  __shared__ double s_red[4][2 * ROW];
+  constexpr int kGroups = 256 / ROW;
+  __shared__ double s_dr[kGroups][ROW], s_ds[kGroups][ROW];
   const uint32_t n_pieces = sc.cnt->n_pieces;
-  const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
+  const int tx = threadIdx.x;
+  const int a = tx & (ROW - 1), g = tx / ROW;
   for (uint32_t p = blockIdx.x; p < n_pieces; p += gridDim.x) {
     const uint4 run = sc.long_runs[sc.piece_run[p]];
     const uint32_t start = run.x + (p - run.z) * kPiece;
     const uint32_t end = min(run.x + run.y, start + kPiece);
-    const RunCtx c = run_ctx(keys[start], lc.slot_bits, segs, tenants, ROW);
+    uint32_t slot;
+    const SegCtx c = ctx_of_key(sc.seg_ctx, keys[start], lc.slot_bits, &slot);
+    const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
-    load_f32_row<ROW>(c.R, rr);
-    RunAcc<ROW> acc;
-    acc.reset();
-#pragma unroll
-    for (uint32_t k = 0; k < kPiece / 256; ++k) {
-      const uint32_t idx = start + (uint32_t)tx + 256u * k;
-      if (idx < end) {
-        const uint2 v = vals[idx];
-        acc.add(rr, c.A, c.tw, v.x, __uint_as_float(v.y));
-      }
+    load_f32_row<ROW>(c.R + row_off, rr);
+    const float r_a = c.R[row_off + a];
+    RowAcc<ROW> acc;
+    for (uint32_t idx = start + (uint32_t)g; idx < end; idx += kGroups) {
+      const uint2 v = vals[idx];
+      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
     }
-#pragma unroll
-    for (int a = 0; a < ROW; ++a) {
-      double x = acc.dr[a], y = acc.ds[a];
-#pragma unroll
-      for (int d = 32; d > 0; d >>= 1) {
-        x += __shfl_xor(x, d, 64);
-        y += __shfl_xor(y, d, 64);
-      }
-      acc.dr[a] = x;
-      acc.ds[a] = y;
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        s_red[w][a] = acc.dr[a];
-        s_red[w][ROW + a] = acc.ds[a];
-      }
-    }
+    s_dr[g][a] = acc.dr;
+    s_ds[g][a] = acc.ds;
     __syncthreads();
-    if (tx < 2 * ROW) sc.partial[(size_t)p * 32 + tx] = ((s_red[0][tx] + s_red[1][tx]) + s_red[2][tx]) + s_red[3][tx];
+    if (tx < ROW) {
+      double dr = 0.0, ds = 0.0;
+      for (int k = 0; k < kGroups; ++k) {
+        dr += s_dr[k][tx];
+        ds += s_ds[k][tx];
+      }
+      sc.partial[(size_t)p * 32 + tx] = dr;
+      sc.partial[(size_t)p * 32 + ROW + tx] = ds;
+    }
     __syncthreads();
   }
 }
 
-// One thread per long run: its pieces' sums in piece order, then the update.
+// One ROW-lane group per long run: its pieces' sums in piece order, then the
+// update.
 template <int ROW>
-__global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                     const DevGame* __restrict__ tenants, DevScratch sc,
-                                                     const uint32_t* __restrict__ keys) {
+__global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n_long = sc.cnt->n_long;
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_long; r += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int a = lane & (ROW - 1);
+  const int g_lane0 = lane & ~(ROW - 1);
+  const uint32_t gstride = gridDim.x * blockDim.x / ROW;
+  for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / ROW; r < n_long; r += gstride) {
     const uint4 run = sc.long_runs[r];
-    const RunCtx c = run_ctx(keys[run.x], lc.slot_bits, segs, tenants, ROW);
-    float rr[ROW], ss[ROW];
-    load_f32_row<ROW>(c.R, rr);
-    load_f32_row<ROW>(c.S, ss);
-    double dr[ROW], ds[ROW];
-#pragma unroll
-    for (int a = 0; a < ROW; ++a) dr[a] = ds[a] = 0.0;
+    uint32_t slot;
+    const SegCtx c = ctx_of_key(sc.seg_ctx, keys[run.x], lc.slot_bits, &slot);
+    const size_t row_off = (size_t)slot * ROW;
+    const float r_a = c.R[row_off + a];
+    const float s_a = c.S[row_off + a];
+    double dr = 0.0, ds = 0.0;
     for (uint32_t j = 0; j < run.w; ++j) {
-      const double* q = sc.partial + (size_t)(run.z + j) * 32;
-#pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        dr[a] += q[a];
-        ds[a] += q[ROW + a];
-      }
+      dr += sc.partial[(size_t)(run.z + j) * 32 + a];
+      ds += sc.partial[(size_t)(run.z + j) * 32 + ROW + a];
     }
-    apply_row<ROW>(c.R, c.S, c.sig, rr, ss, dr, ds, c.cfr_plus);
+    apply_lane<ROW>(c, row_off, a, g_lane0, r_a, s_a, dr, ds);
   }
 }
 
@@ -1073,9 +1060,9 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint64_t max_rec = (uint64_t)lc.lanes * kMaxDepth;
   const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
   const uint32_t sgrid = std::min<uint32_t>(std::max<uint32_t>(max_rec_tiles, 1u), 2048u);
-  const uint32_t passes = key_bits == 0 ? 1u : (key_bits + 7u) / 8u;
+  const uint32_t passes = key_bits == 0 ? 1u : (key_bits + kDigitBits - 1u) / kDigitBits;
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t shift = 8u * p;
+    const uint32_t shift = kDigitBits * p;
     const uint32_t* kin = sc.key[p & 1];
     const uint2* vin = sc.val[p & 1];
     hipLaunchKernelGGL(k_rs_hist, dim3(sgrid), dim3(kSortWG), 0, s, kin, sc.cnt, shift, sc.hist, sc.max_tiles);
@@ -1086,11 +1073,13 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint32_t* keys = sc.key[passes & 1];
   const uint2* vals = sc.val[passes & 1];
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
-#define MCCFR_REDUCE(R)                                                                                       \
-  hipLaunchKernelGGL(k_reduce<R>, dim3(rgrid), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);           \
-  hipLaunchKernelGGL(k_reduce_cand<R>, dim3(256), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);        \
-  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, segs, tenants, sc, keys, vals);      \
-  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(64), dim3(256), 0, s, lc, segs, tenants, sc, keys)
+  hipLaunchKernelGGL(k_seg_ctx, dim3((lc.n_segs + 255) / 256), dim3(256), 0, s, lc, segs, tenants, sc.seg_ctx);
+  hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
+  hipLaunchKernelGGL(k_reduce_cand, dim3(rgrid / 64 + 1), dim3(256), 0, s, sc, keys);
+#define MCCFR_REDUCE(R)                                                                                \
+  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);               \
+  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);              \
+  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(256), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
     MCCFR_REDUCE(8);
   } else {
