@@ -199,13 +199,68 @@ def serving_bench(mccfr, workloads, hr, n_req, sat_req, budget, max_running):
     return out
 
 
+def large_table_bench(args):
+    """6 players, 200bb (4,000 chips), 10-action menu, <= 4 raises per street,
+    2^28-slot tables (2^32 regret + strategy-sum entries, 32 GiB FP32, plus the
+    8 GiB BF16 snapshot), launches of 2^22 traversals, about 1e9 traversals
+    from the captured training graph."""
+    import mccfr
+    cfg = mccfr.cfg_6p_200bb_10(table_log2=args.table_log2 or 28, seed=0)
+    B = 1 << (args.batch_log2 or 22)
+    N = cfg.num_players
+    iters = max(1, int(round(args.large_traversals / (B * N))))
+    ctx = mccfr.Context(cfg, B)
+    ctx.use_generated_hand_table()
+    slots, row = ctx.slots, ctx.row
+    log(f"bench: large_table 2^{cfg.table_log2} slots x {row} actions, batch 2^{B.bit_length() - 1}, "
+        f"{iters} iterations x {N} launches")
+    ctx.randomize_tables(args.table_seed)
+    ctx.set_timing(True)
+    for L in range(2 * N):  # warm-up, timed per kernel
+        ctx.run_launch(L % N, L // N + 1, L * B, B)
+    deal_ms, walk_ms, upd_ms = ctx.kernel_times(2 * N)
+    ctx.set_timing(False)
+    recs, long_runs, pieces = ctx.last_launch_stats()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.train(iters, B)
+    dt = time.perf_counter() - t0
+    total = iters * N * B
+    result = {
+        "metric": "MCCFR traversals/s (6-max 200bb, 10-action menu, 2^32-entry tables), 1 MI355X",
+        "value": total / dt, "unit": "traversals/s", "n_gpus": 1, "steps": iters * N, "warmup": 2 * N,
+        "ms_per_step": dt * 1e3 / (iters * N), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "large_table: 6 players, 4000-chip stacks, blinds 10/20, menu {F, C/C, 1/4, 1/2, "
+                               "3/4, 1, 3/2, 2, 3 pot, AI}, <=4 raises/street",
+                   "table_slots": slots, "row_actions": row, "entries": slots * row,
+                   "table_bytes": slots * row * 10, "traversals_per_launch": B, "traversals": total,
+                   "parallelism": "single GPU"},
+        "kernel_ms": {"deal": float(deal_ms[N:].mean()), "walk": float(walk_ms[N:].mean()),
+                      "table_update": float(upd_ms[N:].mean())},
+        "launch": {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
+                   "table_atomics": 0,
+                   "counter_atomics": 2 * long_runs},
+    }
+    for name in ("large_table_l2.json", "large_table_traffic.json"):
+        path = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(path):
+            with open(path) as f:
+                result[name[len("large_table_"):-5]] = json.load(f)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("6max_100bb", "large_table"), default="6max_100bb",
+                    help="large_table: the 2^32-entry stress workload, one JSON line of its own")
+    ap.add_argument("--large-traversals", type=float, default=1e9)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=12)
-    ap.add_argument("--batch-log2", type=int, default=20)
-    ap.add_argument("--table-log2", type=int, default=26)
+    ap.add_argument("--batch-log2", type=int, default=None, help="default 20 (large_table: 22)")
+    ap.add_argument("--table-log2", type=int, default=None, help="default 26 (large_table: 28)")
     ap.add_argument("--hand-log2", type=int, default=28)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cpus)")
     ap.add_argument("--cpu-launches", type=int, default=24)
@@ -228,6 +283,12 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU delta exchange is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "large_table":
+        large_table_bench(args)
+        return
+    args.batch_log2 = args.batch_log2 or 20
+    args.table_log2 = args.table_log2 or 26
 
     import mccfr
     import workloads

@@ -87,6 +87,21 @@ __device__ __forceinline__ uint32_t colex6(uint32_t a, uint32_t b, uint32_t c, u
   return binom<1>(a) + binom<2>(b) + binom<3>(c) + binom<4>(d) + binom<5>(e) + binom<6>(f);
 }
 
+// Colex ranks from a table of binomials, tb[k * 52 + n] = C(n, k) for
+// k = 0..6, n < 52 (same values as colex5 / colex6).
+__device__ __forceinline__ uint32_t colex5_tb(const uint32_t* tb, uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                              uint32_t e) {
+  cswap(a, b); cswap(d, e); cswap(c, e); cswap(c, d); cswap(b, e);
+  cswap(a, d); cswap(a, c); cswap(b, d); cswap(b, c);
+  return tb[52 + a] + tb[104 + b] + tb[156 + c] + tb[208 + d] + tb[260 + e];
+}
+__device__ __forceinline__ uint32_t colex6_tb(const uint32_t* tb, uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                              uint32_t e, uint32_t f) {
+  cswap(a, b); cswap(c, d); cswap(e, f); cswap(a, c); cswap(b, e); cswap(d, f);
+  cswap(a, b); cswap(c, d); cswap(e, f); cswap(b, c); cswap(d, e); cswap(c, d);
+  return tb[52 + a] + tb[104 + b] + tb[156 + c] + tb[208 + d] + tb[260 + e] + tb[312 + f];
+}
+
 // 7-card rank 1..7462 of distinct suit-major cards by the arithmetic
 // evaluator: a suit holding >= 5 cards makes a flush or straight flush, which
 // no other hand of seven cards beats, ranked by its 13-bit rank mask; otherwise

@@ -36,7 +36,7 @@ struct DevScratch {
   uint32_t* bin_total;   // [kSortBins] keys per digit in the current pass
   // Per-wave regions of the sorted records (64 records each): region w lists
   // its short runs at runs[64 w ..] and the runs it defers at cand[4 w ..].
-  uint2* runs;           // [max_waves][64] short runs: first record, records
+  uint4* runs;           // [max_waves][64] short runs: first record, records, key
   uint32_t* n_runs;      // [max_waves] short runs of each region
   uint32_t* cand;        // [max_waves][4] first records of deferred runs
   uint32_t* n_cand;      // [max_waves] deferred runs of each region

@@ -89,7 +89,7 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   const size_t max_pieces = recs / kPiece + max_long;
   if (e == hipSuccess) e = hipMalloc(&s.bin_total, kSortBins * sizeof(uint32_t));
   const size_t max_waves = recs / 64 + 1;
-  if (e == hipSuccess) e = hipMalloc(&s.runs, max_waves * 64 * sizeof(uint2));
+  if (e == hipSuccess) e = hipMalloc(&s.runs, max_waves * 64 * sizeof(uint4));
   if (e == hipSuccess) e = hipMalloc(&s.n_runs, max_waves * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.cand, max_waves * 4 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.n_cand, max_waves * sizeof(uint32_t));

@@ -71,9 +71,16 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
                                                   DevScratch sc) {
   __shared__ uint32_t s_lut[2704 / 4];
   __shared__ uint16_t s_off[13 * 8 * 5];
+  __shared__ uint32_t s_binom[7 * 52];  // C(n, k) at [k * 52 + n]
   __shared__ uint8_t s_deck[kDeckStride * kDealWG];
   for (int i = threadIdx.x; i < 2704 / 4; i += kDealWG) s_lut[i] = reinterpret_cast<const uint32_t*>(in.lut)[i];
   for (int i = threadIdx.x; i < 13 * 8 * 5; i += kDealWG) s_off[i] = in.at.offsets[i];
+  for (int i = threadIdx.x; i < 7 * 52; i += kDealWG) {
+    const uint32_t k = (uint32_t)i / 52u, nn = (uint32_t)i - 52u * k;
+    uint64_t c = 1;
+    for (uint32_t j = 0; j < k; ++j) c = c * (nn - j) / (j + 1);  // exact at every step; 0 once nn < k
+    s_binom[i] = nn < k ? 0u : (uint32_t)c;
+  }
   __syncthreads();
   const uint32_t t = blockIdx.x * kDealWG + threadIdx.x;
   if (t >= lc.lanes) return;
@@ -117,16 +124,26 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   }
   const uint8_t* lut = reinterpret_cast<const uint8_t*>(s_lut);
   const size_t L = sc.cap_lanes;
+  // every seat's two bucket-table loads first, so all of them are in flight
+  // while the ranks are computed
+  uint32_t bf[kMaxPlayers], btn[kMaxPlayers];
+#pragma unroll
+  for (int pl = 0; pl < kMaxPlayers; ++pl) {
+    bf[pl] = btn[pl] = 0;
+    if (pl < N) {
+      const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
+      bf[pl] = in.bt.flop5[colex5_tb(s_binom, h0, h1, bc[0], bc[1], bc[2])];
+      btn[pl] = in.bt.turn6[colex6_tb(s_binom, h0, h1, bc[0], bc[1], bc[2], bc[3])];
+    }
+  }
 #pragma unroll
   for (int pl = 0; pl < kMaxPlayers; ++pl) {
     if (pl < N) {
       const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
       const uint32_t hb = lut[h0 * 52 + h1];
-      const uint32_t bf = in.bt.flop5[colex5(h0, h1, bc[0], bc[1], bc[2])];
-      const uint32_t btn = in.bt.turn6[colex6(h0, h1, bc[0], bc[1], bc[2], bc[3])];
       const uint32_t c7[7] = {h0, h1, bc[0], bc[1], bc[2], bc[3], bc[4]};
       const uint32_t r7 = arith_rank7(c7, in.at.nonflush, in.at.flush, s_off);
-      sc.root_bkt[(size_t)pl * L + t] = hb | (bf << 8) | (btn << 16) | (bucket_of_rank(r7) << 24);
+      sc.root_bkt[(size_t)pl * L + t] = hb | (bf[pl] << 8) | (btn[pl] << 16) | (bucket_of_rank(r7) << 24);
       sc.root_rank[(size_t)pl * L + t] = (uint16_t)r7;
     }
   }
@@ -581,21 +598,44 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc)
   }
 }
 
+// A block's records are staged in LDS (when they fit) and written out
+// contiguously, instead of each lane storing its own records.
+constexpr uint32_t kEmitStage = 4096;
 __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
   __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_key[kEmitStage];
+  __shared__ uint2 s_val[kEmitStage];
   const uint32_t l0 = blockIdx.x * kEmitLanes + threadIdx.x * 4;
   const uint32_t w4 = l0 < lc.lanes ? *reinterpret_cast<const uint32_t*>(sc.node_count + l0) : 0u;
-  uint32_t pos = sc.blk_cnt[blockIdx.x] + block_excl_scan_256(count4(w4), s_w, nullptr);
-  if (l0 >= lc.lanes) return;
+  uint32_t total = 0;
+  uint32_t pos = block_excl_scan_256(count4(w4), s_w, &total);
+  const uint32_t gbase = sc.blk_cnt[blockIdx.x];
+  const bool staged = total <= kEmitStage;  // block-uniform
   const size_t L = sc.cap_lanes;
+  if (l0 < lc.lanes) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t t = l0 + (uint32_t)q;
-    const uint32_t c = (w4 >> (8 * q)) & 0xFFu;
-    const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
-    for (uint32_t k = 0; k < c; ++k, ++pos) {
-      sc.key[0][pos] = sc.node_slot[(size_t)k * L + t];
-      sc.val[0][pos] = make_uint2(sc.node_av[(size_t)k * L + t], ub);
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t t = l0 + (uint32_t)q;
+      const uint32_t c = (w4 >> (8 * q)) & 0xFFu;
+      const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
+      for (uint32_t k = 0; k < c; ++k, ++pos) {
+        const uint32_t key = sc.node_slot[(size_t)k * L + t];
+        const uint2 v = make_uint2(sc.node_av[(size_t)k * L + t], ub);
+        if (staged) {
+          s_key[pos] = key;
+          s_val[pos] = v;
+        } else {
+          sc.key[0][gbase + pos] = key;
+          sc.val[0][gbase + pos] = v;
+        }
+      }
+    }
+  }
+  if (staged) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+      sc.key[0][gbase + i] = s_key[i];
+      sc.val[0][gbase + i] = s_val[i];
     }
   }
 }
@@ -807,7 +847,7 @@ __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __r
     const bool defer = head && !is_short;
     const uint64_t sm = __ballot(is_short);
     const uint64_t dm = __ballot(defer);
-    if (is_short) sc.runs[(size_t)w * 64 + (uint32_t)__popcll(sm & lt)] = make_uint2(i, e - i);
+    if (is_short) sc.runs[(size_t)w * 64 + (uint32_t)__popcll(sm & lt)] = make_uint4(i, e - i, key, 0u);
     if (defer) sc.cand[(size_t)w * 4 + (uint32_t)__popcll(dm & lt)] = i;
     if (lane == 0) {
       sc.n_runs[w] = (uint32_t)__popcll(sm);
@@ -829,12 +869,18 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
     for (uint32_t c = 0; c < nc; ++c) {
       const uint32_t i = sc.cand[(size_t)w * 4 + c];
       const uint32_t key = keys[i];
-      uint32_t e = i + 1;
-      while (e < n && e - i <= kShortRun && keys[e] == key) ++e;
+      // the next kShortRun keys, loaded together: e = first index whose key differs
+      uint32_t e = i + 1 + kShortRun;
+#pragma unroll
+      for (int k = (int)kShortRun; k >= 1; --k) {
+        const uint32_t idx = i + (uint32_t)k;
+        if (idx >= n || keys[idx] != key) e = idx;
+      }
       if (e - i <= kShortRun) {
-        sc.runs[(size_t)w * 64 + nr++] = make_uint2(i, e - i);
+        sc.runs[(size_t)w * 64 + nr++] = make_uint4(i, e - i, key, 0u);
         continue;
       }
+      e = i + kShortRun + 1;  // keys[i .. i + kShortRun] all equal key
       // keys[e - 1] == key: bracket the end, then bisect
       uint32_t lo = e, step = 32;
       while (lo + step < n && keys[lo + step - 1] == key) {
@@ -928,11 +974,15 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
   const int a = lane & (ROW - 1);
   const int g_lane0 = lane & ~(ROW - 1);
   const uint32_t g = threadIdx.x / ROW;
+  const bool one_seg = lc.n_segs == 1;
+  SegCtx c0{};
+  if (one_seg) c0 = sc.seg_ctx[0];
+  const uint32_t slot_mask = (uint32_t)((1ull << lc.slot_bits) - 1ull);
   for (uint32_t w = blockIdx.x; w < n_waves; w += gridDim.x)
   for (uint32_t j = g, nr = sc.n_runs[w]; j < nr; j += kGroups) {
-    const uint2 run = sc.runs[(size_t)w * 64 + j];
-    uint32_t slot;
-    const SegCtx c = ctx_of_key(sc.seg_ctx, keys[run.x], lc.slot_bits, &slot);
+    const uint4 run = sc.runs[(size_t)w * 64 + j];
+    const uint32_t slot = run.z & slot_mask;
+    const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
     const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
@@ -969,9 +1019,16 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
     load_f32_row<ROW>(c.R + row_off, rr);
     const float r_a = c.R[row_off + a];
     RowAcc<ROW> acc;
-    for (uint32_t idx = start + (uint32_t)g; idx < end; idx += kGroups) {
-      const uint2 v = vals[idx];
-      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
+    for (uint32_t idx = start + (uint32_t)g; idx < end; idx += 4 * kGroups) {
+      uint2 v[4];  // four records' loads in flight at once
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t k = idx + (uint32_t)(b * kGroups);
+        v[b] = k < end ? vals[k] : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (idx + (uint32_t)(b * kGroups) < end) acc.add(rr, r_a, a, c.num_actions, c.tw, v[b].x, __uint_as_float(v[b].y));
     }
     s_dr[g][a] = acc.dr;
     s_ds[g][a] = acc.ds;
