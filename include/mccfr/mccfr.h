@@ -179,6 +179,25 @@ int mccfr_download_tables(mccfr_ctx* ctx, float* regrets, float* strategy_sum,
 int mccfr_upload_tables(mccfr_ctx* ctx, const float* regrets, const float* strategy_sum,
                         size_t elems, int layout);
 
+/* Average-strategy export in the reference's .bin v2 layout (strategy.cpp:6-52):
+ * u64 magic 0x53545241540100, u32 table_size, u32 num_actions (the row width),
+ * u32 entries, then per entry u32 key (the slot) and f32 probs[num_actions].
+ * The rows written are get_strategy's (cfr_gpu.cu:1383-1412): normalised rows
+ * whose probabilities add to >= 1e-6, in ascending slot order. */
+int mccfr_save_strategy(mccfr_ctx* ctx, uint32_t tenant, const char* path);
+/* Writes host rows in the same layout (strategy.cpp:35-52 writes a host table):
+ * keys[entries], probs[entries x num_actions]. */
+int mccfr_write_strategy(const char* path, uint32_t table_size, uint32_t num_actions, uint32_t entries,
+                         const uint32_t* keys, const float* probs);
+/* Reads a .bin v2 file (strategy.cpp:54-105).  With keys == NULL only the
+ * header is returned and *entries counts the rows a full read returns; else up
+ * to `capacity` rows go to keys / probs (probs: entries x num_actions floats).
+ * Rows that are uniform 1/num_actions within 1e-6 (placeholders of older
+ * exports) are skipped, as the reference's loader does.  A checkpoint or a v1
+ * file is rejected with MCCFR_E_FORMAT. */
+int mccfr_load_strategy(const char* path, uint32_t* table_size, uint32_t* num_actions, uint32_t* entries,
+                        uint32_t* keys, float* probs, size_t capacity);
+
 /* Average strategy S/sum(S) per slot, zero rows where sum <= 1e-7
  * (kernel_normalize_strategy, cfr_gpu.cu:1029-1058), into a host array. */
 int mccfr_normalized_strategy(mccfr_ctx* ctx, float* out, size_t elems, int layout);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -1027,6 +1028,98 @@ int mccfr_tenant_upload(mccfr_ctx* c, uint32_t id, const float* R, const float* 
   if ((rc = put(S, t->hg.S))) return rc;
   t->sigma_dirty = true;  // the next launch regret-matches the whole table first (cfr_gpu.cu:1290)
   if (R && t->cfg.use_cfr_plus) t->clamp_pending = true;
+  return MCCFR_OK;
+}
+
+static constexpr uint64_t kStrategyMagicV1 = 0x53545241540000ull;  // strategy.cpp:10
+static constexpr uint64_t kStrategyMagicV2 = 0x53545241540100ull;  // strategy.cpp:11
+
+int mccfr_write_strategy(const char* path, uint32_t table_size, uint32_t num_actions, uint32_t entries,
+                         const uint32_t* keys, const float* probs) {
+  if (!path || (entries && (!keys || !probs)) || num_actions == 0 || num_actions > MCCFR_MAX_ACTIONS)
+    return fail(nullptr, MCCFR_E_ARG, "bad strategy write arguments");
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return fail(nullptr, MCCFR_E_IO, std::string("cannot open ") + path);
+  const uint64_t magic = kStrategyMagicV2;
+  f.write(reinterpret_cast<const char*>(&magic), 8);
+  f.write(reinterpret_cast<const char*>(&table_size), 4);
+  f.write(reinterpret_cast<const char*>(&num_actions), 4);
+  f.write(reinterpret_cast<const char*>(&entries), 4);
+  for (uint32_t i = 0; i < entries; ++i) {
+    f.write(reinterpret_cast<const char*>(&keys[i]), 4);
+    f.write(reinterpret_cast<const char*>(probs + (size_t)i * num_actions), (std::streamsize)(num_actions * 4));
+  }
+  if (!f) return fail(nullptr, MCCFR_E_IO, std::string("write failed: ") + path);
+  return MCCFR_OK;
+}
+
+int mccfr_save_strategy(mccfr_ctx* c, uint32_t id, const char* path) {
+  if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
+  Tenant* t = get_tenant(c, id);
+  if (!t) return fail(c, MCCFR_E_ARG, "no such tenant");
+  if (t->slots > 0xFFFFFFFFull) return fail(c, MCCFR_E_ARG, "table too large for the .bin header");
+  const size_t n = t->elems();
+  std::vector<float> P(n);
+  int rc = tenant_normalized(c, id, P.data(), n, 0);
+  if (rc) return rc;
+  // get_strategy's row selection (cfr_gpu.cu:1383-1412), ascending slot order
+  std::vector<uint32_t> keys;
+  size_t kept = 0;
+  for (uint64_t s = 0; s < t->slots; ++s) {
+    const float* row = &P[s * t->row];
+    float total = 0.f;
+    for (int a = 0; a < t->row; ++a) total += row[a];
+    if (total < 1e-6f) continue;
+    keys.push_back((uint32_t)s);
+    if (kept != s) std::memmove(&P[kept * t->row], row, t->row * sizeof(float));
+    ++kept;
+  }
+  rc = mccfr_write_strategy(path, (uint32_t)t->slots, (uint32_t)t->row, (uint32_t)keys.size(), keys.data(),
+                            P.data());
+  if (rc) c->err = mccfr_last_error(nullptr);
+  return rc;
+}
+
+int mccfr_load_strategy(const char* path, uint32_t* table_size, uint32_t* num_actions, uint32_t* entries,
+                        uint32_t* keys, float* probs, size_t capacity) {
+  if (!path) return fail(nullptr, MCCFR_E_ARG, "null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(nullptr, MCCFR_E_IO, std::string("cannot open ") + path);
+  uint64_t magic = 0;
+  f.read(reinterpret_cast<char*>(&magic), 8);
+  if (!f) return fail(nullptr, MCCFR_E_IO, std::string("cannot read ") + path);
+  if (magic == kCheckpointMagic)
+    return fail(nullptr, MCCFR_E_FORMAT, std::string(path) + " is a training checkpoint, not a strategy file");
+  if (magic == kStrategyMagicV1)
+    return fail(nullptr, MCCFR_E_FORMAT, std::string(path) + " is an older (v1) strategy export");
+  if (magic != kStrategyMagicV2)
+    return fail(nullptr, MCCFR_E_FORMAT, std::string(path) + " does not have a strategy file header");
+  uint32_t ts = 0, na = 0, n = 0;
+  f.read(reinterpret_cast<char*>(&ts), 4);
+  f.read(reinterpret_cast<char*>(&na), 4);
+  f.read(reinterpret_cast<char*>(&n), 4);
+  if (!f || na == 0 || na > MCCFR_MAX_ACTIONS) return fail(nullptr, MCCFR_E_FORMAT, std::string("bad header in ") + path);
+  if (table_size) *table_size = ts;
+  if (num_actions) *num_actions = na;
+  std::vector<float> row(na);
+  const float uniform = 1.0f / (float)na;
+  uint32_t kept = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t k = 0;
+    f.read(reinterpret_cast<char*>(&k), 4);
+    f.read(reinterpret_cast<char*>(row.data()), (std::streamsize)(na * sizeof(float)));
+    if (!f) return fail(nullptr, MCCFR_E_FORMAT, std::string("truncated strategy file ") + path);
+    bool placeholder = true;  // strategy.cpp:22-33
+    for (uint32_t a = 0; a < na; ++a) placeholder = placeholder && std::fabs(row[a] - uniform) <= 1e-6f;
+    if (placeholder) continue;
+    if (keys) {
+      if (kept >= capacity) return fail(nullptr, MCCFR_E_ARG, "capacity smaller than the file's entries");
+      keys[kept] = k;
+      if (probs) std::memcpy(probs + (size_t)kept * na, row.data(), na * sizeof(float));
+    }
+    ++kept;
+  }
+  if (entries) *entries = kept;
   return MCCFR_OK;
 }
 

@@ -26,8 +26,14 @@ HAND_TABLE_WORDS = 32487834
 FOLD, CHECK, CALL, CHECK_CALL, RAISE_MIN, POT_FRAC, ALL_IN = range(7)
 
 
+# error codes of the C ABI (MCCFR_E_*)
+E_ARG, E_HIP, E_TABLE, E_IO, E_FORMAT, E_NOMEM, E_STATE = -1, -2, -3, -4, -5, -6, -7
+
+
 class MccfrError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: Optional[int] = None):
+        super().__init__(msg)
+        self.code = code
 
 
 class Action(C.Structure):
@@ -149,6 +155,10 @@ _SIGS = {
     "mccfr_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_last_launch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint32)]),
+    "mccfr_save_strategy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),
+    "mccfr_write_strategy": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mccfr_load_strategy": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_size_t]),
     "mccfr_tenant_footprint": (C.c_int, [C.POINTER(GameCfg), C.POINTER(C.c_uint64)]),
     "mccfr_tenant_create": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.POINTER(RootState), C.POINTER(C.c_uint32)]),
     "mccfr_tenant_destroy": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -198,7 +208,7 @@ def exported_symbols() -> Sequence[str]:
 def _check(rc: int, ctx=None) -> None:
     if rc != 0:
         msg = lib().mccfr_last_error(ctx)
-        raise MccfrError(f"libmccfr error {rc}: {msg.decode() if msg else ''}")
+        raise MccfrError(f"libmccfr error {rc}: {msg.decode() if msg else ''}", rc)
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -410,6 +420,9 @@ class Context:
         self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), _ptr(d), n))
         return a, b, d
 
+    def save_strategy(self, path: str, tenant: int = 0):
+        self.check(lib().mccfr_save_strategy(self.h, tenant, path.encode()))
+
     def last_launch_stats(self):
         """(records, long runs, pieces) of the last launch."""
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -513,6 +526,53 @@ class Context:
         return int(wg.value), int(tpl.value), float(ms.value)
 
 
+def load_strategy(path: str):
+    """(table_size, num_actions, keys, probs) of a .bin v2 strategy file."""
+    ts, na, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().mccfr_load_strategy(path.encode(), C.byref(ts), C.byref(na), C.byref(n), None, None, 0))
+    keys = np.empty(n.value, np.uint32)
+    probs = np.empty((n.value, na.value), np.float32)
+    _check(lib().mccfr_load_strategy(path.encode(), C.byref(ts), C.byref(na), C.byref(n), _ptr(keys), _ptr(probs),
+                                     n.value))
+    return int(ts.value), int(na.value), keys, probs
+
+
+def write_strategy(path: str, table_size: int, keys, probs):
+    """Writes rows (keys[n], probs[n, num_actions]) as a .bin v2 strategy file."""
+    keys = np.ascontiguousarray(keys, np.uint32)
+    probs = np.ascontiguousarray(probs, np.float32)
+    if probs.ndim != 2 or probs.shape[0] != keys.shape[0]:
+        raise ValueError("probs must be (len(keys), num_actions)")
+    _check(lib().mccfr_write_strategy(path.encode(), table_size, probs.shape[1], keys.shape[0], _ptr(keys),
+                                      _ptr(probs)))
+
+
+def row_totals(P: np.ndarray) -> np.ndarray:
+    """Float32 row sums added left to right, as the reference's host loop adds them."""
+    t = P[:, 0].astype(np.float32)
+    for a in range(1, P.shape[1]):
+        t = (t + P[:, a]).astype(np.float32)
+    return t
+
+
+def strategy_save(strat: dict, path: str, table_size: int = 1 << 26) -> bool:
+    """strategy.cpp:35-52 for a {key: probs} table; rows in ascending key order."""
+    keys = np.array(sorted(strat), np.uint32)
+    row = len(next(iter(strat.values()))) if strat else 8
+    probs = np.array([strat[int(k)] for k in keys], np.float32).reshape(len(keys), row)
+    try:
+        write_strategy(path, table_size, keys, probs)
+    except MccfrError:
+        return False
+    return True
+
+
+def strategy_load(path: str) -> dict:
+    """strategy.cpp:54-110 without the build-constant check: {key: probs}."""
+    _, _, keys, probs = load_strategy(path)
+    return {int(k): probs[i] for i, k in enumerate(keys)}
+
+
 def tenant_footprint(cfg: GameCfg) -> int:
     b = C.c_uint64()
     _check(lib().mccfr_tenant_footprint(C.byref(cfg), C.byref(b)))
@@ -600,12 +660,18 @@ class Trainer:
         self.ctx.train(total_iterations, batch_size)
 
     def get_strategy(self):
-        """Rows whose strategy sum is >= 1e-6 (cfr_gpu.cu:1383-1412): {slot: probs}."""
-        _, S, _ = self.ctx.download()
-        S = S.reshape(-1, self.ctx.row)
+        """Normalised rows whose probabilities add to >= 1e-6 (cfr_gpu.cu:1383-1412): {slot: probs}."""
         P = self.ctx.normalized_strategy().reshape(-1, self.ctx.row)
-        keep = np.nonzero(S.sum(axis=1) >= 1e-6)[0]
+        keep = np.nonzero(row_totals(P) >= np.float32(1e-6))[0]
         return {int(i): P[i] for i in keep}
+
+    def save_strategy(self, path: str) -> bool:
+        """get_strategy() written as a .bin v2 file (main.cpp:371-375 with strategy.cpp:35-52)."""
+        try:
+            self.ctx.save_strategy(path)
+        except MccfrError:
+            return False
+        return True
 
     def num_info_sets_active(self) -> int:
         _, S, _ = self.ctx.download()
