@@ -7,8 +7,9 @@
 //                 big_blind=20, use_cfr_plus=true, use_linear_cfr=true)
 //   load_hand_table, train, get_strategy, save_checkpoint, load_checkpoint,
 //   num_info_sets_active, export_tables, import_tables.
-// Table geometry is the reference's: 2^26 slots x 8 actions, action-major
-// vectors in export/import (cfr_gpu.cu:1437-1459).  Errors follow the
+// The reference constructor gives the reference geometry (2^26 slots x 8
+// actions); a second constructor takes any mccfr_game_cfg.  export/import use
+// action-major vectors (cfr_gpu.cu:1437-1459).  Errors follow the
 // reference: file and format problems return false; a device error prints the
 // library message and exits (the reference's CUDA_CHECK, cfr_gpu.cu:61-69) --
 // the library itself never exits.
@@ -40,6 +41,13 @@ class GPUCFRTrainer {
     mccfr_game_cfg_reference(&cfg_, num_players, starting_stack, small_blind, big_blind, use_cfr_plus ? 1 : 0,
                              use_linear_cfr ? 1 : 0);
     check(mccfr_create(&cfg_, batch_cap_, &ctx_), "create");
+    shape();
+  }
+  // Any game of the C ABI: menu, raise cap, table size and seed set by the caller.
+  explicit GPUCFRTrainer(const mccfr_game_cfg& cfg, uint32_t batch_cap = 65536)
+      : N_(cfg.num_players), cfg_(cfg), batch_cap_(batch_cap) {
+    check(mccfr_create(&cfg_, batch_cap_, &ctx_), "create");
+    shape();
   }
   ~GPUCFRTrainer() { mccfr_destroy(ctx_); }
   GPUCFRTrainer(const GPUCFRTrainer&) = delete;
@@ -94,21 +102,32 @@ class GPUCFRTrainer {
     std::printf("Done: %.0fM hands in %.2fs (%.1fM hands/sec)\n", hands / 1e6, sec, hands / sec / 1e6);
   }
 
-  // cfr_gpu.cu:1383-1412: normalised average strategy of rows with sum >= 1e-6.
+  // cfr_gpu.cu:1383-1412: normalised average strategy of rows with sum >= 1e-6
+  // (8-action games; save_strategy writes any game).
   HostStrategyTable get_strategy() const {
-    std::vector<float> avg((size_t)GPU_NUM_ACTIONS * GPU_TABLE_SIZE);
+    if (row_ != GPU_NUM_ACTIONS) {
+      std::fprintf(stderr, "get_strategy: the game has %d actions, StrategyEntry holds %d\n", row_, GPU_NUM_ACTIONS);
+      std::exit(1);
+    }
+    std::vector<float> avg(elems());
     check(mccfr_normalized_strategy(ctx_, avg.data(), avg.size(), 0), "normalized_strategy");
     HostStrategyTable table;
-    for (uint32_t s = 0; s < (uint32_t)GPU_TABLE_SIZE; ++s) {
+    for (uint64_t s = 0; s < slots_; ++s) {
       const float* row = &avg[(size_t)s * GPU_NUM_ACTIONS];
       float total = 0.f;
       for (int a = 0; a < GPU_NUM_ACTIONS; ++a) total += row[a];
       if (total < 1e-6f) continue;
       StrategyEntry e;
       for (int a = 0; a < GPU_NUM_ACTIONS; ++a) e.probs[a] = row[a];
-      table.emplace(s, e);
+      table.emplace((uint32_t)s, e);
     }
     return table;
+  }
+  // get_strategy() written as a .bin v2 file (main.cpp:371-375), for any game.
+  bool save_strategy(const std::string& path) const {
+    const int rc = mccfr_save_strategy(ctx_, 0, path.c_str());
+    if (rc != MCCFR_OK) std::fprintf(stderr, "ERROR: %s\n", mccfr_last_error(ctx_));
+    return rc == MCCFR_OK;
   }
 
   bool save_checkpoint(const std::string& path) const { return mccfr_save_checkpoint(ctx_, path.c_str()) == MCCFR_OK; }
@@ -120,27 +139,27 @@ class GPUCFRTrainer {
 
   // cfr_gpu.cu:1414-1426: rows with any positive strategy sum.
   int num_info_sets_active() const {
-    std::vector<float> S((size_t)GPU_NUM_ACTIONS * GPU_TABLE_SIZE);
+    std::vector<float> S(elems());
     check(mccfr_download_tables(ctx_, nullptr, S.data(), nullptr, S.size(), 0), "download");
     int count = 0;
-    for (size_t s = 0; s < (size_t)GPU_TABLE_SIZE; ++s)
-      for (int a = 0; a < GPU_NUM_ACTIONS; ++a)
-        if (S[s * GPU_NUM_ACTIONS + a] > 0.f) {
+    for (size_t s = 0; s < slots_; ++s)
+      for (int a = 0; a < row_; ++a)
+        if (S[s * row_ + a] > 0.f) {
           ++count;
           break;
         }
     return count;
   }
 
-  // cfr_gpu.cu:1437-1459 (action-major vectors of GPU_NUM_ACTIONS * GPU_TABLE_SIZE floats)
+  // cfr_gpu.cu:1437-1459 (action-major vectors of actions x slots floats)
   void export_tables(std::vector<float>& regrets, std::vector<float>& strategy_sum) const {
-    const size_t n = (size_t)GPU_NUM_ACTIONS * GPU_TABLE_SIZE;
+    const size_t n = elems();
     regrets.resize(n);
     strategy_sum.resize(n);
     check(mccfr_download_tables(ctx_, regrets.data(), strategy_sum.data(), nullptr, n, 1), "export_tables");
   }
   void import_tables(const std::vector<float>& regrets, const std::vector<float>& strategy_sum) {
-    const size_t n = (size_t)GPU_NUM_ACTIONS * GPU_TABLE_SIZE;
+    const size_t n = elems();
     if (regrets.size() != n || strategy_sum.size() != n) {
       std::fprintf(stderr, "import_tables: expected %zu floats per table\n", n);
       std::exit(1);
@@ -149,8 +168,17 @@ class GPUCFRTrainer {
   }
 
   mccfr_ctx* context() const { return ctx_; }
+  const mccfr_game_cfg& config() const { return cfg_; }
+  uint64_t table_slots() const { return slots_; }
+  int row_actions() const { return row_; }
 
  private:
+  void shape() {
+    uint32_t row = 0;
+    check(mccfr_tenant_table_shape(ctx_, 0, &slots_, &row), "table_shape");
+    row_ = (int)row;
+  }
+  size_t elems() const { return (size_t)slots_ * row_; }
   void check(int rc, const char* what) const {
     if (rc == MCCFR_OK) return;
     std::fprintf(stderr, "mccfr %s failed (%d): %s\n", what, rc, mccfr_last_error(ctx_));
@@ -161,4 +189,6 @@ class GPUCFRTrainer {
   mccfr_game_cfg cfg_{};
   mccfr_ctx* ctx_ = nullptr;
   uint32_t batch_cap_ = 65536;
+  uint64_t slots_ = 0;
+  int row_ = GPU_NUM_ACTIONS;
 };
