@@ -198,6 +198,36 @@ int mccfr_write_strategy(const char* path, uint32_t table_size, uint32_t num_act
 int mccfr_load_strategy(const char* path, uint32_t* table_size, uint32_t* num_actions, uint32_t* entries,
                         uint32_t* keys, float* probs, size_t capacity);
 
+/* Strategy-quality harness: evaluate_strategy_np (eval.cpp:581-627).  Seat 0
+ * plays the normalised average strategy, seats 1..n-1 the scripted archetype
+ * (eval.h:5-13), for n_hands hands with the dealer button at hand % n and the
+ * game's blinds and starting stack.  Hand h draws from its own Philox stream
+ * keyed by `seed` (a fresh 2n+5-card Fisher-Yates deal, then the decisions'
+ * uniforms); the reference plays one std::mt19937 sequence instead, so its
+ * results are matched in distribution, not hand for hand.  Buckets are the
+ * harness's (abstraction.cpp:191-207: clamped at 127), a strategy row counts
+ * when it adds to >= 1e-6 (cfr_gpu.cu:1398-1401), else the passive fallback
+ * plays.  per_hand (host, n_hands entries, or NULL) gets seat 0's net chips. */
+#define MCCFR_OPP_CALLING_STATION 0
+#define MCCFR_OPP_NIT 1
+#define MCCFR_OPP_MANIAC 2
+#define MCCFR_OPP_BALANCED 3
+#define MCCFR_OPP_RANDOM 4
+typedef struct mccfr_eval_result {
+  int64_t hands_played;
+  int64_t net_chips;  /* seat 0 over all hands */
+  double net_bb;      /* net_chips / big blind */
+  double bb_per_100;  /* eval.cpp:620-623 */
+} mccfr_eval_result;
+int mccfr_evaluate_strategy(mccfr_ctx* ctx, uint32_t tenant, int n_players, int opponent, int64_t n_hands,
+                            uint64_t seed, int32_t* per_hand, mccfr_eval_result* out);
+/* Same for a strategy given as rows, as mccfr_load_strategy returns them
+ * (keys[n_rows], probs[n_rows][num_actions]; num_actions = the game's row
+ * width, 8 for the reference game), over the game `cfg`. */
+int mccfr_evaluate_rows(mccfr_ctx* ctx, const mccfr_game_cfg* cfg, const uint32_t* keys, const float* probs,
+                        uint32_t n_rows, uint32_t num_actions, int n_players, int opponent, int64_t n_hands,
+                        uint64_t seed, int32_t* per_hand, mccfr_eval_result* out);
+
 /* Average strategy S/sum(S) per slot, zero rows where sum <= 1e-7
  * (kernel_normalize_strategy, cfr_gpu.cu:1029-1058), into a host array. */
 int mccfr_normalized_strategy(mccfr_ctx* ctx, float* out, size_t elems, int layout);

@@ -36,7 +36,9 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <climits>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -737,6 +739,361 @@ void run_traversal(const mccfr_game_cfg& cfg, const mccfr_root_state* root, cons
   }
 }
 
+// ===========================================================================
+// Strategy-quality harness (eval.cpp:45-627): seat 0 plays the average
+// strategy, every other seat a scripted archetype, for n hands with the
+// dealer button at hand % n.  Two random streams drive the same game:
+//   EvalPhilox  the project's stream.  Hand h draws word k&3 of
+//               Philox(counter = {k>>2, h_lo, h_hi, kEvalTag}, key = seed):
+//               a fresh deck and 2n+5 partial Fisher-Yates draws (as the
+//               traversal deal), uniforms (x>>8) * 2^-24 in [0, 1), and a
+//               RANDOM opponent's pick x % count.  Hands are independent.
+//   EvalMt      the reference's stream (eval.cpp:599-627, card.h:38-48): one
+//               std::mt19937(seed) for the whole run, one Deck kept across
+//               hands and reshuffled j = rng() % (i + 1) for i = 51..1,
+//               std::uniform_real_distribution<float>(0, total).
+// ===========================================================================
+constexpr uint32_t kEvalTag = 0xE7A1u;
+constexpr int kMaxEvalSeats = 9;
+
+// abstraction.cpp:61-70: the CPU bucket clamps at 127 (the device training
+// walk truncates to uint8 instead, cfr_gpu.cu:488; they differ only for rank 0).
+inline int eval_bucket(uint16_t rank) {
+  return std::min((int)((uint32_t)(rank - 1) * 128u / 7462u), 127);
+}
+
+struct EvalSpec {
+  const mccfr_game_cfg* cfg;
+  const int32_t* hr;
+  const uint8_t* lut;
+  const float* P;  // normalised strategy, slot-major rows of `row`
+  int row;
+  int n;
+  int opp;
+  int stack;
+};
+
+struct EvalHand {
+  int n;
+  uint8_t cards[2 * kMaxEvalSeats + 5];
+  int stacks[kMaxEvalSeats], bets[kMaxEvalSeats], invested[kMaxEvalSeats];
+  bool folded[kMaxEvalSeats], all_in[kMaxEvalSeats];
+  int pot, current_bet, lfr, bb_amt, raises;
+  uint32_t ab;
+  int street;
+  int hb[kMaxEvalSeats], bk[kMaxEvalSeats][4];
+};
+
+struct EvalPhilox {
+  Stream s;
+  EvalPhilox(uint64_t seed, uint64_t hand) : s(seed, hand, kEvalTag) {}
+  void deal(int n, uint8_t* cards) {
+    uint8_t deck[52];
+    for (int i = 0; i < 52; ++i) deck[i] = (uint8_t)i;
+    for (int i = 0; i < 2 * n + 5; ++i) {
+      const uint32_t r = s.next();
+      const int j = i + (int)(((uint64_t)r * (uint64_t)(52 - i)) >> 32);
+      std::swap(deck[i], deck[j]);
+    }
+    std::memcpy(cards, deck, 2 * n + 5);
+  }
+  float uniform(float total) { return (float)(s.next() >> 8) * (1.0f / 16777216.0f) * total; }
+  uint32_t pick(uint32_t count) { return s.next() % count; }
+};
+
+struct EvalMt {
+  std::mt19937 rng;
+  uint8_t deck[52];
+  explicit EvalMt(unsigned seed) : rng(seed) {
+    for (int i = 0; i < 52; ++i) deck[i] = (uint8_t)i;
+  }
+  void deal(int n, uint8_t* cards) {
+    for (int i = 51; i > 0; --i) {
+      const int j = (int)(rng() % (unsigned)(i + 1));
+      std::swap(deck[i], deck[j]);
+    }
+    std::memcpy(cards, deck, 2 * n + 5);
+  }
+  float uniform(float total) { return std::uniform_real_distribution<float>(0.f, total)(rng); }
+  uint32_t pick(uint32_t count) { return (uint32_t)(rng() % count); }
+};
+
+// eval.cpp:49-69
+template <class Rng>
+int eval_sample(const mccfr_game_cfg& g, const float* probs, uint32_t vm, Rng& rng) {
+  const int A = g.num_actions;
+  float total = 0.f;
+  for (int a = 0; a < A; ++a)
+    if (vm >> a & 1u) total += probs[a];
+  if (total < 1e-7f) return passive_fallback(g, vm);
+  const float r = rng.uniform(total);
+  float cum = 0.f;
+  int last = -1;
+  for (int a = 0; a < A; ++a) {
+    if (!(vm >> a & 1u)) continue;
+    cum += probs[a];
+    last = a;
+    if (r <= cum) return a;
+  }
+  return last;
+}
+
+// eval.h:5-13 / eval.cpp:100-157: {fold, call, raise} per archetype; RANDOM = 4.
+struct Archetype {
+  float fold, call, raise;
+};
+constexpr Archetype kArchetypes[4] = {{0.08f, 0.72f, 0.20f}, {0.60f, 0.30f, 0.10f}, {0.10f, 0.20f, 0.70f},
+                                      {0.28f, 0.44f, 0.28f}};
+
+template <class Rng>
+int eval_scripted(const mccfr_game_cfg& g, int opp, uint32_t vm, int to_call, Rng& rng) {
+  const int A = g.num_actions;
+  if (opp == 4) {
+    const uint32_t k = rng.pick((uint32_t)__builtin_popcount(vm));
+    uint32_t m = vm;
+    for (uint32_t i = 0; i < k; ++i) m &= m - 1;
+    return __builtin_ctz(m);
+  }
+  const Archetype& q = kArchetypes[opp];
+  float probs[MCCFR_MAX_ACTIONS] = {};
+  int check_a = -1, call_a = -1, fold_a = -1, n_raise = 0;
+  for (int a = 0; a < A; ++a) {
+    if (!(vm >> a & 1u)) continue;
+    const int k = g.menu[a].kind;
+    if (k == MCCFR_ACT_CHECK || (k == MCCFR_ACT_CHECK_CALL && to_call == 0)) {
+      if (check_a < 0) check_a = a;
+    } else if (k == MCCFR_ACT_CALL || k == MCCFR_ACT_CHECK_CALL) {
+      if (call_a < 0) call_a = a;
+    } else if (k == MCCFR_ACT_FOLD) {
+      if (fold_a < 0) fold_a = a;
+    } else {
+      ++n_raise;
+    }
+  }
+  if (check_a >= 0) {
+    probs[check_a] = q.fold + q.call;
+  } else {
+    if (fold_a >= 0) probs[fold_a] = q.fold;
+    if (call_a >= 0) probs[call_a] = q.call;
+  }
+  if (n_raise > 0) {
+    const float per = q.raise / (float)n_raise;
+    for (int a = 0; a < A; ++a) {
+      const int k = g.menu[a].kind;
+      if ((vm >> a & 1u) && (k == MCCFR_ACT_RAISE_MIN || k == MCCFR_ACT_POT_FRAC || k == MCCFR_ACT_ALL_IN))
+        probs[a] = per;
+    }
+  } else if (check_a >= 0) {
+    probs[check_a] += q.raise;
+  } else if (call_a >= 0) {
+    probs[call_a] += q.raise;
+  } else if (fold_a >= 0) {
+    probs[fold_a] += q.raise;
+  }
+  return eval_sample(g, probs, vm, rng);
+}
+
+// eval.cpp:85-97 with get_strategy's row selection (cfr_gpu.cu:1398-1401).
+template <class Rng>
+int eval_trained(const EvalSpec& E, const EvalHand& h, int p, uint32_t vm, Rng& rng) {
+  const mccfr_game_cfg& g = *E.cfg;
+  const uint32_t key = info_hash((uint8_t)p, (uint8_t)h.hb[p], (uint8_t)h.bk[p][h.street], (uint8_t)h.street, h.ab,
+                                 g.table_log2);
+  const float* row = E.P + (size_t)key * E.row;
+  float t = 0.f;
+  for (int a = 0; a < E.row; ++a) t += row[a];
+  if (!(t >= 1e-6f)) return passive_fallback(g, vm);
+  return eval_sample(g, row, vm, rng);
+}
+
+// Seat 0 plays the strategy, the others the archetype.
+template <class Rng>
+struct EvalPlayers {
+  const EvalSpec& E;
+  Rng& rng;
+  bool bad = false;
+  void deal(int n, uint8_t* cards) { rng.deal(n, cards); }
+  int act(const EvalHand& h, int p, uint32_t vm, int to_call) {
+    return p == 0 ? eval_trained(E, h, p, vm, rng) : eval_scripted(*E.cfg, E.opp, vm, to_call, rng);
+  }
+};
+
+// Cards and every decision given (known-answer hands in the tests).
+struct EvalScript {
+  const uint8_t* cards;
+  const int32_t* actions;
+  int n_actions, used = 0;
+  bool bad = false;
+  void deal(int n, uint8_t* out) { std::memcpy(out, cards, 2 * n + 5); }
+  int act(const EvalHand&, int, uint32_t vm, int) {
+    if (used >= n_actions) { bad = true; return __builtin_ctz(vm); }
+    const int a = actions[used++];
+    if (a < 0 || a >= 32 || !(vm >> a & 1u)) { bad = true; return __builtin_ctz(vm); }
+    return a;
+  }
+};
+
+// eval.cpp:322-391
+template <class Players>
+bool eval_round(const EvalSpec& E, EvalHand& h, int first, Players& pl) {
+  const mccfr_game_cfg& g = *E.cfg;
+  const int n = h.n;
+  bool need[kMaxEvalSeats] = {};
+  for (int p = 0; p < n; ++p) need[p] = !h.folded[p] && !h.all_in[p];
+  int p = first;
+  const int safety = n * n * 4;
+  for (int guard = 0; guard < safety; ++guard) {
+    bool anyone = false;
+    for (int i = 0; i < n; ++i) anyone = anyone || need[i];
+    if (!anyone) break;
+    if (!need[p]) { p = (p + 1) % n; continue; }
+    const int to_call = h.current_bet - h.bets[p];
+    const uint32_t vm = valid_mask(g, h.pot, h.stacks[p], to_call, h.current_bet, h.lfr, h.raises);
+    const int chosen = pl.act(h, p, vm, to_call);
+    h.ab = ((h.ab << g.history_bits) | (uint32_t)chosen) & 0x3FFFFFFFu;
+    need[p] = false;
+    const int kind = g.menu[chosen].kind;
+    if (kind == MCCFR_ACT_FOLD) {
+      h.folded[p] = true;
+      int active = 0;
+      for (int i = 0; i < n; ++i) active += !h.folded[i];
+      if (active <= 1) return false;
+    } else if (kind != MCCFR_ACT_CHECK) {
+      const int chips = chips_for(g, chosen, h.pot, h.stacks[p], to_call, h.current_bet, h.lfr);
+      const int prev = h.current_bet;
+      h.stacks[p] -= chips;
+      h.bets[p] += chips;
+      h.invested[p] += chips;
+      h.pot += chips;
+      if (h.stacks[p] == 0) h.all_in[p] = true;
+      if (h.bets[p] > h.current_bet) {
+        const int raise = h.bets[p] - prev;
+        if (raise >= h.lfr || prev == 0) h.lfr = raise;
+        h.current_bet = h.bets[p];
+        ++h.raises;
+        for (int i = 0; i < n; ++i)
+          if (i != p && !h.folded[i] && !h.all_in[i]) need[i] = true;
+      }
+    }
+    p = (p + 1) % n;
+  }
+  return true;
+}
+
+// eval.cpp:479-572 plus the side-pot showdown of eval.cpp:410-466.  Writes
+// every seat's net chips; a side pot no live seat contested (no eligible
+// winner) is left unpaid.
+template <class Players>
+void eval_hand(const EvalSpec& E, int dealer, Players& pl, int32_t* net) {
+  const mccfr_game_cfg& g = *E.cfg;
+  EvalHand h;
+  const int n = E.n;
+  h.n = n;
+  pl.deal(n, h.cards);
+  for (int p = 0; p < n; ++p) {
+    h.stacks[p] = E.stack;
+    h.bets[p] = h.invested[p] = 0;
+    h.folded[p] = h.all_in[p] = false;
+  }
+  h.pot = h.current_bet = 0;
+  h.lfr = g.big_blind;
+  h.bb_amt = g.big_blind;
+  h.raises = 0;
+  h.ab = 0;
+  h.street = 0;
+  const int sb_pos = n == 2 ? dealer : (dealer + 1) % n;
+  const int bb_pos = n == 2 ? (dealer + 1) % n : (dealer + 2) % n;
+  auto post = [&](int p, int amt) {
+    const int chips = std::min(amt, h.stacks[p]);
+    h.stacks[p] -= chips;
+    h.bets[p] = chips;
+    h.invested[p] = chips;
+    h.pot += chips;
+    if (!h.stacks[p]) h.all_in[p] = true;
+  };
+  post(sb_pos, g.small_blind);
+  post(bb_pos, g.big_blind);
+  h.current_bet = h.bets[bb_pos];
+  const uint8_t* comm = h.cards + 2 * n;
+  for (int p = 0; p < n; ++p) {
+    const uint8_t c[7] = {h.cards[2 * p], h.cards[2 * p + 1], comm[0], comm[1], comm[2], comm[3], comm[4]};
+    h.hb[p] = E.lut[c[0] * 52 + c[1]];
+    h.bk[p][0] = 0;
+    h.bk[p][1] = eval_bucket(walk5(E.hr, c));
+    h.bk[p][2] = eval_bucket(walk6(E.hr, c));
+    h.bk[p][3] = eval_bucket(walk7(E.hr, c));
+  }
+  for (int st = 0; st < 4; ++st) {
+    h.street = st;
+    if (st > 0) {
+      for (int p = 0; p < n; ++p) h.bets[p] = 0;
+      h.current_bet = 0;
+      h.lfr = h.bb_amt;
+      h.raises = 0;
+    }
+    int active = 0, can_act = 0;
+    for (int p = 0; p < n; ++p) {
+      active += !h.folded[p];
+      can_act += !h.folded[p] && !h.all_in[p];
+    }
+    if (active <= 1) break;
+    if (can_act <= 1) {
+      if (st == 3) break;
+      continue;
+    }
+    int first = st == 0 ? (bb_pos + 1) % n : (dealer + 1) % n;
+    if (st > 0)
+      for (int i = 0; i < n; ++i) {
+        const int q = (dealer + 1 + i) % n;
+        if (!h.folded[q] && !h.all_in[q]) { first = q; break; }
+      }
+    if (!eval_round(E, h, first, pl)) break;
+    active = 0;
+    for (int p = 0; p < n; ++p) active += !h.folded[p];
+    if (active <= 1 || st == 3) break;
+  }
+  int last = -1, active = 0;
+  for (int p = 0; p < n; ++p)
+    if (!h.folded[p]) { last = p; ++active; }
+  if (active == 1) {
+    for (int p = 0; p < n; ++p) net[p] = (p == last ? h.pot : 0) - h.invested[p];
+    return;
+  }
+  uint16_t ranks[kMaxEvalSeats];
+  int win[kMaxEvalSeats], rem[kMaxEvalSeats];
+  for (int p = 0; p < n; ++p) {
+    const uint8_t c[7] = {h.cards[2 * p], h.cards[2 * p + 1], comm[0], comm[1], comm[2], comm[3], comm[4]};
+    ranks[p] = h.folded[p] ? 0 : walk7(E.hr, c);
+    win[p] = 0;
+    rem[p] = h.invested[p];
+  }
+  while (true) {
+    int lvl = INT32_MAX;
+    for (int p = 0; p < n; ++p)
+      if (rem[p] > 0 && rem[p] < lvl) lvl = rem[p];
+    if (lvl == INT32_MAX) break;
+    int side = 0;
+    bool elig[kMaxEvalSeats] = {};
+    for (int p = 0; p < n; ++p) {
+      if (rem[p] <= 0) continue;
+      const int c = std::min(rem[p], lvl);
+      side += c;
+      rem[p] -= c;
+      if (!h.folded[p]) elig[p] = true;
+    }
+    uint16_t best = 0;
+    for (int p = 0; p < n; ++p)
+      if (elig[p] && ranks[p] > best) best = ranks[p];
+    int nw = 0;
+    for (int p = 0; p < n; ++p) nw += elig[p] && ranks[p] == best;
+    if (nw == 0) continue;
+    const int share = side / nw;
+    for (int p = 0; p < n; ++p)
+      if (elig[p] && ranks[p] == best) win[p] += share;
+  }
+  for (int p = 0; p < n; ++p) net[p] = win[p] - h.invested[p];
+}
+
 bool cfg_ok(const mccfr_game_cfg* c) {
   if (!c) return false;
   if (c->num_players < 2 || c->num_players > 6) return false;
@@ -1026,6 +1383,78 @@ double orc_cpu_baseline(const mccfr_game_cfg* cfg, const int32_t* hr, const uint
   }
   const auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// ---- strategy-quality harness (eval.cpp:581-627) ----
+int orc_eval_hands(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut, const float* P, int row,
+                   int n_players, int opponent, int stack, uint64_t seed, uint64_t first_hand, int64_t n_hands,
+                   int stream, int threads, int32_t* bot_net, int32_t* seat_net) {
+  if (!cfg || !hr || !lut || !P || n_players < 2 || n_players > kMaxEvalSeats || opponent < 0 || opponent > 4 ||
+      n_hands < 0 || (stream == 1 && first_hand != 0) || stream < 0 || stream > 1)
+    return -1;
+  const EvalSpec E{cfg, hr, lut, P, row, n_players, opponent, stack};
+  const int n = n_players;
+  auto store = [&](int64_t i, const int32_t* net) {
+    if (bot_net) bot_net[i] = net[0];
+    if (seat_net)
+      for (int p = 0; p < n; ++p) seat_net[i * n + p] = net[p];
+  };
+  if (stream == 1) {
+    EvalMt rng((unsigned)seed);
+    EvalPlayers<EvalMt> pl{E, rng};
+    for (int64_t h = 0; h < n_hands; ++h) {
+      int32_t net[kMaxEvalSeats];
+      eval_hand(E, (int)(h % n), pl, net);
+      store(h, net);
+    }
+    return 0;
+  }
+  parallel_for(threads, n_hands, [&](int, int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const uint64_t h = first_hand + (uint64_t)i;
+      EvalPhilox rng(seed, h);
+      EvalPlayers<EvalPhilox> pl{E, rng};
+      int32_t net[kMaxEvalSeats];
+      eval_hand(E, (int)(h % (uint64_t)n), pl, net);
+      store(i, net);
+    }
+  });
+  return 0;
+}
+
+int orc_eval_scripted(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut, int n_players, int stack,
+                      int dealer, const uint8_t* cards, const int32_t* actions, int n_actions, int32_t* seat_net) {
+  if (!cfg || n_players < 2 || n_players > kMaxEvalSeats) return -1;
+  const EvalSpec E{cfg, hr, lut, nullptr, 0, n_players, 0, stack};
+  EvalScript pl{cards, actions, n_actions};
+  eval_hand(E, dealer, pl, seat_net);
+  if (pl.bad) return -2;
+  return pl.used;
+}
+
+int orc_eval_scripted_action(const mccfr_game_cfg* cfg, int opponent, uint32_t vm, int to_call, uint64_t seed,
+                             int64_t draws, int64_t* counts) {
+  if (!cfg || opponent < 0 || opponent > 4 || !vm) return -1;
+  for (int a = 0; a < MCCFR_MAX_ACTIONS; ++a) counts[a] = 0;
+  for (int64_t i = 0; i < draws; ++i) {
+    EvalPhilox rng(seed, (uint64_t)i);
+    ++counts[eval_scripted(*cfg, opponent, vm, to_call, rng)];
+  }
+  return 0;
+}
+
+int orc_eval_buckets(const int32_t* hr, const uint8_t* lut, int n_players, const uint8_t* cards, int32_t* hb,
+                     int32_t* bk) {
+  const uint8_t* comm = cards + 2 * n_players;
+  for (int p = 0; p < n_players; ++p) {
+    const uint8_t c[7] = {cards[2 * p], cards[2 * p + 1], comm[0], comm[1], comm[2], comm[3], comm[4]};
+    hb[p] = lut[c[0] * 52 + c[1]];
+    bk[4 * p + 0] = 0;
+    bk[4 * p + 1] = eval_bucket(walk5(hr, c));
+    bk[4 * p + 2] = eval_bucket(walk6(hr, c));
+    bk[4 * p + 3] = eval_bucket(walk7(hr, c));
+  }
+  return 0;
 }
 
 }  // extern "C"

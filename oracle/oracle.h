@@ -97,6 +97,29 @@ double orc_cpu_baseline(const mccfr_game_cfg* cfg, const int32_t* hr, const uint
                         uint32_t batch, int first_player, int64_t first_iter,
                         uint64_t first_tid, int threads);
 
+/* ---- strategy-quality harness (eval.cpp:45-627).  Seat 0 plays the
+ * normalised strategy P (slot-major rows of `row`; a row is present when it
+ * adds to >= 1e-6, as get_strategy keeps it), seats 1..n-1 the archetype
+ * `opponent` (0 calling station, 1 nit, 2 maniac, 3 balanced, 4 random);
+ * dealer = hand % n.  stream 0: the project's per-hand Philox stream, hands
+ * first_hand.. (threads > 1 splits them); stream 1: the reference's single
+ * std::mt19937(seed) stream with a persistent deck (first_hand must be 0).
+ * bot_net[n_hands] / seat_net[n_hands][n] (either may be NULL) get net chips. */
+int orc_eval_hands(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut, const float* P, int row,
+                   int n_players, int opponent, int stack, uint64_t seed, uint64_t first_hand, int64_t n_hands,
+                   int stream, int threads, int32_t* bot_net, int32_t* seat_net);
+/* One hand with the cards (2n hole cards seat by seat, then 5 board cards) and
+ * every decision given in order; returns decisions used, -2 if an action was
+ * illegal or the list ran out. */
+int orc_eval_scripted(const mccfr_game_cfg* cfg, const int32_t* hr, const uint8_t* lut, int n_players, int stack,
+                      int dealer, const uint8_t* cards, const int32_t* actions, int n_actions, int32_t* seat_net);
+/* Counts of `draws` archetype decisions at legal mask vm (draw i uses hand stream i). */
+int orc_eval_scripted_action(const mccfr_game_cfg* cfg, int opponent, uint32_t vm, int to_call, uint64_t seed,
+                             int64_t draws, int64_t* counts);
+/* The harness's buckets (abstraction.cpp:191-207): hb[n], bk[n][4]. */
+int orc_eval_buckets(const int32_t* hr, const uint8_t* lut, int n_players, const uint8_t* cards, int32_t* hb,
+                     int32_t* bk);
+
 #ifdef __cplusplus
 }
 #endif

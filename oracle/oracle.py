@@ -46,6 +46,11 @@ _SIGS = {
                                          C.c_uint32, _P, _P, _P, _P, _P]),
     "orc_deal_root": (None, [_P, _P, C.c_uint64, C.c_uint32, _P, _P]),
     "orc_regret_matching": (None, [_P, _P, C.c_uint64, C.c_int]),
+    "orc_eval_hands": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                                 C.c_int64, C.c_int, C.c_int, _P, _P]),
+    "orc_eval_scripted": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
+    "orc_eval_scripted_action": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int, C.c_uint64, C.c_int64, _P]),
+    "orc_eval_buckets": (C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
     "orc_cpu_baseline": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_uint32, C.c_int, C.c_int64,
                                       C.c_uint64, C.c_int]),
 }
@@ -80,6 +85,8 @@ def ref_lib():
     L.ref_preflop_bucket.argtypes = [C.c_int, C.c_int]
     L.ref_valid_actions_mask.argtypes = [C.c_int] * 6
     L.ref_action_to_chips.argtypes = [C.c_int] * 7
+    if hasattr(L, "ref_precompute_buckets"):
+        L.ref_precompute_buckets.argtypes = [_P, _P, C.c_int, _P, _P]
     return L
 
 
@@ -132,3 +139,49 @@ def regret_matching(R, row):
     out = np.empty(R.size, np.uint16)
     lib().orc_regret_matching(p(R), p(out), R.size // row, row)
     return out
+
+
+# ---- strategy-quality harness (eval.cpp:45-627) ----
+CALLING_STATION, NIT, MANIAC, BALANCED, RANDOM = range(5)
+
+
+def eval_hands(cfg, hr, lut, P, n_players, opponent, n_hands, seed=42, first_hand=0, stream=0, threads=1,
+               stack=None, seats=False):
+    """Net chips of seat 0 per hand (and of every seat with seats=True)."""
+    row = 8 if cfg.num_actions <= 8 else 16
+    P = np.ascontiguousarray(P, np.float32)
+    bot = np.zeros(n_hands, np.int32)
+    seat = np.zeros((n_hands, n_players), np.int32) if seats else None
+    st = cfg.starting_stack if stack is None else stack
+    rc = lib().orc_eval_hands(C.byref(cfg), p(hr), p(lut), p(P), row, n_players, opponent, st, seed, first_hand,
+                              n_hands, stream, threads, p(bot), p(seat) if seats else None)
+    if rc != 0:
+        raise RuntimeError("orc_eval_hands rejected its arguments")
+    return (bot, seat) if seats else bot
+
+
+def eval_scripted(cfg, hr, lut, n_players, dealer, cards, actions, stack=None):
+    """(decisions used, net chips per seat) of one hand with given cards and decisions."""
+    cards = np.ascontiguousarray(cards, np.uint8)
+    acts = np.ascontiguousarray(actions, np.int32)
+    net = np.zeros(n_players, np.int32)
+    st = cfg.starting_stack if stack is None else stack
+    used = lib().orc_eval_scripted(C.byref(cfg), p(hr), p(lut), n_players, st, dealer, p(cards), p(acts),
+                                   len(acts), p(net))
+    return used, net
+
+
+def eval_scripted_counts(cfg, opponent, vm, to_call, draws, seed=1):
+    counts = np.zeros(16, np.int64)
+    rc = lib().orc_eval_scripted_action(C.byref(cfg), opponent, vm, to_call, seed, draws, p(counts))
+    if rc != 0:
+        raise RuntimeError("orc_eval_scripted_action rejected its arguments")
+    return counts
+
+
+def eval_buckets(hr, lut, n_players, cards):
+    cards = np.ascontiguousarray(cards, np.uint8)
+    hb = np.zeros(n_players, np.int32)
+    bk = np.zeros((n_players, 4), np.int32)
+    lib().orc_eval_buckets(p(hr), p(lut), n_players, p(cards), p(hb), p(bk))
+    return hb, bk

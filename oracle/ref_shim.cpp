@@ -35,4 +35,10 @@ int ref_valid_actions_mask(int pot, int stack, int to_call, int current_bet, int
 int ref_action_to_chips(int a, int pot, int stack, int to_call, int current_bet, int lfr, int bb) {
   return action_to_chips((Action)a, pot, stack, to_call, current_bet, lfr, bb);
 }
+
+// abstraction.cpp:191-207 (the strategy harness's buckets: Chen percentile
+// preflop, clamped rank buckets on flop / turn / river)
+void ref_precompute_buckets(const unsigned char* hole, const unsigned char* community, int n, int* hb, int* bb) {
+  precompute_buckets(hole, community, n, hb, bb);
+}
 }

@@ -1,6 +1,6 @@
 // poker_mccfr — command-line trainer over libmccfr, with the reference CLI's
-// train, benchmark and info modes, flags and defaults (poker_cuda/src/main.cpp:29-69,
-// :98-139, :295-398).
+// train, benchmark, eval and info modes, flags and defaults (poker_cuda/src/main.cpp:29-69,
+// :98-139, :146-198, :295-398).
 //
 // Differences from the reference, all additive or fixes:
 //   * --seed, --table-log2, --menu and --raise-cap configure the game beyond
@@ -12,8 +12,11 @@
 //   * the hand table falls back to the project's generated table when the
 //     default data/handranks.dat is absent (a path given with --handranks must
 //     load, as in the reference);
-//   * one process drives one GPU (no MPI ranks); eval, play and human modes are
-//     CPU tools of the reference that this program does not provide;
+//   * eval mode plays its hands on the GPU (one lane per hand, each hand with
+//     its own Philox stream) instead of one std::mt19937 sequence on the CPU:
+//     the same game and archetypes, results equal in distribution;
+//   * one process drives one GPU (no MPI ranks); the reference's interactive
+//     play and human modes are not provided;
 //   * an unknown or incomplete option is an error (exit 2) rather than ignored.
 #include <hip/hip_runtime.h>
 
@@ -35,7 +38,7 @@ void print_usage(const char* prog) {
       "Usage: %s [options]\n"
       "\n"
       "Training / benchmark flags:\n"
-      "  --mode <train|benchmark|info>       (default: train)\n"
+      "  --mode <train|benchmark|eval|info>  (default: train)\n"
       "  --players  <N>                      (default: 2)\n"
       "  --iters    <N>                      (default: 1000)\n"
       "  --batch    <N>                      (default: 65536)\n"
@@ -56,6 +59,13 @@ void print_usage(const char* prog) {
       "                                      pot, allin and pot fractions N/D\n"
       "                                      (default: fold,check,call,min,1/2,2/3,pot,allin)\n"
       "  --raise-cap  <N>                    aggressive actions per street, 0 = no cap (default: 0)\n"
+      "\n"
+      "Eval flags (--mode eval):\n"
+      "  --strategy  <file>                  strategy .bin (or .ckpt) to evaluate (default: --save)\n"
+      "  --opponent  <type>                  calling_station|nit|maniac|balanced|random\n"
+      "                                      (default: calling_station)\n"
+      "  --hands     <N>                     number of hands to play (default: 10000)\n"
+      "  --players   <N>                     2-9 seats, the strategy in seat 0\n"
       "\n"
       "Benchmark flags:\n"
       "  --batch-sizes <list>                (default: 4096,16384,65536,262144)\n"
@@ -148,6 +158,95 @@ uint32_t strategy_entries(const std::string& path) {
   return n;
 }
 
+// main.cpp:146-198: load a strategy (.bin, or a .ckpt converted through the
+// trainer as strategy_load_auto does, strategy.cpp:112-156), play it against
+// the archetype, print the reference's result lines.
+int run_eval_mode(mccfr_game_cfg cfg, const std::string& path, const std::string& opp_name, int n_players,
+                  long long hands, uint64_t seed, const std::string& hr_path, bool hr_given) {
+  int opp = MCCFR_OPP_CALLING_STATION;  // main.cpp:176-181: unknown names fall back to calling_station
+  if (opp_name == "nit") opp = MCCFR_OPP_NIT;
+  else if (opp_name == "maniac") opp = MCCFR_OPP_MANIAC;
+  else if (opp_name == "balanced") opp = MCCFR_OPP_BALANCED;
+  else if (opp_name == "random") opp = MCCFR_OPP_RANDOM;
+  std::printf("Eval: loading strategy from %s ...\n", path.c_str());
+  uint64_t magic = 0;
+  uint32_t ts = 0, na = 0;
+  {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    const bool ok = f && std::fread(&magic, 8, 1, f) == 1 && std::fread(&ts, 4, 1, f) == 1 &&
+                    std::fread(&na, 4, 1, f) == 1;
+    if (f) std::fclose(f);
+    if (!ok) {
+      std::fprintf(stderr, "ERROR: Could not load strategy from %s\n", path.c_str());
+      return 1;
+    }
+  }
+  constexpr uint64_t kCkpt = 0x43465247505500ull, kV2 = 0x53545241540100ull;
+  if ((magic != kCkpt && magic != kV2) || ts == 0 || (ts & (ts - 1)) != 0) {
+    uint32_t t0 = 0, a0 = 0, e0 = 0;
+    if (mccfr_load_strategy(path.c_str(), &t0, &a0, &e0, nullptr, nullptr, 0) != MCCFR_OK)
+      std::fprintf(stderr, "ERROR: %s\n", mccfr_last_error(nullptr));
+    std::fprintf(stderr, "ERROR: Could not load strategy from %s\n", path.c_str());
+    return 1;
+  }
+  int log2 = 0;
+  while ((1u << log2) < ts) ++log2;
+  cfg.table_log2 = log2;  // the file fixes the table size
+  GPUCFRTrainer bot(cfg, 4096);
+  if (!load_table(bot, hr_path, hr_given)) return 1;
+  if (bot.row_actions() != (int)na) {
+    std::fprintf(stderr, "ERROR: %s holds %u actions per row, this game %d (see --menu)\n", path.c_str(), na,
+                 bot.row_actions());
+    return 1;
+  }
+  mccfr_eval_result res{};
+  int rc;
+  if (magic == kCkpt) {
+    std::fprintf(stderr, "NOTE: %s is a checkpoint; converting it to a playable strategy in memory...\n",
+                 path.c_str());
+    if (!bot.load_checkpoint(path)) {
+      std::fprintf(stderr, "ERROR: Could not load checkpoint from %s\n", path.c_str());
+      return 1;
+    }
+    std::string out = path;
+    if (out.size() >= 5 && out.compare(out.size() - 5, 5, ".ckpt") == 0)
+      out.erase(out.size() - 5);
+    else
+      out += ".bin";
+    if (bot.save_strategy(out)) {
+      std::fprintf(stderr, "NOTE: Exported playable strategy to %s\n", out.c_str());
+      std::printf("  Loaded %u info sets.\n", strategy_entries(out));
+    }
+    std::printf("Opponent: %s  |  Players: %d  |  Hands: %lld\n\n", opp_name.c_str(), n_players, hands);
+    rc = mccfr_evaluate_strategy(bot.context(), 0, n_players, opp, hands, seed, nullptr, &res);
+  } else {
+    uint32_t t1 = 0, a1 = 0, n = 0;
+    if (mccfr_load_strategy(path.c_str(), &t1, &a1, &n, nullptr, nullptr, 0) != MCCFR_OK) {
+      std::fprintf(stderr, "ERROR: %s\n", mccfr_last_error(nullptr));
+      return 1;
+    }
+    std::vector<uint32_t> keys(n);
+    std::vector<float> probs((size_t)n * a1);
+    if (mccfr_load_strategy(path.c_str(), &t1, &a1, &n, keys.data(), probs.data(), keys.size()) != MCCFR_OK) {
+      std::fprintf(stderr, "ERROR: %s\n", mccfr_last_error(nullptr));
+      return 1;
+    }
+    std::printf("  Loaded %u info sets.\n", n);
+    std::printf("Opponent: %s  |  Players: %d  |  Hands: %lld\n\n", opp_name.c_str(), n_players, hands);
+    rc = mccfr_evaluate_rows(bot.context(), &bot.config(), keys.data(), probs.data(), n, a1, n_players, opp, hands,
+                             seed, nullptr, &res);
+  }
+  if (rc != MCCFR_OK) {
+    std::fprintf(stderr, "ERROR: %s\n", mccfr_last_error(bot.context()));
+    return 1;
+  }
+  std::printf("=== Eval Results (%d-player) ===\n", n_players);
+  std::printf("  Hands played : %lld\n", (long long)res.hands_played);
+  std::printf("  Net BB       : %.1f\n", res.net_bb);
+  std::printf("  BB/100       : %+.2f\n", res.bb_per_100);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
@@ -163,6 +262,8 @@ int main(int argc, char* argv[]) {
   int table_log2 = 26, raise_cap = 0;
   std::string menu, batch_sizes = "4096,16384,65536,262144";
   int bench_iters = 10;
+  std::string strategy_path, opponent_str = "calling_station";
+  long long eval_hands = 10000;
 
   for (int i = 1; i < argc; ++i) {
     const char* a = argv[i];
@@ -187,6 +288,9 @@ int main(int argc, char* argv[]) {
     else if (!std::strcmp(a, "--raise-cap") && v) raise_cap = std::atoi(argv[++i]);
     else if (!std::strcmp(a, "--batch-sizes") && v) batch_sizes = argv[++i];
     else if (!std::strcmp(a, "--bench-iters") && v) bench_iters = std::atoi(argv[++i]);
+    else if (!std::strcmp(a, "--strategy") && v) strategy_path = argv[++i];
+    else if (!std::strcmp(a, "--opponent") && v) opponent_str = argv[++i];
+    else if (!std::strcmp(a, "--hands") && v) eval_hands = std::atoll(argv[++i]);
     else if (!std::strcmp(a, "--help")) {
       print_usage(argv[0]);
       return 0;
@@ -200,14 +304,26 @@ int main(int argc, char* argv[]) {
     print_gpu_info();
     return 0;
   }
-  if (mode == "eval" || mode == "play" || mode == "human") {
-    std::fprintf(stderr, "ERROR: --mode %s is a CPU tool of the reference CLI and is not part of poker_mccfr\n",
+  if (mode == "play" || mode == "human") {
+    std::fprintf(stderr, "ERROR: --mode %s is an interactive tool of the reference CLI and is not part of poker_mccfr\n",
                  mode.c_str());
     return 2;
   }
-  if (mode != "train" && mode != "benchmark") {
-    std::fprintf(stderr, "ERROR: unknown --mode %s (train, benchmark or info)\n", mode.c_str());
+  if (mode != "train" && mode != "benchmark" && mode != "eval") {
+    std::fprintf(stderr, "ERROR: unknown --mode %s (train, benchmark, eval or info)\n", mode.c_str());
     return 2;
+  }
+  const int eval_players = n_players;
+  if (mode == "eval") {
+    if (n_players < 2 || n_players > 9) {
+      std::fprintf(stderr, "ERROR: --players must be between 2 and 9 for eval mode\n");
+      return 2;
+    }
+    if (eval_hands < 0) {
+      std::fprintf(stderr, "ERROR: --hands must be >= 0\n");
+      return 2;
+    }
+    if (n_players > 6) n_players = 6;  // the game the strategy was trained on holds at most 6 seats
   }
 
   // The game: the reference's, then the extra flags.
@@ -242,6 +358,9 @@ int main(int argc, char* argv[]) {
 
   std::printf("=== poker_mccfr (MI355X) ===\n");
   print_gpu_info();
+
+  if (mode == "eval") return run_eval_mode(cfg, strategy_path.empty() ? save_path : strategy_path, opponent_str,
+                                           eval_players, eval_hands, (uint64_t)seed, hr_path, hr_given);
 
   if (mode == "train") {
     GPUCFRTrainer trainer(cfg, (uint32_t)batch_size);

@@ -213,6 +213,12 @@ __host__ __device__ inline uint32_t bucket_of_rank(uint32_t rank16) {
   return (((uint32_t)((int)rank16 - 1)) * 128u / 7462u) & 0xFFu;
 }
 
+// The strategy harness's bucket (abstraction.cpp:61-70): clamped at 127.
+__host__ __device__ inline uint32_t eval_bucket_of_rank(uint32_t rank16) {
+  const uint32_t b = ((uint32_t)((int)rank16 - 1)) * 128u / 7462u;
+  return b < 127u ? b : 127u;
+}
+
 // ---------------------------------------------------------------------------
 // Betting rules over a menu (cfr_gpu.cu:186-244; abstraction.cpp:144-234).
 // ---------------------------------------------------------------------------

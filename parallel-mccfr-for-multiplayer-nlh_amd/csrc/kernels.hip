@@ -212,7 +212,9 @@ __global__ __launch_bounds__(256) void k_randomize(float* __restrict__ R, float*
 // the bucket the reference derives from the rank-walk word after K cards
 // (cfr_gpu.cu:114-122, 140-150, 476-489).  The walk state depends only on the
 // card set, so walking the sorted cards gives the reference's word.
-template <int K>
+// CLAMP: the strategy harness's bucket (abstraction.cpp:61-70), min(.., 127),
+// instead of the training walk's uint8 truncation; they differ for rank 0 only.
+template <int K, bool CLAMP>
 __global__ __launch_bounds__(256) void k_build_buckets(const int32_t* __restrict__ hr, uint8_t* __restrict__ out,
                                                        uint32_t count) {
   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
@@ -243,7 +245,8 @@ __global__ __launch_bounds__(256) void k_build_buckets(const int32_t* __restrict
     int p = hr[53 + card_rm(cards[0])];
 #pragma unroll
     for (int i = 1; i < K; ++i) p = hr[p + card_rm(cards[i])];
-    out[idx] = (uint8_t)bucket_of_rank(decode_packed((uint32_t)p & 0xFFFFu));
+    const uint32_t rank = decode_packed((uint32_t)p & 0xFFFFu);
+    out[idx] = (uint8_t)(CLAMP ? eval_bucket_of_rank(rank) : bucket_of_rank(rank));
   }
 }
 
@@ -303,11 +306,18 @@ void launch_randomize(hipStream_t s, int row, int num_actions, float* R, float* 
     hipLaunchKernelGGL(k_randomize<16>, dim3(grid), dim3(256), 0, s, R, S, slots, num_actions, (uint32_t)seed,
                        (uint32_t)(seed >> 32));
 }
-void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b) {
-  hipLaunchKernelGGL(k_build_buckets<5>, dim3(grid_for(kFlopSets, 256, 65536)), dim3(256), 0, s, hr, b.flop5,
-                     (uint32_t)kFlopSets);
-  hipLaunchKernelGGL(k_build_buckets<6>, dim3(grid_for(kTurnSets, 256, 65536)), dim3(256), 0, s, hr, b.turn6,
-                     (uint32_t)kTurnSets);
+void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b, bool clamp) {
+  if (clamp) {
+    hipLaunchKernelGGL((k_build_buckets<5, true>), dim3(grid_for(kFlopSets, 256, 65536)), dim3(256), 0, s, hr,
+                       b.flop5, (uint32_t)kFlopSets);
+    hipLaunchKernelGGL((k_build_buckets<6, true>), dim3(grid_for(kTurnSets, 256, 65536)), dim3(256), 0, s, hr,
+                       b.turn6, (uint32_t)kTurnSets);
+    return;
+  }
+  hipLaunchKernelGGL((k_build_buckets<5, false>), dim3(grid_for(kFlopSets, 256, 65536)), dim3(256), 0, s, hr,
+                     b.flop5, (uint32_t)kFlopSets);
+  hipLaunchKernelGGL((k_build_buckets<6, false>), dim3(grid_for(kTurnSets, 256, 65536)), dim3(256), 0, s, hr,
+                     b.turn6, (uint32_t)kTurnSets);
 }
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 256, 1u << 16)), dim3(256), 0, s, src, dst, n16);

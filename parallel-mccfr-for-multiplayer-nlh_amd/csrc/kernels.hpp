@@ -81,7 +81,7 @@ struct WalkShape {
   uint32_t tpl = 1;
 };
 
-void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b);
+void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b, bool clamp = false);
 void launch_set_tenant(hipStream_t s, DevGame* dst, const DevGame& g);
 void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants, uint32_t n_segs);
 

@@ -28,14 +28,6 @@ namespace {
 std::string g_err;
 std::mutex g_err_mu;
 
-#define HIPX(ctx, call)                                                                      \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail((ctx), e_ == hipErrorOutOfMemory ? MCCFR_E_NOMEM : MCCFR_E_HIP,           \
-                  std::string(#call) + ": " + hipGetErrorString(e_));                       \
-  } while (0)
-
 const std::vector<int32_t>* generated_table() {
   static std::once_flag once;
   static std::vector<int32_t>* tbl = nullptr;
@@ -129,7 +121,7 @@ void free_all(mccfr_ctx* c) {
   for (auto& t : c->tenants)
     if (t.used && t.owned) (void)hipFree(t.owned);
   void* ptrs[] = {c->d_tenants, c->d_segs, c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets,
-                  c->dtrace, c->buckets.flop5, c->buckets.turn6};
+                  c->dtrace, c->buckets.flop5, c->buckets.turn6, c->eval_bk.flop5, c->eval_bk.turn6};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -609,6 +601,7 @@ int mccfr_set_hand_table(mccfr_ctx* c, const int32_t* hr, size_t words) {
   HIPX(c, hipStreamSynchronize(c->stream));
   HIPX(c, hipMemcpy(c->hr, hr, words * sizeof(int32_t), hipMemcpyHostToDevice));
   launch_build_buckets(c->stream, c->hr, c->buckets);  // flop/turn buckets of every 5-/6-card set
+  c->eval_bk_ready = false;                            // the harness's buckets follow the table
   HIPX(c, hipGetLastError());
   HIPX(c, hipStreamSynchronize(c->stream));
   return MCCFR_OK;

@@ -55,6 +55,8 @@ struct mccfr_ctx {
   uint8_t* lut = nullptr;
   mccfr::ArithTables arith{};
   mccfr::BucketTables buckets{};
+  mccfr::BucketTables eval_bk{};  // the strategy harness's clamped buckets, built on first use
+  bool eval_bk_ready = false;
 
   // tenants
   std::vector<mccfr::Tenant> tenants;
@@ -86,6 +88,15 @@ struct mccfr_ctx {
 namespace mccfr {
 
 int fail(mccfr_ctx* c, int code, const std::string& msg);
+
+// Returns from the calling entry point with the HIP error recorded.
+#define HIPX(ctx, call)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ::mccfr::fail((ctx), e_ == hipErrorOutOfMemory ? MCCFR_E_NOMEM : MCCFR_E_HIP,  \
+                           std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
 // Validates a game configuration (+ root) and fills the device descriptor
 // (without table pointers).
 bool make_dev_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, DevGame& g, const char** why);

@@ -26,6 +26,15 @@ HAND_TABLE_WORDS = 32487834
 FOLD, CHECK, CALL, CHECK_CALL, RAISE_MIN, POT_FRAC, ALL_IN = range(7)
 
 
+# scripted archetypes of the strategy harness (MCCFR_OPP_*, eval.h:5-22)
+CALLING_STATION, NIT, MANIAC, BALANCED, RANDOM = range(5)
+
+
+class EvalResult(C.Structure):
+    _fields_ = [("hands_played", C.c_int64), ("net_chips", C.c_int64), ("net_bb", C.c_double),
+                ("bb_per_100", C.c_double)]
+
+
 # error codes of the C ABI (MCCFR_E_*)
 E_ARG, E_HIP, E_TABLE, E_IO, E_FORMAT, E_NOMEM, E_STATE = -1, -2, -3, -4, -5, -6, -7
 
@@ -159,6 +168,11 @@ _SIGS = {
     "mccfr_write_strategy": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mccfr_load_strategy": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_size_t]),
+    "mccfr_evaluate_strategy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_int64, C.c_uint64,
+                                          C.c_void_p, C.c_void_p]),
+    "mccfr_evaluate_rows": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_uint32, C.c_int, C.c_int, C.c_int64, C.c_uint64, C.c_void_p,
+                                      C.c_void_p]),
     "mccfr_tenant_footprint": (C.c_int, [C.POINTER(GameCfg), C.POINTER(C.c_uint64)]),
     "mccfr_tenant_create": (C.c_int, [C.c_void_p, C.POINTER(GameCfg), C.POINTER(RootState), C.POINTER(C.c_uint32)]),
     "mccfr_tenant_destroy": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -419,6 +433,28 @@ class Context:
         d = np.empty(n, np.float32)
         self.check(lib().mccfr_kernel_times(self.h, _ptr(a), _ptr(b), _ptr(d), n))
         return a, b, d
+
+    def evaluate_strategy(self, n_players: int, opponent: int, n_hands: int, seed: int = 42, tenant: int = 0,
+                          per_hand: bool = False):
+        """evaluate_strategy_np (eval.cpp:581-627) on the device: EvalResult (and seat 0's
+        net chips per hand with per_hand=True)."""
+        res = EvalResult()
+        hands = np.empty(n_hands, np.int32) if per_hand else None
+        self.check(lib().mccfr_evaluate_strategy(self.h, tenant, n_players, opponent, n_hands, seed,
+                                                 _ptr(hands) if per_hand else None, C.byref(res)))
+        return (res, hands) if per_hand else res
+
+    def evaluate_rows(self, cfg: GameCfg, keys, probs, n_players: int, opponent: int, n_hands: int,
+                      seed: int = 42, per_hand: bool = False):
+        """The harness over a strategy given as rows (as load_strategy returns them)."""
+        keys = np.ascontiguousarray(keys, np.uint32)
+        probs = np.ascontiguousarray(probs, np.float32)
+        res = EvalResult()
+        hands = np.empty(n_hands, np.int32) if per_hand else None
+        self.check(lib().mccfr_evaluate_rows(self.h, C.byref(cfg), _ptr(keys), _ptr(probs), keys.size,
+                                             probs.shape[1] if probs.ndim == 2 else 8, n_players, opponent,
+                                             n_hands, seed, _ptr(hands) if per_hand else None, C.byref(res)))
+        return (res, hands) if per_hand else res
 
     def save_strategy(self, path: str, tenant: int = 0):
         self.check(lib().mccfr_save_strategy(self.h, tenant, path.encode()))

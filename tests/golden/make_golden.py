@@ -10,6 +10,9 @@ Fixtures (data only):
   hand_walks.npz    4,096 seeded 7-card deals with the reference's
                     evaluate_best at 3/4/5 board cards and evaluate_7cards, run on
                     the generated rank-walk table (hand_eval.cpp:107-173)
+  eval_buckets.npz  2,048 seeded 9-seat deals (18 hole + 5 board cards) with the
+                    reference's precompute_buckets, the buckets its strategy
+                    harness hashes (abstraction.cpp:191-207)
 
 Usage: python tests/golden/make_golden.py   (needs oracle/_ref/libref.so)
 """
@@ -74,7 +77,18 @@ def main():
             ranks[i, k] = ref.ref_evaluate_best(oracle.p(h), oracle.p(b), n_comm)
         ranks[i, 3] = ref.ref_evaluate_7cards(oracle.p(np.ascontiguousarray(c)))
     np.savez_compressed(os.path.join(OUT, "hand_walks.npz"), cards=cards, ranks=ranks)
-    print("wrote preflop_lut.bin, rules_grid.npz, hand_walks.npz")
+
+    rng = np.random.default_rng(20261204)
+    deals = np.stack([rng.permutation(52)[:23] for _ in range(2048)]).astype(np.uint8)
+    hb = np.zeros((len(deals), 9), np.int32)
+    bk = np.zeros((len(deals), 36), np.int32)
+    for i, d in enumerate(deals):
+        hole = np.ascontiguousarray(d[:18])
+        comm = np.ascontiguousarray(d[18:])
+        ref.ref_precompute_buckets(oracle.p(hole), oracle.p(comm), 9, oracle.p(hb[i]), oracle.p(bk[i]))
+    np.savez_compressed(os.path.join(OUT, "eval_buckets.npz"), cards=deals, hole_buckets=hb,
+                        board_buckets=bk.reshape(-1, 9, 4).astype(np.uint8))
+    print("wrote preflop_lut.bin, rules_grid.npz, hand_walks.npz, eval_buckets.npz")
 
 
 if __name__ == "__main__":

@@ -32,7 +32,7 @@ def test_help_lists_the_reference_flags():
         line = next(l for l in r.stdout.splitlines() if l.strip().startswith(flag + " "))
         assert f"default: {default}" in line, line
     for flag in ("--load", "--no-cfrplus", "--cfrplus", "--no-lcfr", "--lcfr", "--menu", "--batch-sizes",
-                 "--bench-iters"):
+                 "--bench-iters", "--strategy", "--opponent", "--hands"):
         assert flag in r.stdout
 
 
@@ -43,7 +43,8 @@ def test_help_lists_the_reference_flags():
     (["--players", "9"], "num_players"),
     (["--table-log2", "40"], "table_log2"),
     (["--batch-sizes", "4096,x"], "bad --batch-sizes entry 'x'"),
-    (["--mode", "eval"], "not part of poker_mccfr"),
+    (["--mode", "play"], "not part of poker_mccfr"),
+    (["--mode", "eval", "--players", "10"], "between 2 and 9 for eval mode"),
     (["--mode", "fly"], "unknown --mode fly"),
     (["--frobnicate"], "unknown or incomplete option --frobnicate"),
     (["--iters"], "unknown or incomplete option --iters"),
@@ -130,3 +131,49 @@ def test_benchmark_mode_prints_one_line_per_batch_size():
         assert m and int(m.group(1)) == bs, l
         assert float(m.group(4)) > 0
     assert "setup excluded" in r.stdout
+
+
+def eval_lines(out):
+    vals = {}
+    for key, pat in [("hands", r"Hands played : (\d+)"), ("net_bb", r"Net BB       : (\S+)"),
+                     ("bb100", r"BB/100       : (\S+)"), ("loaded", r"Loaded (\d+) info sets")]:
+        m = re.search(pat, out)
+        vals[key] = float(m.group(1)) if m else None
+    return vals
+
+
+@pytest.mark.gpu
+def test_eval_mode_on_strategy_and_checkpoint(tmp_path, hand_table):
+    """main.cpp:146-198 on the device: a .bin is played from its rows, a .ckpt is
+    converted through the trainer (and the converted .bin written next to it);
+    both give the library's result for the same hands."""
+    s = tmp_path / "e.bin"
+    common = ["--players", 3, "--table-log2", 15, "--handranks", "generated"]
+    assert run(*common, "--iters", 4, "--batch", 4096, "--save", s).returncode == 0
+    print("poker_mccfr eval", flush=True)
+    r = run("--mode", "eval", "--strategy", s, "--opponent", "nit", "--hands", 20000, *common)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = eval_lines(r.stdout)
+    cfg = cli_game(3, 15, 42)
+    ts, na, keys, probs = mccfr.load_strategy(str(s))
+    with mccfr.Context(cfg, 4096) as ctx:
+        ctx.use_generated_hand_table()
+        want = ctx.evaluate_rows(cfg, keys, probs, 3, mccfr.NIT, 20000, seed=42)
+    assert got["hands"] == 20000 and got["loaded"] == keys.size
+    assert got["net_bb"] == pytest.approx(want.net_bb, abs=0.051)
+    assert got["bb100"] == pytest.approx(want.bb_per_100, abs=0.0051)
+    # the checkpoint gives the same hands: its normalised rows are the .bin's rows
+    ck = tmp_path / "e.bin.ckpt"
+    before = open(s, "rb").read()
+    os.remove(s)
+    r2 = run("--mode", "eval", "--strategy", ck, "--opponent", "nit", "--hands", 20000, *common)
+    assert r2.returncode == 0, r2.stderr[-2000:]
+    assert "Exported playable strategy to" in r2.stderr
+    assert eval_lines(r2.stdout)["bb100"] == got["bb100"]
+    # x.ckpt converts to x (strategy.cpp:140-146): here the .bin training wrote
+    assert open(s, "rb").read() == before
+    # nine seats against random opponents run too
+    r3 = run("--mode", "eval", "--strategy", s, "--opponent", "random", "--hands", 5000, "--players", 9,
+             "--table-log2", 15, "--handranks", "generated")
+    assert r3.returncode == 0, r3.stderr[-2000:]
+    assert "=== Eval Results (9-player) ===" in r3.stdout
