@@ -111,6 +111,31 @@ def cpu_baseline(mccfr, ctx, cfg, launch, tid, batch, launches, threads):
                       f"{secs:.1f} s"}
 
 
+def harness_bench(mccfr, ctx, cfg, n_players=6, opponent=3, log2_hands=22, cpu_hands=100_000):
+    """The strategy-quality harness (evaluate_strategy_np, eval.cpp:581-627) on the
+    trained tables: device hands/s, and the oracle's hands/s on one thread with the
+    reference's own stream (std::mt19937, one sequence) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n = 1 << log2_hands
+    ctx.evaluate_strategy(n_players, opponent, 1 << 16, seed=1)  # builds the harness buckets
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = ctx.evaluate_strategy(n_players, opponent, n, seed=42)
+    gpu_s = time.perf_counter() - t0
+    P = ctx.normalized_strategy().reshape(-1, ctx.row)
+    hr = mccfr.generate_hand_table()
+    lut = mccfr.preflop_lut()
+    t0 = time.perf_counter()
+    ref = oracle.eval_hands(cfg, hr, lut, P, n_players, opponent, cpu_hands, seed=42, stream=1)
+    cpu_s = time.perf_counter() - t0
+    return {"hands": n, "players": n_players, "opponent": "balanced", "gpu_hands_per_s": n / gpu_s,
+            "gpu_ms": gpu_s * 1e3, "bb_per_100": res.bb_per_100,
+            "cpu_reference_stream": {"hands_per_s": cpu_hands / cpu_s, "cores": 1, "hands": cpu_hands,
+                                     "bb_per_100": float(ref.astype(np.float64).sum()) / cfg.big_blind
+                                     / cpu_hands * 100.0}}
+
+
 def batched_bench(mccfr, workloads, hr, launches, n_games=32, table_log2=20, total=1 << 21):
     """Uneven games in shared launches vs each game in its own context."""
     games, counts = workloads.uneven_mix(seed=2, n_games=n_games, table_log2=table_log2, traversals=total)
@@ -272,6 +297,7 @@ def main():
     ap.add_argument("--no-hand-bench", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--profile", action="store_true", help="headline launches only (for rocprofv3 runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="with --profile: also run a plain device copy of this size (byte-counter calibration)")
@@ -393,6 +419,12 @@ def main():
         log(f"bench: CPU baseline on {threads} threads")
         result["cpu_baseline"] = cpu_baseline(mccfr, ctx, cfg, state["launch"], state["tid"], B,
                                               args.cpu_launches, threads)
+    if not args.no_harness:
+        log("bench: strategy-quality harness")
+        result["strategy_harness"] = harness_bench(mccfr, ctx, cfg)
+        log(f"bench: harness {result['strategy_harness']['gpu_hands_per_s'] / 1e6:.1f} M hands/s on the GPU, "
+            f"{result['strategy_harness']['cpu_reference_stream']['hands_per_s'] / 1e6:.3f} M hands/s "
+            f"CPU (1 thread)")
     ctx.close()
 
     if not args.no_batched:

@@ -27,7 +27,11 @@ struct DevScratch {
   uint8_t* node_count;   // [lanes] update-player nodes of the traversal (0 on padding lanes)
   float* node_u;         // [lanes] terminal utility of the update player
   uint32_t* root_bkt;    // [kMaxPlayers][lanes] hole | flop << 8 | turn << 16 | river << 24 buckets
+                         // (turn = 0xFF until the walk fetches it; real buckets are <= 198)
   uint16_t* root_rank;   // [kMaxPlayers][lanes] 7-card rank (showdown)
+  uint32_t* root_tidx;   // [kMaxPlayers][lanes] colex rank of hole + first four board cards:
+                         // the turn bucket's index, read only when the seat acts on the turn
+  const uint8_t* turn6;  // the turn bucket table (set for each walk launch)
   uint8_t* root_dealer;  // [lanes]
   uint32_t* blk_cnt;     // [lanes / kEmitLanes] records per emission block, then their offsets
   uint32_t* key[2];      // [kMaxDepth * lanes] record keys (segment << slot_bits | slot), ping-pong

@@ -43,6 +43,10 @@ constexpr int kDealWG = 256;
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Turn byte of a seat's bucket word before the walk has loaded it (the
+// device's buckets are at most 198: bucket_of_rank of a decoded rank <= 11547).
+constexpr uint32_t kTurnUnknown = 0xFFu;
+
 // Segment holding `lane` (lane0 ascending; the caller passes a wave-uniform lane).
 __device__ __forceinline__ uint32_t find_seg(const DevSeg* __restrict__ segs, uint32_t n_segs, uint32_t lane) {
   uint32_t lo = 0, hi = n_segs;
@@ -124,16 +128,18 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   }
   const uint8_t* lut = reinterpret_cast<const uint8_t*>(s_lut);
   const size_t L = sc.cap_lanes;
-  // every seat's two bucket-table loads first, so all of them are in flight
-  // while the ranks are computed
-  uint32_t bf[kMaxPlayers], btn[kMaxPlayers];
+  // every seat's flop-bucket load first (a 2.6 MB table: L2 hits), so all of
+  // them are in flight while the ranks are computed.  The turn bucket (a 20 MB
+  // table: a sector from memory per load) is left to the walk, which loads it
+  // only for the seats that act on the turn; its index is stored here.
+  uint32_t bf[kMaxPlayers];
 #pragma unroll
   for (int pl = 0; pl < kMaxPlayers; ++pl) {
-    bf[pl] = btn[pl] = 0;
+    bf[pl] = 0;
     if (pl < N) {
       const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
       bf[pl] = in.bt.flop5[colex5_tb(s_binom, h0, h1, bc[0], bc[1], bc[2])];
-      btn[pl] = in.bt.turn6[colex6_tb(s_binom, h0, h1, bc[0], bc[1], bc[2], bc[3])];
+      sc.root_tidx[(size_t)pl * L + t] = colex6_tb(s_binom, h0, h1, bc[0], bc[1], bc[2], bc[3]);
     }
   }
 #pragma unroll
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
       const uint32_t hb = lut[h0 * 52 + h1];
       const uint32_t c7[7] = {h0, h1, bc[0], bc[1], bc[2], bc[3], bc[4]};
       const uint32_t r7 = arith_rank7(c7, in.at.nonflush, in.at.flush, s_off);
-      sc.root_bkt[(size_t)pl * L + t] = hb | (bf[pl] << 8) | (btn[pl] << 16) | (bucket_of_rank(r7) << 24);
+      sc.root_bkt[(size_t)pl * L + t] = hb | (bf[pl] << 8) | (kTurnUnknown << 16) | (bucket_of_rank(r7) << 24);
       sc.root_rank[(size_t)pl * L + t] = (uint16_t)r7;
     }
   }
@@ -382,7 +388,12 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
       const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
       const int to_call = current_bet - bet;
       const uint32_t vm = rules_mask(rules, m_kind, m_num, m_den, pot, stk, to_call, current_bet, lfr, raises);
-      const uint32_t Bp = bk[player * WG];
+      uint32_t Bp = bk[player * WG];
+      if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
+        const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
+        Bp = (Bp & 0xFF00FFFFu) | (tb << 16);
+        bk[player * WG] = Bp;
+      }
       const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
       const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
       float sg[ROW];
@@ -1101,10 +1112,12 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
                  hipEvent_t after_deal) {
   hipLaunchKernelGGL(k_deal, dim3(lc.lanes / kDealWG), dim3(kDealWG), 0, s, lc, segs, tenants, in, sc);
   if (after_deal) (void)hipEventRecord(after_deal, s);
+  DevScratch ws = sc;
+  ws.turn6 = in.bt.turn6;
   if (row == 8)
-    walk_dispatch<8>(s, lc, segs, tenants, sc, shape, trace);
+    walk_dispatch<8>(s, lc, segs, tenants, ws, shape, trace);
   else
-    walk_dispatch<16>(s, lc, segs, tenants, sc, shape, trace);
+    walk_dispatch<16>(s, lc, segs, tenants, ws, shape, trace);
 }
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
