@@ -154,8 +154,10 @@ int mccfr_load_checkpoint(mccfr_ctx* ctx, const char* path);
 
 /* Launch shape of the walk kernel: workgroup size (64, 128, 256, 512 or
  * 1024 lanes) and traversals per lane (1, 2 or 4; a lane walks its traversals
- * one after another, so a wave covers 64, 128 or 256 traversals).  Defaults
- * 256 and 1.  Results do not depend on the shape. */
+ * one after another, so a wave covers 64, 128 or 256 traversals; 0: one wave
+ * per resident wave slot of the device, each lane walking every (slots x 64)-th
+ * traversal, for launches of one tenant; launches of several tenants then use
+ * 1).  Defaults 256 and 1.  Results do not depend on the shape. */
 int mccfr_set_launch_shape(mccfr_ctx* ctx, uint32_t walk_wg, uint32_t traversals_per_lane);
 int mccfr_get_launch_shape(const mccfr_ctx* ctx, uint32_t* walk_wg, uint32_t* traversals_per_lane);
 

@@ -864,7 +864,8 @@ int mccfr_set_launch_shape(mccfr_ctx* c, uint32_t walk_wg, uint32_t tpl) {
   if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
   if (walk_wg != 64 && walk_wg != 128 && walk_wg != 256 && walk_wg != 512 && walk_wg != 1024)
     return fail(c, MCCFR_E_ARG, "walk workgroup must be 64, 128, 256, 512 or 1024");
-  if (tpl != 1 && tpl != 2 && tpl != 4) return fail(c, MCCFR_E_ARG, "traversals per lane must be 1, 2 or 4");
+  if (tpl != 0 && tpl != 1 && tpl != 2 && tpl != 4)
+    return fail(c, MCCFR_E_ARG, "traversals per lane must be 1, 2 or 4 (0: persistent)");
   if (c->shape.wg != walk_wg || c->shape.tpl != tpl) drop_graph(c);
   c->shape.wg = walk_wg;
   c->shape.tpl = tpl;
@@ -903,10 +904,11 @@ int mccfr_autotune(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, int
   WalkShape best = keep;
   float best_ms = 1e30f;
   const uint32_t wgs[5] = {64, 128, 256, 512, 1024};
-  const uint32_t tpls[3] = {1, 2, 4};
+  const uint32_t tpls[4] = {1, 2, 4, 0};
   hipError_t err = hipSuccess;
   for (uint32_t wg : wgs)
     for (uint32_t tp : tpls) {
+      if (tp == 0 && p.lc.n_segs > 1) continue;  // persistent waves serve one-tenant launches only
       WalkShape sh;
       sh.wg = wg;
       sh.tpl = tp;

@@ -47,6 +47,9 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgc
 // device's buckets are at most 198: bucket_of_rank of a decoded rank <= 11547).
 constexpr uint32_t kTurnUnknown = 0xFFu;
 
+// Waiting lanes (with a next traversal) that trigger a refill round of the walk.
+constexpr int kRefillLanes = 48;
+
 // Segment holding `lane` (lane0 ascending; the caller passes a wave-uniform lane).
 __device__ __forceinline__ uint32_t find_seg(const DevSeg* __restrict__ segs, uint32_t n_segs, uint32_t lane) {
   uint32_t lo = 0, hi = n_segs;
@@ -182,8 +185,11 @@ __device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, 
 }
 
 // ---------------------------------------------------------------------------
-// k_walk.  Wave w covers lanes [w*64*tpl, (w+1)*64*tpl); in round j its lane
-// l walks the traversal at lane w*64*tpl + j*64 + l.
+// k_walk.  Wave w's lane l walks the traversals at lanes base + j * jstride + l
+// for j < tpl: contiguous (jstride = 64, base = w * 64 * tpl: the wave covers
+// 64 * tpl lanes of one segment) or, for single-segment launches, persistent
+// (one wave per resident wave slot of the device, base = w * 64, jstride = 64
+// times the number of waves).
 //
 // Per-lane state: seat stacks and bets in LDS (field-major: lane-consecutive
 // banks), a bet counts only while its 2-bit street tag matches the street, so
@@ -234,6 +240,7 @@ __device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint8_t* ki
 template <int ROW, int WG, bool TRACE>
 __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                              const DevGame* __restrict__ tenants, DevScratch sc, uint32_t tpl,
+                                             uint32_t jstride,
                                              mccfr_trace* __restrict__ trace) {
   __shared__ uint32_t s_sb[kMaxPlayers * WG];    // stack | bet << 15 | street << 30
   __shared__ uint32_t s_bk[kMaxPlayers * WG];    // buckets: hole | flop << 8 | turn << 16 | river << 24
@@ -243,7 +250,8 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   __shared__ uint8_t s_menu[WG / 64][3][MCCFR_MAX_ACTIONS];  // kind, num, den of the wave's tenant
 
   const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
-  const uint32_t base = uniform((uint32_t)((blockIdx.x * WG + tx) >> 6) * 64u * tpl);
+  const uint32_t base =
+      uniform((uint32_t)((blockIdx.x * WG + tx) >> 6) * 64u * (jstride == 64u ? tpl : 1u));
   if (base >= lc.lanes) return;
   const uint32_t si = find_seg(segs, lc.n_segs, base);
   const DevSeg& Sg = segs[si];
@@ -295,239 +303,274 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
 
-  for (uint32_t j = 0; j < tpl; ++j) {
-    const uint32_t t = base + j * 64u + (uint32_t)lane;
-    const bool active = t - lane0 < seg_n;
-    if (!__any(active)) break;
+  // Each lane walks the traversals t_j = base + j * jstride + lane, j < tpl, one
+  // after another.  A lane whose traversal has ended waits until kRefillLanes
+  // lanes that have a next traversal are waiting, or until no lane walks; then
+  // every waiting lane finishes its traversal (showdown, stores) and starts
+  // its next one.  The finishing code runs for many lanes at once, and lanes
+  // are not held idle until the longest traversal of the wave has ended.
+  DrawStream rng;
+  uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
+  int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
+  int npend = 0, decisions = 0, upd_legal = 0;
+  int status = 5;  // 0 walking, 1 fold, 2 showdown, 3 depth limit, 5 waiting to start, 6 no traversal left
+  float u = 0.0f;
+  uint32_t t = 0, jn = 0;
 
-    DrawStream rng;
-    uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
-    int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
-    int npend = 0, decisions = 0, upd_legal = 0;
-    int status = 4;  // 0 walking, 1 fold, 2 showdown, 3 depth limit, 4 no traversal
-    float u = 0.0f;
-
-    // Root state (cfr_gpu.cu:954-981: stacks, blinds, preflop queue from the
-    // seat after the big blind; or the tenant's postflop root with the queue
-    // from the seat after the dealer, as at a street change, cfr_gpu.cu:676-686).
-    if (active) {
+  while (true) {
+    const uint32_t t_next = base + jn * jstride + (uint32_t)lane;
+    const bool has_next = jn < tpl && t_next - lane0 < seg_n;
+    const uint64_t walking = __ballot(status == 0);
+    const uint64_t waiting = __ballot(status != 0 && status != 6);
+    const uint64_t ready = __ballot(status != 0 && status != 6 && has_next);
+    if (waiting == 0 && walking == 0) break;
+    if (waiting != 0 && (walking == 0 || __popcll(ready) >= kRefillLanes)) {
+      // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending
+      // order, for every lane of the wave that reached one
+      if (__any(status == 2) && status == 2) {
+        int inv[kMaxPlayers];
+        uint32_t rkv[kMaxPlayers];
 #pragma unroll
-      for (int p = 0; p < kMaxPlayers; ++p) {
-        if (p < N) {
-          sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
-          bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
-          rk[p * WG] = sc.root_rank[(size_t)p * L + t];
+        for (int p = 0; p < kMaxPlayers; ++p) {
+          inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
+          rkv[p] = p < N ? (uint32_t)rk[p * WG] : 0u;
+        }
+        const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
+        const uint32_t rk_up = rk[up * WG];
+        const bool up_in = !(folded >> up & 1u);
+        float pay = -(float)inv_up;
+        int prev = 0;
+#pragma unroll
+        for (int it = 0; it < kMaxPlayers; ++it) {
+          int lvl = 0x7FFFFFFF;
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p)
+            if (inv[p] > prev && inv[p] < lvl) lvl = inv[p];
+          if (lvl != 0x7FFFFFFF) {
+            int layer_pot = 0;
+            uint32_t best = 0;
+#pragma unroll
+            for (int p = 0; p < kMaxPlayers; ++p) {
+              const int c = (inv[p] < lvl ? inv[p] : lvl) - prev;
+              layer_pot += c > 0 ? c : 0;
+              if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] > best) best = rkv[p];
+            }
+            if (best > 0) {
+              int n_win = 0;
+#pragma unroll
+              for (int p = 0; p < kMaxPlayers; ++p)
+                if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] == best) ++n_win;
+              if (up_in && inv_up >= lvl && rk_up == best) pay += (float)layer_pot / (float)n_win;
+            }
+            prev = lvl;
+          }
+        }
+        u = pay;
+      }
+      if (status >= 1 && status <= 3) {
+        if (status == 3) u = 0.0f;  // depth guard (cfr_gpu.cu:661)
+        if (TRACE) {
+          mccfr_trace r;
+          r.utility = u;
+          r.path_hash = phash;
+          r.decisions = (uint16_t)decisions;
+          r.updates = (uint8_t)npend;
+          r.terminal = (uint8_t)status;
+          r.update_legal = (uint16_t)upd_legal;
+          r.reserved = 0;
+          trace[t] = r;
+        }
+        sc.node_count[t] = (uint8_t)npend;
+        sc.node_u[t] = u;
+        const int kr = npend < kRegPending ? npend : kRegPending;
+        for (int k = 0; k < kr; ++k) {
+          sc.node_slot[(size_t)k * L + t] = ps[k * WG];
+          sc.node_av[(size_t)k * L + t] = pa[k * WG];
         }
       }
-      dealer = sc.root_dealer[t];
-      if (root_street == 0) {
-        const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
-        const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
-        const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
-        const int sb_amt = small_blind < s_sb ? small_blind : s_sb;
-        sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
-        if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
-        const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
-        const int bb_amt = rules.bb < s_bb ? rules.bb : s_bb;
-        sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 15);
-        if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
-        pot = sb_amt + bb_amt;
-        current_bet = bb_amt;
-        qmask = allmask & ~all_in;
-        qpos = wrap(bb_pos + 1, N);
-      } else {
-        folded = root_folded;
-        all_in = root_allin;
-        pot = root_pot;
-        street = root_street;
-        ab = root_ab;
-        qmask = allmask & ~(folded | all_in);
-        qpos = wrap(dealer + 1, N);
+      if (status != 0 && status != 6) {
+        status = 6;
+        if (has_next) {
+          t = t_next;
+          ++jn;
+          folded = 0;
+          all_in = 0;
+          ab = 0;
+          qmask = 0;
+          phash = 2166136261u;
+          qpos = 0;
+          pot = 0;
+          current_bet = 0;
+          lfr = rules.bb;
+          street = 0;
+          raises = 0;
+          depth = 0;
+          npend = 0;
+          decisions = 0;
+          upd_legal = 0;
+          u = 0.0f;
+          // Root state (cfr_gpu.cu:954-981: stacks, blinds, preflop queue from the
+          // seat after the big blind; or the tenant's postflop root with the queue
+          // from the seat after the dealer, as at a street change, cfr_gpu.cu:676-686).
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p) {
+            if (p < N) {
+              sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
+              bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
+              rk[p * WG] = sc.root_rank[(size_t)p * L + t];
+            }
+          }
+          dealer = sc.root_dealer[t];
+          if (root_street == 0) {
+            const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
+            const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
+            const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
+            const int sb_amt = small_blind < s_sb ? small_blind : s_sb;
+            sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
+            if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
+            const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
+            const int bb_amt = rules.bb < s_bb ? rules.bb : s_bb;
+            sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 15);
+            if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
+            pot = sb_amt + bb_amt;
+            current_bet = bb_amt;
+            qmask = allmask & ~all_in;
+            qpos = wrap(bb_pos + 1, N);
+          } else {
+            folded = root_folded;
+            all_in = root_allin;
+            pot = root_pot;
+            street = root_street;
+            ab = root_ab;
+            qmask = allmask & ~(folded | all_in);
+            qpos = wrap(dealer + 1, N);
+          }
+          rng.init(seed_lo, seed_hi, tid0 + (t - lane0), tag);
+          rng.k = walk_k0;  // the deal (and the preflop dealer draw) came first
+          status = 0;
+        }
       }
-      rng.init(seed_lo, seed_hi, tid0 + (t - lane0), tag);
-      rng.k = walk_k0;  // the deal (and the preflop dealer draw) came first
-      status = 0;
+      continue;
     }
-
-    while (__any(status == 0)) {
-      if (status != 0) continue;
-      // one recursion level of traverse_gpu (cfr_gpu.cu:644-860)
+    if (status != 0) continue;
+    // one recursion level of traverse_gpu (cfr_gpu.cu:644-860)
+    if (depth >= kMaxDepth) {
+      status = 3;
+      continue;
+    }
+    const uint32_t alive = allmask & ~folded;
+    if (__popc(alive) == 1) {
+      const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
+      u = ((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
+      status = 1;
+      continue;
+    }
+    if (qmask == 0) {
+      // street change (cfr_gpu.cu:668-692), then the next level's decision in
+      // the same iteration; with nobody left to act, straight to the showdown
+      if (street == 3) {
+        status = 2;
+        continue;
+      }
+      ++street;
+      current_bet = 0;
+      lfr = rules.bb;
+      raises = 0;
+      qmask = allmask & ~(folded | all_in);
+      qpos = wrap(dealer + 1, N);
+      if (qmask == 0) {
+        status = 2;
+        continue;
+      }
+      ++depth;
       if (depth >= kMaxDepth) {
         status = 3;
         continue;
       }
-      const uint32_t alive = allmask & ~folded;
-      if (__popc(alive) == 1) {
-        const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
-        u = ((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
-        status = 1;
-        continue;
-      }
-      if (qmask == 0) {
-        if (street < 3) {  // street change (cfr_gpu.cu:668-692)
-          ++street;
-          current_bet = 0;
-          lfr = rules.bb;
-          raises = 0;
-          qmask = allmask & ~(folded | all_in);
-          qpos = wrap(dealer + 1, N);
-          if (qmask) {
-            ++depth;
-            continue;
-          }
-        }
-        status = 2;  // showdown (nobody left to act: straight there, cfr_gpu.cu:688-694)
-        continue;
-      }
+    }
 
-      // decision node: the first queued seat at or after qpos
-      const uint32_t rot = ((qmask >> qpos) | (qmask << (N - qpos))) & allmask;
-      const int player = wrap(qpos + (int)__builtin_ctz(rot), N);
-      const uint32_t sbv = sb[player * WG];
-      const int stk = (int)(sbv & kStackMask);
-      const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
-      const int to_call = current_bet - bet;
-      const uint32_t vm = rules_mask(rules, m_kind, m_num, m_den, pot, stk, to_call, current_bet, lfr, raises);
-      uint32_t Bp = bk[player * WG];
-      if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
-        const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
-        Bp = (Bp & 0xFF00FFFFu) | (tb << 16);
-        bk[player * WG] = Bp;
-      }
-      const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
-      const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
-      float sg[ROW];
-      load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
-      // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
-      float total = 0.0f;
+    // decision node: the first queued seat at or after qpos
+    const uint32_t rot = ((qmask >> qpos) | (qmask << (N - qpos))) & allmask;
+    const int player = wrap(qpos + (int)__builtin_ctz(rot), N);
+    const uint32_t sbv = sb[player * WG];
+    const int stk = (int)(sbv & kStackMask);
+    const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+    const int to_call = current_bet - bet;
+    const uint32_t vm = rules_mask(rules, m_kind, m_num, m_den, pot, stk, to_call, current_bet, lfr, raises);
+    uint32_t Bp = bk[player * WG];
+    if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
+      const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
+      Bp = (Bp & 0xFF00FFFFu) | (tb << 16);
+      bk[player * WG] = Bp;
+    }
+    const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
+    const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
+    float sg[ROW];
+    load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
+    // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
+    float total = 0.0f;
 #pragma unroll
-      for (int a = 0; a < ROW; ++a)
-        if (a < A && (vm >> a & 1u)) total += sg[a];
-      int act = -1;
-      if (total >= 1e-7f) {
-        const float r = u01(rng.next()) * total;
-        float cum = 0.0f;
+    for (int a = 0; a < ROW; ++a)
+      if (a < A && (vm >> a & 1u)) total += sg[a];
+    int act = -1;
+    if (total >= 1e-7f) {
+      const float r = u01(rng.next()) * total;
+      float cum = 0.0f;
 #pragma unroll
-        for (int a = 0; a < ROW; ++a) {
-          const bool valid = a < A && (vm >> a & 1u);
-          if (valid) cum += sg[a];
-          if (valid && act < 0 && r <= cum) act = a;
+      for (int a = 0; a < ROW; ++a) {
+        const bool valid = a < A && (vm >> a & 1u);
+        if (valid) cum += sg[a];
+        if (valid && act < 0 && r <= cum) act = a;
+      }
+    }
+    if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508, 520-521)
+      for (int i = 0; i < n_passive; ++i) {
+        const int a = (int)((passive >> (4 * i)) & 15u);
+        if (vm >> a & 1u) {
+          act = a;
+          break;
         }
       }
-      if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508, 520-521)
-        for (int i = 0; i < n_passive; ++i) {
-          const int a = (int)((passive >> (4 * i)) & 15u);
-          if (vm >> a & 1u) {
-            act = a;
-            break;
-          }
-        }
-        if (act < 0) act = 0;
-      }
-      if (TRACE) {
-        phash = fnv_path(phash, slot, (uint32_t)act);
-        ++decisions;
-      }
-      if (player == up) {
-        const uint32_t av = (uint32_t)act | (vm << 8);
-        const uint32_t key = seg_key | slot;
-        if (npend < kRegPending) {
-          ps[npend * WG] = key;
-          pa[npend * WG] = av;
-        } else {
-          sc.node_slot[(size_t)npend * L + t] = key;
-          sc.node_av[(size_t)npend * L + t] = av;
-        }
-        ++npend;
-        if (TRACE) upd_legal += __popc(vm);
-      }
-      // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
-      qmask &= ~(1u << player);
-      qpos = wrap(player + 1, N);
-      const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
-      ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
-      if (rules.fold_mask >> act & 1u) {
-        folded |= 1u << player;
+      if (act < 0) act = 0;
+    }
+    if (TRACE) {
+      phash = fnv_path(phash, slot, (uint32_t)act);
+      ++decisions;
+    }
+    if (player == up) {
+      const uint32_t av = (uint32_t)act | (vm << 8);
+      const uint32_t key = seg_key | slot;
+      if (npend < kRegPending) {
+        ps[npend * WG] = key;
+        pa[npend * WG] = av;
       } else {
-        const int ns = stk - chips, nb = bet + chips;
-        sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 15) | ((uint32_t)street << 30);
-        pot += chips;
-        if (ns == 0) all_in |= 1u << player;
-        if (nb > current_bet) {
-          const int raise = nb - current_bet;
-          if (raise >= lfr || current_bet == 0) lfr = raise;
-          current_bet = nb;
-          ++raises;
-          qmask = allmask & ~(folded | all_in) & ~(1u << player);
-        }
+        sc.node_slot[(size_t)npend * L + t] = key;
+        sc.node_av[(size_t)npend * L + t] = av;
       }
-      ++depth;
+      ++npend;
+      if (TRACE) upd_legal += __popc(vm);
     }
-
-    // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending
-    // order, for every lane of the wave that reached one
-    if (__any(status == 2) && status == 2) {
-      int inv[kMaxPlayers];
-      uint32_t rkv[kMaxPlayers];
-#pragma unroll
-      for (int p = 0; p < kMaxPlayers; ++p) {
-        inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
-        rkv[p] = p < N ? (uint32_t)rk[p * WG] : 0u;
-      }
-      const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
-      const uint32_t rk_up = rk[up * WG];
-      const bool up_in = !(folded >> up & 1u);
-      float pay = -(float)inv_up;
-      int prev = 0;
-#pragma unroll
-      for (int it = 0; it < kMaxPlayers; ++it) {
-        int lvl = 0x7FFFFFFF;
-#pragma unroll
-        for (int p = 0; p < kMaxPlayers; ++p)
-          if (inv[p] > prev && inv[p] < lvl) lvl = inv[p];
-        if (lvl != 0x7FFFFFFF) {
-          int layer_pot = 0;
-          uint32_t best = 0;
-#pragma unroll
-          for (int p = 0; p < kMaxPlayers; ++p) {
-            const int c = (inv[p] < lvl ? inv[p] : lvl) - prev;
-            layer_pot += c > 0 ? c : 0;
-            if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] > best) best = rkv[p];
-          }
-          if (best > 0) {
-            int n_win = 0;
-#pragma unroll
-            for (int p = 0; p < kMaxPlayers; ++p)
-              if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] == best) ++n_win;
-            if (up_in && inv_up >= lvl && rk_up == best) pay += (float)layer_pot / (float)n_win;
-          }
-          prev = lvl;
-        }
-      }
-      u = pay;
-    }
-
-    if (active) {
-      if (status == 3) u = 0.0f;  // depth guard (cfr_gpu.cu:661)
-      if (TRACE) {
-        mccfr_trace r;
-        r.utility = u;
-        r.path_hash = phash;
-        r.decisions = (uint16_t)decisions;
-        r.updates = (uint8_t)npend;
-        r.terminal = (uint8_t)status;
-        r.update_legal = (uint16_t)upd_legal;
-        r.reserved = 0;
-        trace[t] = r;
-      }
-      sc.node_count[t] = (uint8_t)npend;
-      sc.node_u[t] = u;
-      const int kr = npend < kRegPending ? npend : kRegPending;
-      for (int k = 0; k < kr; ++k) {
-        sc.node_slot[(size_t)k * L + t] = ps[k * WG];
-        sc.node_av[(size_t)k * L + t] = pa[k * WG];
+    // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
+    qmask &= ~(1u << player);
+    qpos = wrap(player + 1, N);
+    const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
+    ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
+    if (rules.fold_mask >> act & 1u) {
+      folded |= 1u << player;
+    } else {
+      const int ns = stk - chips, nb = bet + chips;
+      sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 15) | ((uint32_t)street << 30);
+      pot += chips;
+      if (ns == 0) all_in |= 1u << player;
+      if (nb > current_bet) {
+        const int raise = nb - current_bet;
+        if (raise >= lfr || current_bet == 0) lfr = raise;
+        current_bet = nb;
+        ++raises;
+        qmask = allmask & ~(folded | all_in) & ~(1u << player);
       }
     }
+    ++depth;
   }
 }
 
@@ -1085,12 +1128,30 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc
 template <int ROW, int WG>
 void walk_launch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DevScratch& sc, uint32_t tpl, mccfr_trace* trace) {
-  const uint32_t waves = lc.lanes / (64u * tpl);
-  const dim3 grid((waves * 64u + WG - 1) / WG);
+  uint32_t jstride = 64u, grid;
+  if (tpl == 0 && lc.n_segs == 1) {
+    // persistent: as many workgroups as the device keeps resident
+    static const uint32_t resident = [] {
+      int per_cu = 0, cus = 0, dev = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk<ROW, WG, false>, WG, 0);
+      return (uint32_t)(per_cu > 0 && cus > 0 ? per_cu * cus : 1);
+    }();
+    grid = std::min<uint32_t>(resident, (lc.lanes + WG - 1) / WG);
+    jstride = grid * WG;
+    tpl = (lc.lanes + jstride - 1) / jstride;
+  } else {
+    if (tpl == 0) tpl = 1;
+    const uint32_t waves = lc.lanes / (64u * tpl);
+    grid = (waves * 64u + WG - 1) / WG;
+  }
   if (trace)
-    hipLaunchKernelGGL((k_walk<ROW, WG, true>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
+    hipLaunchKernelGGL((k_walk<ROW, WG, true>), dim3(grid), dim3(WG), 0, s, lc, segs, tenants, sc, tpl, jstride,
+                       trace);
   else
-    hipLaunchKernelGGL((k_walk<ROW, WG, false>), grid, dim3(WG), 0, s, lc, segs, tenants, sc, tpl, trace);
+    hipLaunchKernelGGL((k_walk<ROW, WG, false>), dim3(grid), dim3(WG), 0, s, lc, segs, tenants, sc, tpl, jstride,
+                       trace);
 }
 
 template <int ROW>
