@@ -54,7 +54,7 @@ struct DevScratch {
 };
 constexpr uint32_t kEmitLanes = 1024;  // lanes per emission block
 constexpr uint32_t kShortRun = 16;     // longer runs are split into pieces reduced by whole workgroups
-constexpr uint32_t kPiece = 1024;      // records per piece of a long run
+constexpr uint32_t kPiece = 256;       // records per piece of a long run (one wave each)
 
 // Tables of the arithmetic 7-card evaluator (built on the host).
 constexpr int kNonFlushClasses = 49205;  // rank multisets of 7 cards, <= 4 per rank

@@ -23,7 +23,7 @@
 //                 (cfr_gpu.cu:784-837) in FP64 in record order, then applies
 //                 them: regrets += sum (CFR+ clamp), strategy sums += sum, BF16
 //                 snapshot re-matched.  Runs longer than kShortRun are cut
-//                 into kPiece-record pieces (k_reduce_piece, one workgroup
+//                 into kPiece-record pieces (k_reduce_piece, one wave
 //                 each) whose sums k_reduce_runs adds in piece order.
 // No atomics touch the tables, and every sum has a fixed order, so a tenant's
 // result does not depend on which other tenants share its launches.
@@ -1051,18 +1051,20 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
   }
 }
 
-// One workgroup per piece of a long run: its groups take records in a fixed
-// stride, their sums are added in group order, and the piece's FP64 sums
-// stored.
+// One wave per piece of a long run (kPiece records): its 64 / ROW groups take
+// the records in a fixed stride, and lane a of the first group adds the
+// groups' sums in group order (shuffles); the piece's FP64 sums are stored.
+// A piece is defined from its run's first record, so the sums depend only on
+// the run's records, not on where the run sits in the launch.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ vals) {
-  constexpr int kGroups = 256 / ROW;
-  __shared__ double s_dr[kGroups][ROW], s_ds[kGroups][ROW];
+  constexpr int kGroups = 64 / ROW;
   const uint32_t n_pieces = sc.cnt->n_pieces;
-  const int tx = threadIdx.x;
-  const int a = tx & (ROW - 1), g = tx / ROW;
-  for (uint32_t p = blockIdx.x; p < n_pieces; p += gridDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int a = lane & (ROW - 1), g = lane / ROW;
+  const uint32_t wstride = gridDim.x * (blockDim.x / 64);
+  for (uint32_t p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); p < n_pieces; p += wstride) {
     const uint4 run = sc.long_runs[sc.piece_run[p]];
     const uint32_t start = run.x + (p - run.z) * kPiece;
     const uint32_t end = min(run.x + run.y, start + kPiece);
@@ -1084,44 +1086,50 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
       for (int b = 0; b < 4; ++b)
         if (idx + (uint32_t)(b * kGroups) < end) acc.add(rr, r_a, a, c.num_actions, c.tw, v[b].x, __uint_as_float(v[b].y));
     }
-    s_dr[g][a] = acc.dr;
-    s_ds[g][a] = acc.ds;
-    __syncthreads();
-    if (tx < ROW) {
-      double dr = 0.0, ds = 0.0;
-      for (int k = 0; k < kGroups; ++k) {
-        dr += s_dr[k][tx];
-        ds += s_ds[k][tx];
-      }
-      sc.partial[(size_t)p * 32 + tx] = dr;
-      sc.partial[(size_t)p * 32 + ROW + tx] = ds;
+    double dr = 0.0, ds = 0.0;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) {
+      dr += __shfl(acc.dr, k * ROW + a, 64);
+      ds += __shfl(acc.ds, k * ROW + a, 64);
     }
-    __syncthreads();
+    if (g == 0) {
+      sc.partial[(size_t)p * 32 + a] = dr;
+      sc.partial[(size_t)p * 32 + ROW + a] = ds;
+    }
   }
 }
 
-// One ROW-lane group per long run: its pieces' sums in piece order, then the
-// update.
+// One wave per long run: its 64 / ROW groups take the run's pieces in a fixed
+// stride, the groups' sums are added in group order (shuffles), and the first
+// group applies the update.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys) {
+  constexpr int kGroups = 64 / ROW;
   const uint32_t n_long = sc.cnt->n_long;
   const int lane = threadIdx.x & 63;
-  const int a = lane & (ROW - 1);
-  const int g_lane0 = lane & ~(ROW - 1);
-  const uint32_t gstride = gridDim.x * blockDim.x / ROW;
-  for (uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) / ROW; r < n_long; r += gstride) {
+  const int a = lane & (ROW - 1), g = lane / ROW;
+  const uint32_t wstride = gridDim.x * (blockDim.x / 64);
+  for (uint32_t r = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); r < n_long; r += wstride) {
     const uint4 run = sc.long_runs[r];
     uint32_t slot;
     const SegCtx c = ctx_of_key(sc.seg_ctx, keys[run.x], lc.slot_bits, &slot);
     const size_t row_off = (size_t)slot * ROW;
-    const float r_a = c.R[row_off + a];
-    const float s_a = c.S[row_off + a];
     double dr = 0.0, ds = 0.0;
-    for (uint32_t j = 0; j < run.w; ++j) {
+    for (uint32_t j = (uint32_t)g; j < run.w; j += kGroups) {
       dr += sc.partial[(size_t)(run.z + j) * 32 + a];
       ds += sc.partial[(size_t)(run.z + j) * 32 + ROW + a];
     }
-    apply_lane<ROW>(c, row_off, a, g_lane0, r_a, s_a, dr, ds);
+    double tr = 0.0, ts = 0.0;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) {
+      tr += __shfl(dr, k * ROW + a, 64);
+      ts += __shfl(ds, k * ROW + a, 64);
+    }
+    if (g == 0) {
+      const float r_a = c.R[row_off + a];
+      const float s_a = c.S[row_off + a];
+      apply_lane<ROW>(c, row_off, a, 0, r_a, s_a, tr, ts);
+    }
   }
 }
 
@@ -1210,7 +1218,7 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
 #define MCCFR_REDUCE(R)                                                                                \
   hipLaunchKernelGGL(k_reduce_rows<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);               \
   hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);              \
-  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(256), dim3(256), 0, s, lc, sc, keys)
+  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(2048), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
     MCCFR_REDUCE(8);
   } else {
