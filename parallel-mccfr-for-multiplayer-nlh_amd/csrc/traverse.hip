@@ -94,10 +94,7 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   const uint32_t si = find_seg(segs, lc.n_segs, uniform(t & ~63u));
   const DevSeg& S = segs[si];
   const uint32_t i = t - S.lane0;
-  if (i >= S.n) {  // padding lane of the segment's last wave
-    sc.node_count[t] = 0;
-    return;
-  }
+  if (i >= S.n) return;  // padding lane of the segment's last wave (the walk clears its record count)
   const DevGame& G = tenants[uniform(S.tenant)];
   const int N = G.N;
   const int nb = G.n_board;
@@ -283,13 +280,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   rules.bb = G.bb;
   const int N = G.N;
   const int A = G.num_actions;
-  const int small_blind = G.sb;
   const int hist_bits = G.hist_bits;
-  const int root_street = G.root_street;
-  const uint32_t root_folded = G.root_folded, root_allin = G.root_allin, root_ab = G.root_ab;
-  const int root_pot = G.root_pot;
-  const uint32_t walk_k0 = G.walk_k0;
-  const uint32_t seed_lo = G.seed_lo, seed_hi = G.seed_hi, tag = G.tag;
   const int n_passive = G.n_passive;
   uint64_t passive = 0;  // fallback preference list, 4 bits per entry
   for (int i = 0; i < n_passive; ++i) passive |= (uint64_t)(G.passive_order[i] & 15u) << (4 * i);
@@ -299,7 +290,6 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   const uint32_t lane0 = Sg.lane0, seg_n = Sg.n;
   const int up = Sg.update_player;
   const int up_stack0 = G.stack[up];
-  const uint64_t tid0 = Sg.first_tid;
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
 
@@ -316,6 +306,11 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   int status = 5;  // 0 walking, 1 fold, 2 showdown, 3 depth limit, 5 waiting to start, 6 no traversal left
   float u = 0.0f;
   uint32_t t = 0, jn = 0;
+  // this lane's slots that are a segment's padding lanes hold no records
+  for (uint32_t j = 0; j < tpl; ++j) {
+    const uint32_t tt = base + j * jstride + (uint32_t)lane;
+    if (tt < lc.lanes && tt - lane0 >= seg_n) sc.node_count[tt] = 0;
+  }
 
   while (true) {
     const uint32_t t_next = base + jn * jstride + (uint32_t)lane;
@@ -325,6 +320,10 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     const uint64_t ready = __ballot(status != 0 && status != 6 && has_next);
     if (waiting == 0 && walking == 0) break;
     if (waiting != 0 && (walking == 0 || __popcll(ready) >= kRefillLanes)) {
+      // The tenant's root and stack fields are read here, where they are used,
+      // rather than kept in scalar registers through the walk (the compiler
+      // barrier stops their loads from being hoisted out of the loop).
+      asm volatile("" ::: "memory");
       // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending
       // order, for every lane of the wave that reached one
       if (__any(status == 2) && status == 2) {
@@ -409,6 +408,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
           decisions = 0;
           upd_legal = 0;
           u = 0.0f;
+          const int root_street = G.root_street;
           // Root state (cfr_gpu.cu:954-981: stacks, blinds, preflop queue from the
           // seat after the big blind; or the tenant's postflop root with the queue
           // from the seat after the dealer, as at a street change, cfr_gpu.cu:676-686).
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
             const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
             const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
             const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
-            const int sb_amt = small_blind < s_sb ? small_blind : s_sb;
+            const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
             sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
             if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
             const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
@@ -437,16 +437,16 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
             qmask = allmask & ~all_in;
             qpos = wrap(bb_pos + 1, N);
           } else {
-            folded = root_folded;
-            all_in = root_allin;
-            pot = root_pot;
+            folded = G.root_folded;
+            all_in = G.root_allin;
+            pot = G.root_pot;
             street = root_street;
-            ab = root_ab;
+            ab = G.root_ab;
             qmask = allmask & ~(folded | all_in);
             qpos = wrap(dealer + 1, N);
           }
-          rng.init(seed_lo, seed_hi, tid0 + (t - lane0), tag);
-          rng.k = walk_k0;  // the deal (and the preflop dealer draw) came first
+          rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + (t - lane0), G.tag);
+          rng.k = G.walk_k0;  // the deal (and the preflop dealer draw) came first
           status = 0;
         }
       }
