@@ -9,12 +9,27 @@ namespace mccfr {
 
 constexpr int kDeckStride = 52;  // bytes of LDS deck per lane (13 dwords: odd stride, no bank aliasing)
 
+// Pointers read out of descriptors (tenant tables) are generic to the
+// compiler, which then emits flat accesses; these are global buffers, so the
+// accessors below say so (global loads and stores wait on their own counter).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 template <int ROW>
 __device__ __forceinline__ void load_bf16_row(const uint16_t* p, float (&out)[ROW]) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+  const auto q = gbl(reinterpret_cast<const u32x4*>(p));
 #pragma unroll
   for (int v = 0; v < ROW / 8; ++v) {
-    const uint4 w = q[v];
+    const u32x4 w = q[v];
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -26,10 +41,10 @@ __device__ __forceinline__ void load_bf16_row(const uint16_t* p, float (&out)[RO
 
 template <int ROW>
 __device__ __forceinline__ void load_f32_row(const float* p, float (&out)[ROW]) {
-  const float4* q = reinterpret_cast<const float4*>(p);
+  const auto q = gbl(reinterpret_cast<const f32x4*>(p));
 #pragma unroll
   for (int v = 0; v < ROW / 4; ++v) {
-    const float4 w = q[v];
+    const f32x4 w = q[v];
     out[4 * v] = w.x;
     out[4 * v + 1] = w.y;
     out[4 * v + 2] = w.z;

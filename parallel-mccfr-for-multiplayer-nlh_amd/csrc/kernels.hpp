@@ -29,6 +29,8 @@ struct DevScratch {
   uint32_t* root_bkt;    // [kMaxPlayers][lanes] hole | flop << 8 | turn << 16 | river << 24 buckets
                          // (turn = 0xFF until the walk fetches it; real buckets are <= 198)
   uint16_t* root_rank;   // [kMaxPlayers][lanes] 7-card rank (showdown)
+  uint16_t* sd_inv;      // [kMaxPlayers][lanes] chips each seat put in, for traversals ending in a showdown
+  uint8_t* sd_state;     // [lanes] 0x80 | folded seats after a showdown, else 0
   uint32_t* root_tidx;   // [kMaxPlayers][lanes] colex rank of hole + first four board cards:
                          // the turn bucket's index, read only when the seat acts on the turn
   const uint8_t* turn6;  // the turn bucket table (set for each walk launch)

@@ -72,6 +72,8 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   if (e == hipSuccess) e = hipMalloc(&s.root_bkt, (size_t)kMaxPlayers * L * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.root_rank, (size_t)kMaxPlayers * L * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMalloc(&s.root_tidx, (size_t)kMaxPlayers * L * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.sd_inv, (size_t)kMaxPlayers * L * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc(&s.sd_state, L);
   if (e == hipSuccess) e = hipMalloc(&s.root_dealer, L);
   if (e == hipSuccess) e = hipMalloc(&s.blk_cnt, ((L + kEmitLanes - 1) / kEmitLanes) * sizeof(uint32_t));
   for (int k = 0; k < 2; ++k) {
@@ -97,7 +99,7 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
 }
 
 void free_scratch(DevScratch& s) {
-  void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_tidx, s.root_dealer,
+  void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_tidx, s.sd_inv, s.sd_state, s.root_dealer,
                   s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.n_cand,
                   s.runs, s.n_runs,
                   s.seg_ctx, s.long_runs,

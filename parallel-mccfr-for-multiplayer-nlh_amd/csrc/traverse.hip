@@ -47,6 +47,9 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgc
 // device's buckets are at most 198: bucket_of_rank of a decoded rank <= 11547).
 constexpr uint32_t kTurnUnknown = 0xFFu;
 
+// sd_state bit of a traversal that ended in a showdown (the low bits hold the folded seats).
+constexpr uint32_t kShowdownFlag = 0x80u;
+
 // Waiting lanes (with a next traversal) that trigger a refill round of the walk.
 constexpr int kRefillLanes = 48;
 
@@ -157,12 +160,17 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   sc.root_dealer[t] = G.root_street == 0 ? (uint8_t)(rng.next() % (uint32_t)N) : (uint8_t)G.root_dealer;
 }
 
+// A menu entry packed for the walk's LDS copy: kind | num << 8 | den << 16.
+__device__ __forceinline__ uint32_t menu_word(const DevGame& g, int a) {
+  return (uint32_t)g.kind[a] | ((uint32_t)g.num[a] << 8) | ((uint32_t)g.den[a] << 16);
+}
+
 // chips_for over the wave's LDS copy of the menu (divergent action index).
-__device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, const uint8_t* num,
-                                         const uint8_t* den, int a, int pot, int stack, int to_call,
-                                         int current_bet, int lfr) {
+__device__ __forceinline__ int chips_lds(const DevGame& g, const uint32_t* menu, int a, int pot, int stack,
+                                         int to_call, int current_bet, int lfr) {
   if (pot <= 0) pot = 1;
-  const int k = kind[a];
+  const uint32_t m = menu[a];
+  const int k = (int)(m & 0xFFu);
   if (k == MCCFR_ACT_FOLD || k == MCCFR_ACT_CHECK) return 0;
   if (k == MCCFR_ACT_CALL || k == MCCFR_ACT_CHECK_CALL) return to_call < stack ? to_call : stack;
   if (k == MCCFR_ACT_ALL_IN) return stack;
@@ -171,7 +179,7 @@ __device__ __forceinline__ int chips_lds(const DevGame& g, const uint8_t* kind, 
   const int maxrt = current_bet + (head > 0 ? head : 0);
   int target = mrt;
   if (k == MCCFR_ACT_POT_FRAC) {
-    int inc = ((int)num[a] * pot) / (int)den[a];
+    int inc = (int)__umul24((m >> 8) & 0xFFu, (uint32_t)pot) / (int)(m >> 16);
     inc = inc > 1 ? inc : 1;
     target = current_bet + inc > mrt ? current_bet + inc : mrt;
   }
@@ -205,9 +213,8 @@ struct Rules {
   int max_raises, bb;
 };
 
-__device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint8_t* kind, const uint8_t* num,
-                                               const uint8_t* den, int pot, int stack, int to_call, int current_bet,
-                                               int lfr, int raises) {
+__device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint32_t* menu, int pot, int stack, int to_call,
+                                               int current_bet, int lfr, int raises) {
   if (pot <= 0) pot = 1;
   uint32_t mask;
   if (to_call == 0) {
@@ -224,11 +231,15 @@ __device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint8_t* ki
   if (current_bet + head >= mrt) {
     // max(current_bet + max(1, num*pot/den), mrt) <= current_bet + head
     //   <=> num*pot < (head + 1)*den, given mrt <= current_bet + head
+    // (pot < 2^18, head < 2^15, num and den < 2^8: 24-bit multiplies suffice)
     uint32_t rm = g.raise_mask;
     while (rm) {
       const int a = __builtin_ctz(rm);
       rm &= rm - 1;
-      if (kind[a] == MCCFR_ACT_RAISE_MIN || (int)num[a] * pot < (head + 1) * (int)den[a]) mask |= 1u << a;
+      const uint32_t m = menu[a];
+      if ((m & 0xFFu) == MCCFR_ACT_RAISE_MIN ||
+          __umul24((m >> 8) & 0xFFu, (uint32_t)pot) < __umul24((uint32_t)(head + 1), m >> 16))
+        mask |= 1u << a;
     }
   }
   return mask | g.allin_mask;
@@ -241,10 +252,9 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
                                              mccfr_trace* __restrict__ trace) {
   __shared__ uint32_t s_sb[kMaxPlayers * WG];    // stack | bet << 15 | street << 30
   __shared__ uint32_t s_bk[kMaxPlayers * WG];    // buckets: hole | flop << 8 | turn << 16 | river << 24
-  __shared__ uint16_t s_rk[kMaxPlayers * WG];    // 7-card rank at showdown
   __shared__ uint32_t s_ps[kRegPending * WG];    // pending update nodes: key
   __shared__ uint32_t s_pa[kRegPending * WG];    //                       action | legal mask << 8
-  __shared__ uint8_t s_menu[WG / 64][3][MCCFR_MAX_ACTIONS];  // kind, num, den of the wave's tenant
+  __shared__ uint32_t s_menu[WG / 64][MCCFR_MAX_ACTIONS];  // kind | num << 8 | den << 16 of the wave's tenant
 
   const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
   const uint32_t base =
@@ -253,19 +263,14 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   const uint32_t si = find_seg(segs, lc.n_segs, base);
   const DevSeg& Sg = segs[si];
   const DevGame& G = tenants[uniform(Sg.tenant)];
-  uint8_t* m_kind = s_menu[wv][0];
-  uint8_t* m_num = s_menu[wv][1];
-  uint8_t* m_den = s_menu[wv][2];
+  uint32_t* menu = s_menu[wv];
   if (lane < MCCFR_MAX_ACTIONS) {
-    m_kind[lane] = G.kind[lane];
-    m_num[lane] = G.num[lane];
-    m_den[lane] = G.den[lane];
+    menu[lane] = menu_word(G, lane);
   }
   __builtin_amdgcn_wave_barrier();
 
   uint32_t* sb = s_sb + tx;
   uint32_t* bk = s_bk + tx;
-  uint16_t* rk = s_rk + tx;
   uint32_t* ps = s_ps + tx;
   uint32_t* pa = s_pa + tx;
 
@@ -309,7 +314,10 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   // this lane's slots that are a segment's padding lanes hold no records
   for (uint32_t j = 0; j < tpl; ++j) {
     const uint32_t tt = base + j * jstride + (uint32_t)lane;
-    if (tt < lc.lanes && tt - lane0 >= seg_n) sc.node_count[tt] = 0;
+    if (tt < lc.lanes && tt - lane0 >= seg_n) {
+      sc.node_count[tt] = 0;
+      sc.sd_state[tt] = 0;
+    }
   }
 
   while (true) {
@@ -324,47 +332,12 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
       // rather than kept in scalar registers through the walk (the compiler
       // barrier stops their loads from being hoisted out of the loop).
       asm volatile("" ::: "memory");
-      // showdown with side pots (cfr_gpu.cu:694-752), levels in ascending
-      // order, for every lane of the wave that reached one
-      if (__any(status == 2) && status == 2) {
-        int inv[kMaxPlayers];
-        uint32_t rkv[kMaxPlayers];
+      // a showdown's payoff is settled by k_showdown after the walk: keep
+      // every seat's investment and the folded seats
+      if (status == 2) {
 #pragma unroll
-        for (int p = 0; p < kMaxPlayers; ++p) {
-          inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
-          rkv[p] = p < N ? (uint32_t)rk[p * WG] : 0u;
-        }
-        const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
-        const uint32_t rk_up = rk[up * WG];
-        const bool up_in = !(folded >> up & 1u);
-        float pay = -(float)inv_up;
-        int prev = 0;
-#pragma unroll
-        for (int it = 0; it < kMaxPlayers; ++it) {
-          int lvl = 0x7FFFFFFF;
-#pragma unroll
-          for (int p = 0; p < kMaxPlayers; ++p)
-            if (inv[p] > prev && inv[p] < lvl) lvl = inv[p];
-          if (lvl != 0x7FFFFFFF) {
-            int layer_pot = 0;
-            uint32_t best = 0;
-#pragma unroll
-            for (int p = 0; p < kMaxPlayers; ++p) {
-              const int c = (inv[p] < lvl ? inv[p] : lvl) - prev;
-              layer_pot += c > 0 ? c : 0;
-              if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] > best) best = rkv[p];
-            }
-            if (best > 0) {
-              int n_win = 0;
-#pragma unroll
-              for (int p = 0; p < kMaxPlayers; ++p)
-                if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] == best) ++n_win;
-              if (up_in && inv_up >= lvl && rk_up == best) pay += (float)layer_pot / (float)n_win;
-            }
-            prev = lvl;
-          }
-        }
-        u = pay;
+        for (int p = 0; p < kMaxPlayers; ++p)
+          if (p < N) sc.sd_inv[(size_t)p * L + t] = (uint16_t)(G.stack[p] - (int)(sb[p * WG] & kStackMask));
       }
       if (status >= 1 && status <= 3) {
         if (status == 3) u = 0.0f;  // depth guard (cfr_gpu.cu:661)
@@ -380,7 +353,8 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
           trace[t] = r;
         }
         sc.node_count[t] = (uint8_t)npend;
-        sc.node_u[t] = u;
+        sc.sd_state[t] = status == 2 ? (uint8_t)(kShowdownFlag | folded) : (uint8_t)0;
+        if (status != 2) sc.node_u[t] = u;
         const int kr = npend < kRegPending ? npend : kRegPending;
         for (int k = 0; k < kr; ++k) {
           sc.node_slot[(size_t)k * L + t] = ps[k * WG];
@@ -417,7 +391,6 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
             if (p < N) {
               sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
               bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
-              rk[p * WG] = sc.root_rank[(size_t)p * L + t];
             }
           }
           dealer = sc.root_dealer[t];
@@ -496,7 +469,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     const int stk = (int)(sbv & kStackMask);
     const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
     const int to_call = current_bet - bet;
-    const uint32_t vm = rules_mask(rules, m_kind, m_num, m_den, pot, stk, to_call, current_bet, lfr, raises);
+    const uint32_t vm = rules_mask(rules, menu, pot, stk, to_call, current_bet, lfr, raises);
     uint32_t Bp = bk[player * WG];
     if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
       const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
@@ -510,15 +483,15 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
     float total = 0.0f;
 #pragma unroll
-    for (int a = 0; a < ROW; ++a)
-      if (a < A && (vm >> a & 1u)) total += sg[a];
+    for (int a = 0; a < ROW; ++a)  // legal actions are menu indices, all below A
+      if (vm >> a & 1u) total += sg[a];
     int act = -1;
     if (total >= 1e-7f) {
       const float r = u01(rng.next()) * total;
       float cum = 0.0f;
 #pragma unroll
       for (int a = 0; a < ROW; ++a) {
-        const bool valid = a < A && (vm >> a & 1u);
+        const bool valid = vm >> a & 1u;
         if (valid) cum += sg[a];
         if (valid && act < 0 && r <= cum) act = a;
       }
@@ -553,7 +526,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
     qmask &= ~(1u << player);
     qpos = wrap(player + 1, N);
-    const int chips = chips_lds(G, m_kind, m_num, m_den, act, pot, stk, to_call, current_bet, lfr);
+    const int chips = chips_lds(G, menu, act, pot, stk, to_call, current_bet, lfr);
     ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
     if (rules.fold_mask >> act & 1u) {
       folded |= 1u << player;
@@ -572,6 +545,74 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     }
     ++depth;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Showdown payoffs (cfr_gpu.cu:694-752): side pots in ascending levels over
+// every seat's investment, each level shared by the best live ranks that
+// reach it.  One lane per traversal that ended in a showdown, after the walk;
+// the walk leaves the investments (sd_inv) and the folded seats (sd_state).
+// ---------------------------------------------------------------------------
+template <bool TRACE>
+__global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                  const DevGame* __restrict__ tenants, DevScratch sc,
+                                                  mccfr_trace* __restrict__ trace) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= lc.lanes) return;
+  const uint32_t si = find_seg(segs, lc.n_segs, uniform(t & ~63u));
+  const uint32_t st = sc.sd_state[t];
+  if (!(st & kShowdownFlag)) return;
+  const DevSeg& Sg = segs[si];
+  const DevGame& G = tenants[uniform(Sg.tenant)];
+  const int N = G.N;
+  const int up = Sg.update_player;
+  const uint32_t folded = st & ~kShowdownFlag;
+  const size_t L = sc.cap_lanes;
+  int inv[kMaxPlayers];
+  uint32_t rkv[kMaxPlayers];
+#pragma unroll
+  for (int p = 0; p < kMaxPlayers; ++p) {
+    inv[p] = p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
+    rkv[p] = p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+  }
+  int inv_up = inv[0];
+  uint32_t rk_up = rkv[0];
+#pragma unroll
+  for (int p = 1; p < kMaxPlayers; ++p)
+    if (p == up) {
+      inv_up = inv[p];
+      rk_up = rkv[p];
+    }
+  const bool up_in = !(folded >> up & 1u);
+  float pay = -(float)inv_up;
+  int prev = 0;
+#pragma unroll
+  for (int it = 0; it < kMaxPlayers; ++it) {
+    int lvl = 0x7FFFFFFF;
+#pragma unroll
+    for (int p = 0; p < kMaxPlayers; ++p)
+      if (inv[p] > prev && inv[p] < lvl) lvl = inv[p];
+    if (lvl != 0x7FFFFFFF) {
+      int layer_pot = 0;
+      uint32_t best = 0;
+#pragma unroll
+      for (int p = 0; p < kMaxPlayers; ++p) {
+        const int c = (inv[p] < lvl ? inv[p] : lvl) - prev;
+        layer_pot += c > 0 ? c : 0;
+        if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] > best) best = rkv[p];
+      }
+      if (best > 0) {
+        int n_win = 0;
+#pragma unroll
+        for (int p = 0; p < kMaxPlayers; ++p)
+          if (!(folded >> p & 1u) && inv[p] >= lvl && rkv[p] == best) ++n_win;
+        if (up_in && inv_up >= lvl && rk_up == best) pay += (float)layer_pot / (float)n_win;
+      }
+      prev = lvl;
+    }
+  }
+  sc.node_u[t] = pay;
+  if (TRACE) trace[t].utility = pay;
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,9 +1047,9 @@ __device__ __forceinline__ void apply_lane(const SegCtx& c, size_t row_off, int 
 #pragma unroll
   for (int b = 0; b < ROW; ++b) sum += fmaxf(0.0f, __shfl(nr, group_lane0 + b, 64));
   const float x = (sum > 1e-7f) ? fmaxf(0.0f, nr) / sum : 0.0f;
-  c.R[row_off + a] = nr;
-  c.S[row_off + a] = ns;
-  c.sig[row_off + a] = f32_to_bf16(x);
+  gbl(c.R)[row_off + a] = nr;
+  gbl(c.S)[row_off + a] = ns;
+  gbl(c.sig)[row_off + a] = f32_to_bf16(x);
 }
 
 __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, uint32_t key, uint32_t slot_bits,
@@ -1040,8 +1081,8 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
     const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
-    const float r_a = c.R[row_off + a];
-    const float s_a = c.S[row_off + a];
+    const float r_a = gbl(c.R)[row_off + a];
+    const float s_a = gbl(c.S)[row_off + a];
     RowAcc<ROW> acc;
     for (uint32_t k = 0; k < run.y; ++k) {
       const uint2 v = vals[run.x + k];
@@ -1073,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
     const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
-    const float r_a = c.R[row_off + a];
+    const float r_a = gbl(c.R)[row_off + a];
     RowAcc<ROW> acc;
     for (uint32_t idx = start + (uint32_t)g; idx < end; idx += 4 * kGroups) {
       uint2 v[4];  // four records' loads in flight at once
@@ -1126,8 +1167,8 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc
       ts += __shfl(ds, k * ROW + a, 64);
     }
     if (g == 0) {
-      const float r_a = c.R[row_off + a];
-      const float s_a = c.S[row_off + a];
+      const float r_a = gbl(c.R)[row_off + a];
+      const float s_a = gbl(c.S)[row_off + a];
       apply_lane<ROW>(c, row_off, a, 0, r_a, s_a, tr, ts);
     }
   }
@@ -1187,6 +1228,11 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
     walk_dispatch<8>(s, lc, segs, tenants, ws, shape, trace);
   else
     walk_dispatch<16>(s, lc, segs, tenants, ws, shape, trace);
+  const dim3 grid((lc.lanes + 255) / 256);
+  if (trace)
+    hipLaunchKernelGGL(k_showdown<true>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
+  else
+    hipLaunchKernelGGL(k_showdown<false>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
 }
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,

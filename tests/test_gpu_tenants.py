@@ -169,6 +169,47 @@ def test_train_tenants_graph_equals_launch_sequence(hand_table):
         assert ctrs[g] == (launches * n, launches, launches // len(seats) + 1, seats[launches % len(seats)])
 
 
+def test_graph_replays_one_at_a_time_with_eager_work_between(hand_table):
+    """The captured training graph replayed launch by launch, with eager
+    launches of another tenant, downloads and synchronizations between the
+    replays: every replay's tables equal the eager sequence's after as many
+    launches (bit for bit)."""
+    games = mix_games()[:4]
+    counts = [n for _, _, n in games]
+    launches = 6
+    other = mccfr.cfg_3p_20bb(table_log2=12, seed=99)
+    graph_states, seq_states = [], []
+    with mccfr.Context(small_ref(), 16384) as ctx:
+        ctx.set_hand_table(hand_table)
+        ids = [ctx.tenant_create(cfg, root) for cfg, root, _ in games]
+        x = ctx.tenant_create(other)
+        x_tid = 0
+        for L in range(launches):
+            ctx.train_tenants(ids, counts, 1)
+            graph_states.append([ctx.tenant_download(t)[:2] for t in ids])
+            ctx.run_segments([(x, 2048, L % 3, L // 3 + 1, x_tid)])  # eager work between replays
+            x_tid += 2048
+            ctx.tenant_normalized_strategy(x)
+            ctx.sync()
+    with mccfr.Context(small_ref(), 16384) as ctx:
+        ctx.set_hand_table(hand_table)
+        ids = [ctx.tenant_create(cfg, root) for cfg, root, _ in games]
+        tids = [0] * len(games)
+        for L in range(launches):
+            segs = []
+            for g, ((cfg, root, n), t) in enumerate(zip(games, ids)):
+                seats = seats_of(cfg, root)
+                it = L // len(seats) + 1
+                segs.append((t, n, seats[L % len(seats)], it if cfg.use_linear_cfr else 1, tids[g]))
+                tids[g] += n
+            ctx.run_segments(segs)
+            seq_states.append([ctx.tenant_download(t)[:2] for t in ids])
+    for L in range(launches):
+        for g in range(len(games)):
+            assert np.array_equal(graph_states[L][g][0], seq_states[L][g][0]), (L, g)
+            assert np.array_equal(graph_states[L][g][1], seq_states[L][g][1]), (L, g)
+
+
 def test_results_do_not_depend_on_launch_shape(hand_table):
     cfg = mccfr.cfg_6p_100bb(table_log2=14, seed=3)
     out = []
