@@ -54,7 +54,7 @@ struct DevScratch {
   uint32_t cap_lanes;
   uint32_t max_tiles;
 };
-constexpr uint32_t kEmitLanes = 1024;  // lanes per emission block
+constexpr uint32_t kEmitLanes = 256;   // lanes per emission block (one per thread)
 constexpr uint32_t kShortRun = 16;     // longer runs are split into pieces reduced by whole workgroups
 constexpr uint32_t kPiece = 256;       // records per piece of a long run (one wave each)
 

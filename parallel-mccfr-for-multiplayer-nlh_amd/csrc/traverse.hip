@@ -617,7 +617,7 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
 
 // ---------------------------------------------------------------------------
 // Record emission: lane-ordered (key, (action | mask << 8, utility)) pairs.
-// Blocks of kEmitLanes lanes, 4 lanes per thread.
+// Blocks of kEmitLanes lanes, one lane per thread.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -647,8 +647,8 @@ __device__ __forceinline__ uint32_t count4(uint32_t w) {
 
 __global__ __launch_bounds__(256) void k_emit_count(LaunchCfg lc, DevScratch sc) {
   __shared__ uint32_t s_w[4];
-  const uint32_t l0 = blockIdx.x * kEmitLanes + threadIdx.x * 4;
-  const uint32_t c = l0 < lc.lanes ? count4(*reinterpret_cast<const uint32_t*>(sc.node_count + l0)) : 0u;
+  const uint32_t t = blockIdx.x * kEmitLanes + threadIdx.x;
+  const uint32_t c = t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u;
   uint32_t total = 0;
   (void)block_excl_scan_256(c, s_w, &total);
   if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
@@ -694,36 +694,30 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc)
 }
 
 // A block's records are staged in LDS (when they fit) and written out
-// contiguously, instead of each lane storing its own records.
-constexpr uint32_t kEmitStage = 4096;
+// contiguously, instead of each lane storing its own records.  One lane per
+// thread, so every read of the node lists ([node][lane]) is coalesced.
+constexpr uint32_t kEmitStage = 2048;
 __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_key[kEmitStage];
   __shared__ uint2 s_val[kEmitStage];
-  const uint32_t l0 = blockIdx.x * kEmitLanes + threadIdx.x * 4;
-  const uint32_t w4 = l0 < lc.lanes ? *reinterpret_cast<const uint32_t*>(sc.node_count + l0) : 0u;
+  const uint32_t t = blockIdx.x * kEmitLanes + threadIdx.x;
+  const uint32_t c = t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u;
   uint32_t total = 0;
-  uint32_t pos = block_excl_scan_256(count4(w4), s_w, &total);
+  uint32_t pos = block_excl_scan_256(c, s_w, &total);
   const uint32_t gbase = sc.blk_cnt[blockIdx.x];
   const bool staged = total <= kEmitStage;  // block-uniform
   const size_t L = sc.cap_lanes;
-  if (l0 < lc.lanes) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t t = l0 + (uint32_t)q;
-      const uint32_t c = (w4 >> (8 * q)) & 0xFFu;
-      const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
-      for (uint32_t k = 0; k < c; ++k, ++pos) {
-        const uint32_t key = sc.node_slot[(size_t)k * L + t];
-        const uint2 v = make_uint2(sc.node_av[(size_t)k * L + t], ub);
-        if (staged) {
-          s_key[pos] = key;
-          s_val[pos] = v;
-        } else {
-          sc.key[0][gbase + pos] = key;
-          sc.val[0][gbase + pos] = v;
-        }
-      }
+  const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
+  for (uint32_t k = 0; k < c; ++k, ++pos) {
+    const uint32_t key = sc.node_slot[(size_t)k * L + t];
+    const uint2 v = make_uint2(sc.node_av[(size_t)k * L + t], ub);
+    if (staged) {
+      s_key[pos] = key;
+      s_val[pos] = v;
+    } else {
+      sc.key[0][gbase + pos] = key;
+      sc.val[0][gbase + pos] = v;
     }
   }
   if (staged) {
