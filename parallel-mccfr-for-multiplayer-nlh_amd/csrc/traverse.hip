@@ -737,6 +737,20 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t n_tiles_of(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs by workgroup index.
+// Work item i (i = blockIdx.x + k * gridDim.x, gridDim.x a multiple of 8)
+// names tile (i % 8) * per + i / 8, so every XCD holds one contiguous range of
+// tiles: a digit's output segments from neighbouring tiles then meet in that
+// XCD's L2 and leave it as whole lines.  Returns nt for items past the end.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t nt) {
+  const uint32_t per = (nt + kXcds - 1) / kXcds;
+  const uint32_t j = i / kXcds;
+  const uint32_t tile = (i % kXcds) * per + j;
+  return (j < per && tile < nt) ? tile : nt;
+}
+__device__ __forceinline__ uint32_t xcd_items(uint32_t nt) { return (nt + kXcds - 1) / kXcds * kXcds; }
+
 // Lanes of the wave whose digit equals this lane's, among `valid` lanes.
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
   uint64_t peers = __ballot(valid);
@@ -756,7 +770,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
   const uint32_t nt = n_tiles_of(n);
   const int tx = threadIdx.x, lane = tx & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
-  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+  const uint32_t items = xcd_items(nt);
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t tile = xcd_tile(it, nt);
+    if (tile >= nt) continue;  // workgroup-uniform
     for (uint32_t b = tx; b < kSortBins; b += kSortWG) s_h[b] = 0;
     __syncthreads();
 #pragma unroll 4
@@ -831,7 +848,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) digit_base[q] += b0;
   }
-  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+  const uint32_t items = xcd_items(nt);
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t tile = xcd_tile(it, nt);
+    if (tile >= nt) continue;  // workgroup-uniform
 #pragma unroll
     for (int k = 0; k < kWaves; ++k)
 #pragma unroll
@@ -1238,7 +1258,8 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   // at most lanes * kMaxDepth records; the sort kernels loop over the tiles
   const uint64_t max_rec = (uint64_t)lc.lanes * kMaxDepth;
   const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
-  const uint32_t sgrid = std::min<uint32_t>(std::max<uint32_t>(max_rec_tiles, 1u), 2048u);
+  // a multiple of the XCD count (the tile order assumes it)
+  const uint32_t sgrid = std::min<uint32_t>((std::max<uint32_t>(max_rec_tiles, 1u) + 7u) / 8u * 8u, 2048u);
   const uint32_t passes = key_bits == 0 ? 1u : (key_bits + kDigitBits - 1u) / kDigitBits;
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = kDigitBits * p;
