@@ -52,6 +52,44 @@ __device__ __forceinline__ void load_f32_row(const float* p, float (&out)[ROW]) 
   }
 }
 
+// Orders one wave's LDS writes before its later LDS reads by other lanes (a
+// wave's LDS instructions execute in order; this keeps the compiler from
+// moving accesses across it).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int ROW>
+__device__ __forceinline__ void store_f32_row(float* p, const float (&v)[ROW]) {
+  const auto q = gbl(reinterpret_cast<f32x4*>(p));
+#pragma unroll
+  for (int k = 0; k < ROW / 4; ++k) {
+    f32x4 w;
+    w.x = v[4 * k];
+    w.y = v[4 * k + 1];
+    w.z = v[4 * k + 2];
+    w.w = v[4 * k + 3];
+    q[k] = w;
+  }
+}
+
+// A BF16 row from ROW / 2 packed pairs (action 2k in the low half).
+template <int ROW>
+__device__ __forceinline__ void store_bf16_pairs(uint16_t* p, const uint32_t (&pairs)[ROW / 2]) {
+  const auto q = gbl(reinterpret_cast<u32x4*>(p));
+#pragma unroll
+  for (int k = 0; k < ROW / 8; ++k) {
+    u32x4 w;
+    w.x = pairs[4 * k];
+    w.y = pairs[4 * k + 1];
+    w.z = pairs[4 * k + 2];
+    w.w = pairs[4 * k + 3];
+    q[k] = w;
+  }
+}
+
 __device__ __forceinline__ uint32_t fnv_path(uint32_t h, uint32_t slot, uint32_t a) {
   h = (h ^ (slot & 0xFFu)) * 16777619u;
   h = (h ^ ((slot >> 8) & 0xFFu)) * 16777619u;

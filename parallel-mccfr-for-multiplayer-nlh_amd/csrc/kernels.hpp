@@ -48,7 +48,7 @@ struct DevScratch {
   uint32_t* n_cand;      // [max_waves] deferred runs of each region
   SegCtx* seg_ctx;       // [kMaxSegs] table pointers and weights of the launch's segments
   uint4* long_runs;      // [max_long] long runs: first record, records, first piece, pieces
-  uint32_t* piece_run;   // [max_pieces] long run of each piece
+  uint4* piece;          // [max_pieces] pieces of long runs: first record, end, key, long run
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
   LaunchCounters* cnt;
   uint32_t cap_lanes;

@@ -91,7 +91,7 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   if (e == hipSuccess) e = hipMalloc(&s.n_cand, max_waves * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&s.seg_ctx, (size_t)kMaxSegs * sizeof(SegCtx));
   if (e == hipSuccess) e = hipMalloc(&s.long_runs, max_long * sizeof(uint4));
-  if (e == hipSuccess) e = hipMalloc(&s.piece_run, max_pieces * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.piece, max_pieces * sizeof(uint4));
   if (e == hipSuccess) e = hipMalloc(&s.partial, max_pieces * 32 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&s.cnt, sizeof(LaunchCounters));
   if (e == hipSuccess) e = hipMemset(s.cnt, 0, sizeof(LaunchCounters));
@@ -103,7 +103,7 @@ void free_scratch(DevScratch& s) {
                   s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.n_cand,
                   s.runs, s.n_runs,
                   s.seg_ctx, s.long_runs,
-                  s.piece_run, s.partial, s.cnt};
+                  s.piece, s.partial, s.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   std::memset(&s, 0, sizeof s);

@@ -1006,7 +1006,8 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
       const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
       const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
       sc.long_runs[r] = make_uint4(i, len, p0, np);
-      for (uint32_t j = 0; j < np; ++j) sc.piece_run[p0 + j] = r;
+      for (uint32_t j = 0; j < np; ++j)
+        sc.piece[p0 + j] = make_uint4(i + j * kPiece, min(lo, i + (j + 1) * kPiece), key, r);
     }
     if (nc) sc.n_runs[w] = nr;
   }
@@ -1072,74 +1073,144 @@ __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, ui
   return sctx[(uint32_t)((uint64_t)key >> slot_bits)];
 }
 
-// Short runs, one ROW-lane group each; a workgroup takes wave regions in turn
-// and spreads each region's runs over its groups.
+// Short runs, one lane each: a wave takes one wave region's run list (at most
+// 64 runs) and lane j run j.  The lane holds the slot's regret and
+// strategy-sum rows, sums every action's contributions in FP64 in record
+// order (the arithmetic of RowAcc, per action), and stores the updated rows
+// and the re-matched BF16 strategy whole.  A lane per run keeps up to 64 row
+// gathers of a wave in flight; row accesses are latency-bound, not
+// bandwidth-bound.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                      const uint2* __restrict__ vals) {
-  constexpr uint32_t kGroups = 256 / ROW;
   const uint32_t n_waves = (sc.cnt->n_rec + 63) / 64;
-  const int lane = threadIdx.x & 63;
-  const int a = lane & (ROW - 1);
-  const int g_lane0 = lane & ~(ROW - 1);
-  const uint32_t g = threadIdx.x / ROW;
+  const uint32_t lane = threadIdx.x & 63u;
   const bool one_seg = lc.n_segs == 1;
   SegCtx c0{};
   if (one_seg) c0 = sc.seg_ctx[0];
   const uint32_t slot_mask = (uint32_t)((1ull << lc.slot_bits) - 1ull);
-  for (uint32_t w = blockIdx.x; w < n_waves; w += gridDim.x)
-  for (uint32_t j = g, nr = sc.n_runs[w]; j < nr; j += kGroups) {
-    const uint4 run = sc.runs[(size_t)w * 64 + j];
+  constexpr int kWaves = 256 / 64;
+  // a region's runs start among its 64 records and hold at most kShortRun
+  __shared__ uint2 s_v[kWaves][64 + kShortRun];
+  const uint32_t n = sc.cnt->n_rec;
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t wstride = gridDim.x * kWaves;
+  for (uint32_t w = blockIdx.x * kWaves + wv; w < n_waves; w += wstride) {
+    const uint32_t base = w * 64u;
+    s_v[wv][lane] = base + lane < n ? vals[base + lane] : make_uint2(0u, 0u);
+    if (lane < kShortRun) s_v[wv][64 + lane] = base + 64u + lane < n ? vals[base + 64u + lane] : make_uint2(0u, 0u);
+    wave_lds_sync();
+    if (lane >= sc.n_runs[w]) {
+      wave_lds_sync();
+      continue;
+    }
+    const uint4 run = sc.runs[(size_t)w * 64 + lane];
     const uint32_t slot = run.z & slot_mask;
     const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
     const size_t row_off = (size_t)slot * ROW;
-    float rr[ROW];
+    const int A = c.num_actions;
+    float rr[ROW], ss[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
-    const float r_a = gbl(c.R)[row_off + a];
-    const float s_a = gbl(c.S)[row_off + a];
-    RowAcc<ROW> acc;
+    load_f32_row<ROW>(c.S + row_off, ss);
+    double dr[ROW], ds[ROW];
+#pragma unroll
+    for (int b = 0; b < ROW; ++b) dr[b] = ds[b] = 0.0;
+    // sigma and q of every action for the current legal mask (RowAcc::add's
+    // expressions, evaluated once per mask instead of once per record)
+    uint32_t cached_vm = 0;
+    float sig[ROW], qv[ROW];
+#pragma unroll
+    for (int b = 0; b < ROW; ++b) sig[b] = qv[b] = 0.0f;
     for (uint32_t k = 0; k < run.y; ++k) {
-      const uint2 v = vals[run.x + k];
-      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
+      const uint2 v = s_v[wv][run.x - base + k];
+      const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+      const float u = __uint_as_float(v.y);
+      if (vm != cached_vm) {
+        cached_vm = vm;
+        float sum_r = 0.0f;
+#pragma unroll
+        for (int b = 0; b < ROW; ++b) sum_r += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
+        const float uni = 1.0f / (float)__popc(vm);
+        const bool pos = sum_r > 1e-7f;
+#pragma unroll
+        for (int b = 0; b < ROW; ++b) {
+          const bool legal = b < A && (vm >> b & 1u);
+          sig[b] = pos ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
+          qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / sum_r : uni, 0.05f);
+        }
+      }
+      float q = qv[0];
+#pragma unroll
+      for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+      const float rd_sampled = u / q - u;
+      const float rd_other = 0.0f - u;
+#pragma unroll
+      for (int b = 0; b < ROW; ++b) {
+        if (b < A && (vm >> b & 1u)) {
+          dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+          ds[b] += (double)(c.tw * sig[b]);
+        }
+      }
     }
-    apply_lane<ROW>(c, row_off, a, g_lane0, r_a, s_a, acc.dr, acc.ds);
+    // apply_lane's arithmetic for every action of the row
+    float nr[ROW], ns[ROW];
+    float sum = 0.0f;
+#pragma unroll
+    for (int b = 0; b < ROW; ++b) {
+      nr[b] = (float)((double)rr[b] + dr[b]);
+      if (c.cfr_plus) nr[b] = fmaxf(0.0f, nr[b]);
+      ns[b] = (float)((double)ss[b] + ds[b]);
+      sum += fmaxf(0.0f, nr[b]);
+    }
+    uint32_t sg[ROW / 2];
+#pragma unroll
+    for (int b = 0; b < ROW; b += 2) {
+      const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
+      const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
+      sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+    }
+    store_f32_row<ROW>(c.R + row_off, nr);
+    store_f32_row<ROW>(c.S + row_off, ns);
+    store_bf16_pairs<ROW>(c.sig + row_off, sg);
+    wave_lds_sync();  // the next region's staging overwrites s_v
   }
 }
 
-// One wave per piece of a long run (kPiece records): its 64 / ROW groups take
-// the records in a fixed stride, and lane a of the first group adds the
-// groups' sums in group order (shuffles); the piece's FP64 sums are stored.
-// A piece is defined from its run's first record, so the sums depend only on
-// the run's records, not on where the run sits in the launch.
+// One wave per piece of a long run (kPiece records): the piece's records are
+// staged in LDS with coalesced loads, its 64 / ROW groups take them in a fixed
+// stride, and lane a of the first group adds the groups' sums in group order
+// (shuffles); the piece's FP64 sums are stored.  A piece is defined from its
+// run's first record, so the sums depend only on the run's records, not on
+// where the run sits in the launch.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ vals) {
   constexpr int kGroups = 64 / ROW;
+  constexpr int kWaves = 256 / 64;
+  __shared__ uint2 s_v[kWaves][kPiece];
   const uint32_t n_pieces = sc.cnt->n_pieces;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a = lane & (ROW - 1), g = lane / ROW;
-  const uint32_t wstride = gridDim.x * (blockDim.x / 64);
-  for (uint32_t p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); p < n_pieces; p += wstride) {
-    const uint4 run = sc.long_runs[sc.piece_run[p]];
-    const uint32_t start = run.x + (p - run.z) * kPiece;
-    const uint32_t end = min(run.x + run.y, start + kPiece);
+  const uint32_t wstride = gridDim.x * kWaves;
+  for (uint32_t p = blockIdx.x * kWaves + (uint32_t)wv; p < n_pieces; p += wstride) {
+    const uint4 pc = sc.piece[p];
+    const uint32_t len = pc.y - pc.x;
+#pragma unroll
+    for (int b = 0; b < (int)(kPiece / 64); ++b) {
+      const uint32_t k = (uint32_t)(lane + 64 * b);
+      s_v[wv][k] = k < len ? vals[pc.x + k] : make_uint2(0u, 0u);
+    }
     uint32_t slot;
-    const SegCtx c = ctx_of_key(sc.seg_ctx, keys[start], lc.slot_bits, &slot);
+    const SegCtx c = ctx_of_key(sc.seg_ctx, pc.z, lc.slot_bits, &slot);
     const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
     const float r_a = gbl(c.R)[row_off + a];
+    wave_lds_sync();
     RowAcc<ROW> acc;
-    for (uint32_t idx = start + (uint32_t)g; idx < end; idx += 4 * kGroups) {
-      uint2 v[4];  // four records' loads in flight at once
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint32_t k = idx + (uint32_t)(b * kGroups);
-        v[b] = k < end ? vals[k] : make_uint2(0u, 0u);
-      }
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if (idx + (uint32_t)(b * kGroups) < end) acc.add(rr, r_a, a, c.num_actions, c.tw, v[b].x, __uint_as_float(v[b].y));
+    for (uint32_t k = (uint32_t)g; k < len; k += kGroups) {
+      const uint2 v = s_v[wv][k];
+      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
     }
     double dr = 0.0, ds = 0.0;
 #pragma unroll
@@ -1151,6 +1222,7 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
       sc.partial[(size_t)p * 32 + a] = dr;
       sc.partial[(size_t)p * 32 + ROW + a] = ds;
     }
+    wave_lds_sync();  // the next piece's staging overwrites s_v
   }
 }
 
