@@ -1176,18 +1176,23 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
   }
 }
 
-// One wave per piece of a long run (kPiece records): the piece's records are
-// staged in LDS with coalesced loads, its 64 / ROW groups take them in a fixed
-// stride, and lane a of the first group adds the groups' sums in group order
-// (shuffles); the piece's FP64 sums are stored.  A piece is defined from its
-// run's first record, so the sums depend only on the run's records, not on
-// where the run sits in the launch.
+// One wave per piece of a long run (kPiece records), in two phases.  First a
+// lane per record: the record's q and sampled-action regret u / q - u (one
+// pair of divisions per record instead of one per record and action), staged
+// in LDS with the record's action word and utility.  Then its 64 / ROW groups
+// take the records in a fixed stride, lane a summing action a's contributions
+// in FP64 (RowAcc's expressions; sigma cached per legal mask), and lane a of
+// the first group adds the groups' sums in group order (shuffles); the
+// piece's FP64 sums are stored.  A piece is defined from its run's first
+// record, so the sums depend only on the run's records, not on where the run
+// sits in the launch.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ vals) {
   constexpr int kGroups = 64 / ROW;
   constexpr int kWaves = 256 / 64;
-  __shared__ uint2 s_v[kWaves][kPiece];
+  __shared__ uint32_t s_av[kWaves][kPiece];
+  __shared__ float s_rs[kWaves][kPiece], s_u[kWaves][kPiece];
   const uint32_t n_pieces = sc.cnt->n_pieces;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a = lane & (ROW - 1), g = lane / ROW;
@@ -1195,34 +1200,80 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
   for (uint32_t p = blockIdx.x * kWaves + (uint32_t)wv; p < n_pieces; p += wstride) {
     const uint4 pc = sc.piece[p];
     const uint32_t len = pc.y - pc.x;
+    uint2 v[kPiece / 64];
 #pragma unroll
     for (int b = 0; b < (int)(kPiece / 64); ++b) {
       const uint32_t k = (uint32_t)(lane + 64 * b);
-      s_v[wv][k] = k < len ? vals[pc.x + k] : make_uint2(0u, 0u);
+      v[b] = k < len ? vals[pc.x + k] : make_uint2(0u, 0u);
     }
     uint32_t slot;
     const SegCtx c = ctx_of_key(sc.seg_ctx, pc.z, lc.slot_bits, &slot);
+    const int A = c.num_actions;
     const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
     load_f32_row<ROW>(c.R + row_off, rr);
-    const float r_a = gbl(c.R)[row_off + a];
-    wave_lds_sync();
-    RowAcc<ROW> acc;
-    for (uint32_t k = (uint32_t)g; k < len; k += kGroups) {
-      const uint2 v = s_v[wv][k];
-      acc.add(rr, r_a, a, c.num_actions, c.tw, v.x, __uint_as_float(v.y));
+    // phase 1: one lane per record
+    uint32_t m_vm = 0;
+    float m_sum = 0.0f, m_uni = 0.0f;
+#pragma unroll
+    for (int b = 0; b < (int)(kPiece / 64); ++b) {
+      const uint32_t act = v[b].x & 0xFFu, vm = v[b].x >> 8;
+      const float u = __uint_as_float(v[b].y);
+      if (vm != m_vm) {
+        m_vm = vm;
+        float t = 0.0f;
+#pragma unroll
+        for (int q = 0; q < ROW; ++q) t += (q < A && (vm >> q & 1u)) ? fmaxf(0.0f, rr[q]) : 0.0f;
+        m_sum = t;
+        m_uni = 1.0f / (float)__popc(vm);
+      }
+      float r_act = rr[0];
+#pragma unroll
+      for (int q = 1; q < ROW; ++q) r_act = ((uint32_t)q == act) ? rr[q] : r_act;
+      const float qa = fmaxf(m_sum > 1e-7f ? fmaxf(0.0f, r_act) / m_sum : m_uni, 0.05f);
+      const uint32_t k = (uint32_t)(lane + 64 * b);
+      s_av[wv][k] = v[b].x;
+      s_rs[wv][k] = u / qa - u;
+      s_u[wv][k] = u;
     }
+    wave_lds_sync();
+    // phase 2: lane a of each group sums action a
+    float ra = rr[0];
+#pragma unroll
+    for (int q = 1; q < ROW; ++q) ra = (q == a) ? rr[q] : ra;
     double dr = 0.0, ds = 0.0;
+    uint32_t cached_vm = 0;
+    float sig_a = 0.0f;
+    bool legal = false;
+    for (uint32_t k = (uint32_t)g; k < len; k += kGroups) {
+      const uint32_t av = s_av[wv][k];
+      const uint32_t act = av & 0xFFu, vm = av >> 8;
+      if (vm != cached_vm) {
+        cached_vm = vm;
+        float t = 0.0f;
+#pragma unroll
+        for (int q = 0; q < ROW; ++q) t += (q < A && (vm >> q & 1u)) ? fmaxf(0.0f, rr[q]) : 0.0f;
+        const float uni = 1.0f / (float)__popc(vm);
+        const bool pos = t > 1e-7f;
+        legal = a < A && (vm >> a & 1u);
+        sig_a = pos ? (legal ? fmaxf(0.0f, ra) : 0.0f) / t : (legal ? uni : 0.0f);
+      }
+      if (legal) {
+        dr += (double)((uint32_t)a == act ? s_rs[wv][k] : 0.0f - s_u[wv][k]);
+        ds += (double)(c.tw * sig_a);
+      }
+    }
+    double tr = 0.0, ts = 0.0;
 #pragma unroll
     for (int k = 0; k < kGroups; ++k) {
-      dr += __shfl(acc.dr, k * ROW + a, 64);
-      ds += __shfl(acc.ds, k * ROW + a, 64);
+      tr += __shfl(dr, k * ROW + a, 64);
+      ts += __shfl(ds, k * ROW + a, 64);
     }
     if (g == 0) {
-      sc.partial[(size_t)p * 32 + a] = dr;
-      sc.partial[(size_t)p * 32 + ROW + a] = ds;
+      sc.partial[(size_t)p * 32 + a] = tr;
+      sc.partial[(size_t)p * 32 + ROW + a] = ts;
     }
-    wave_lds_sync();  // the next piece's staging overwrites s_v
+    wave_lds_sync();  // the next piece's staging overwrites the LDS arrays
   }
 }
 
