@@ -76,6 +76,25 @@ __device__ __forceinline__ uint32_t find_seg(const DevSeg* __restrict__ segs, ui
 // 2N hole cards and the 5 - n_board missing board cards are drawn from the
 // remaining deck_size cards, and the dealer is the root's.
 // ---------------------------------------------------------------------------
+// C(n, k) for n < 52, k <= 6 at [k * 52 + n], built at compile time.
+struct BinomTable {
+  uint32_t v[7 * 52];
+};
+constexpr BinomTable make_binom() {
+  BinomTable t{};
+  for (uint32_t k = 0; k < 7; ++k)
+    for (uint32_t n = 0; n < 52; ++n) {
+      uint64_t c = n < k ? 0 : 1;
+      for (uint32_t j = 0; j < k && n >= k; ++j) c = c * (n - j) / (j + 1);  // exact at every step
+      t.v[k * 52 + n] = (uint32_t)c;
+    }
+  return t;
+}
+static_assert(make_binom().v[6 * 52 + 51] == 18009460u && make_binom().v[5 * 52 + 51] == 2349060u &&
+                  make_binom().v[2 * 52 + 1] == 0u && make_binom().v[0] == 1u,
+              "C(51, 6), C(51, 5), C(1, 2), C(0, 0)");
+__constant__ BinomTable kBinom = make_binom();
+
 __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                   const DevGame* __restrict__ tenants, DealInputs in,
                                                   DevScratch sc) {
@@ -85,12 +104,7 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   __shared__ uint8_t s_deck[kDeckStride * kDealWG];
   for (int i = threadIdx.x; i < 2704 / 4; i += kDealWG) s_lut[i] = reinterpret_cast<const uint32_t*>(in.lut)[i];
   for (int i = threadIdx.x; i < 13 * 8 * 5; i += kDealWG) s_off[i] = in.at.offsets[i];
-  for (int i = threadIdx.x; i < 7 * 52; i += kDealWG) {
-    const uint32_t k = (uint32_t)i / 52u, nn = (uint32_t)i - 52u * k;
-    uint64_t c = 1;
-    for (uint32_t j = 0; j < k; ++j) c = c * (nn - j) / (j + 1);  // exact at every step; 0 once nn < k
-    s_binom[i] = nn < k ? 0u : (uint32_t)c;
-  }
+  for (int i = threadIdx.x; i < 7 * 52; i += kDealWG) s_binom[i] = kBinom.v[i];
   __syncthreads();
   const uint32_t t = blockIdx.x * kDealWG + threadIdx.x;
   if (t >= lc.lanes) return;
@@ -106,7 +120,10 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
   rng.init(G.seed_lo, G.seed_hi, S.first_tid + i, G.tag);
   uint8_t* dk = s_deck + threadIdx.x * kDeckStride;
   if (nb == 0) {
-    for (int k = 0; k < 52; ++k) dk[k] = (uint8_t)k;
+    static_assert(kDeckStride % 4 == 0, "dword-aligned decks");
+    uint32_t* dw = reinterpret_cast<uint32_t*>(dk);
+#pragma unroll
+    for (uint32_t q = 0; q < 13; ++q) dw[q] = 0x03020100u + 0x04040404u * q;  // cards 4q .. 4q + 3
   } else {
     const uint64_t bm = ((uint64_t)G.board_hi << 32) | G.board_lo;
     int j = 0;
@@ -1073,106 +1090,125 @@ __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, ui
   return sctx[(uint32_t)((uint64_t)key >> slot_bits)];
 }
 
-// Short runs, one lane each: a wave takes one wave region's run list (at most
-// 64 runs) and lane j run j.  The lane holds the slot's regret and
-// strategy-sum rows, sums every action's contributions in FP64 in record
-// order (the arithmetic of RowAcc, per action), and stores the updated rows
-// and the re-matched BF16 strategy whole.  A lane per run keeps up to 64 row
-// gathers of a wave in flight; row accesses are latency-bound, not
-// bandwidth-bound.
+// Short runs, one lane each.  A wave takes two neighbouring wave regions' run
+// lists (at most 64 runs each, ~48 together) and lane j the pair's j-th run;
+// the records those runs can reach are staged in LDS with coalesced loads.
+// The lane holds the slot's regret and strategy-sum rows, sums every action's
+// contributions in FP64 in record order, and stores the updated rows and the
+// re-matched BF16 strategy whole.  Per record and legal action a:
+//   dR_a += a == sampled ? u / q - u : 0 - u,   q = max(sigma_sampled, 0.05)
+//   dS_a += tw * sigma_a
+// with sigma from the pre-launch regret row and the record's legal mask
+// (cfr_gpu.cu:784-801, :828).  dS_a is the same for every record of a mask,
+// so it is added once per stretch of equal masks as count * (tw * sigma_a):
+// exact in FP64, and equal to the record-by-record sum for a run of one mask
+// (the rare earlier stretches of a run wait in LDS).
+// A lane per run keeps a wave's row gathers in flight together; row accesses
+// are latency-bound, not bandwidth-bound.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                      const uint2* __restrict__ vals) {
-  const uint32_t n_waves = (sc.cnt->n_rec + 63) / 64;
-  const uint32_t lane = threadIdx.x & 63u;
+  constexpr int kWaves = 256 / 64;
+  // runs of a pair of regions start among its 128 records and hold at most kShortRun
+  constexpr uint32_t kSpan = 2 * 64 + kShortRun;
+  __shared__ uint2 s_v[kWaves][kSpan];
+  __shared__ double s_ds[ROW][256];
+  const uint32_t n = sc.cnt->n_rec;
+  const uint32_t n_regions = (n + 63) / 64;
+  const uint32_t n_pairs = (n_regions + 1) / 2;
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const bool one_seg = lc.n_segs == 1;
   SegCtx c0{};
   if (one_seg) c0 = sc.seg_ctx[0];
   const uint32_t slot_mask = (uint32_t)((1ull << lc.slot_bits) - 1ull);
-  constexpr int kWaves = 256 / 64;
-  // a region's runs start among its 64 records and hold at most kShortRun
-  __shared__ uint2 s_v[kWaves][64 + kShortRun];
-  const uint32_t n = sc.cnt->n_rec;
-  const uint32_t wv = threadIdx.x >> 6;
   const uint32_t wstride = gridDim.x * kWaves;
-  for (uint32_t w = blockIdx.x * kWaves + wv; w < n_waves; w += wstride) {
-    const uint32_t base = w * 64u;
-    s_v[wv][lane] = base + lane < n ? vals[base + lane] : make_uint2(0u, 0u);
-    if (lane < kShortRun) s_v[wv][64 + lane] = base + 64u + lane < n ? vals[base + 64u + lane] : make_uint2(0u, 0u);
+  for (uint32_t pw = blockIdx.x * kWaves + wv; pw < n_pairs; pw += wstride) {
+    const uint32_t r0 = 2 * pw, base = r0 * 64u;
+    for (uint32_t k = lane; k < kSpan; k += 64) s_v[wv][k] = base + k < n ? vals[base + k] : make_uint2(0u, 0u);
+    const uint32_t nr0 = sc.n_runs[r0];
+    const uint32_t nr1 = r0 + 1 < n_regions ? sc.n_runs[r0 + 1] : 0u;
     wave_lds_sync();
-    if (lane >= sc.n_runs[w]) {
-      wave_lds_sync();
-      continue;
-    }
-    const uint4 run = sc.runs[(size_t)w * 64 + lane];
-    const uint32_t slot = run.z & slot_mask;
-    const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
-    const size_t row_off = (size_t)slot * ROW;
-    const int A = c.num_actions;
-    float rr[ROW], ss[ROW];
-    load_f32_row<ROW>(c.R + row_off, rr);
-    load_f32_row<ROW>(c.S + row_off, ss);
-    double dr[ROW], ds[ROW];
+    for (uint32_t j = lane; j < nr0 + nr1; j += 64) {
+      const uint4 run = sc.runs[j < nr0 ? (size_t)r0 * 64 + j : (size_t)(r0 + 1) * 64 + (j - nr0)];
+      const uint32_t slot = run.z & slot_mask;
+      const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
+      const size_t row_off = (size_t)slot * ROW;
+      const int A = c.num_actions;
+      float rr[ROW];
+      load_f32_row<ROW>(c.R + row_off, rr);
+      double dr[ROW];
 #pragma unroll
-    for (int b = 0; b < ROW; ++b) dr[b] = ds[b] = 0.0;
-    // sigma and q of every action for the current legal mask (RowAcc::add's
-    // expressions, evaluated once per mask instead of once per record)
-    uint32_t cached_vm = 0;
-    float sig[ROW], qv[ROW];
+      for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
+      // the current mask's sum of positive regrets, uniform share and q per action
+      uint32_t cur_vm = 0, cnt = 0;
+      float sum_r = 0.0f, uni = 0.0f, qv[ROW];
 #pragma unroll
-    for (int b = 0; b < ROW; ++b) sig[b] = qv[b] = 0.0f;
-    for (uint32_t k = 0; k < run.y; ++k) {
-      const uint2 v = s_v[wv][run.x - base + k];
-      const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
-      const float u = __uint_as_float(v.y);
-      if (vm != cached_vm) {
-        cached_vm = vm;
-        float sum_r = 0.0f;
+      for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
+      // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
+      auto stretch_ds = [&](int b) -> double {
+        const bool legal = b < A && (cur_vm >> b & 1u);
+        const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
+        return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
+      };
+      bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
+      for (uint32_t k = 0; k < run.y; ++k) {
+        const uint2 v = s_v[wv][run.x - base + k];
+        const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+        const float u = __uint_as_float(v.y);
+        if (vm != cur_vm) {
+          if (cnt) {
 #pragma unroll
-        for (int b = 0; b < ROW; ++b) sum_r += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
-        const float uni = 1.0f / (float)__popc(vm);
-        const bool pos = sum_r > 1e-7f;
+            for (int b = 0; b < ROW; ++b) s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
+            spilled = true;
+          }
+          cnt = 0;
+          cur_vm = vm;
+          float t = 0.0f;
 #pragma unroll
-        for (int b = 0; b < ROW; ++b) {
-          const bool legal = b < A && (vm >> b & 1u);
-          sig[b] = pos ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
-          qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / sum_r : uni, 0.05f);
+          for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
+          sum_r = t;
+          uni = 1.0f / (float)__popc(vm);
+          const bool pos = t > 1e-7f;
+#pragma unroll
+          for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
         }
-      }
-      float q = qv[0];
+        ++cnt;
+        float q = qv[0];
 #pragma unroll
-      for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-      const float rd_sampled = u / q - u;
-      const float rd_other = 0.0f - u;
+        for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+        const float rd_sampled = u / q - u;
+        const float rd_other = 0.0f - u;
+#pragma unroll
+        for (int b = 0; b < ROW; ++b)
+          if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+      }
+      float ss[ROW];
+      load_f32_row<ROW>(c.S + row_off, ss);
+      double ds[ROW];
+#pragma unroll
+      for (int b = 0; b < ROW; ++b) ds[b] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
+      // R <- R + dR (CFR+ clamp), S <- S + dS, sigma re-matched from the new row
+      float nr[ROW], ns[ROW];
+      float sum = 0.0f;
 #pragma unroll
       for (int b = 0; b < ROW; ++b) {
-        if (b < A && (vm >> b & 1u)) {
-          dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
-          ds[b] += (double)(c.tw * sig[b]);
-        }
+        nr[b] = (float)((double)rr[b] + dr[b]);
+        if (c.cfr_plus) nr[b] = fmaxf(0.0f, nr[b]);
+        ns[b] = (float)((double)ss[b] + ds[b]);
+        sum += fmaxf(0.0f, nr[b]);
       }
-    }
-    // apply_lane's arithmetic for every action of the row
-    float nr[ROW], ns[ROW];
-    float sum = 0.0f;
+      uint32_t sg[ROW / 2];
 #pragma unroll
-    for (int b = 0; b < ROW; ++b) {
-      nr[b] = (float)((double)rr[b] + dr[b]);
-      if (c.cfr_plus) nr[b] = fmaxf(0.0f, nr[b]);
-      ns[b] = (float)((double)ss[b] + ds[b]);
-      sum += fmaxf(0.0f, nr[b]);
+      for (int b = 0; b < ROW; b += 2) {
+        const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
+        const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
+        sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+      }
+      store_f32_row<ROW>(c.R + row_off, nr);
+      store_f32_row<ROW>(c.S + row_off, ns);
+      store_bf16_pairs<ROW>(c.sig + row_off, sg);
     }
-    uint32_t sg[ROW / 2];
-#pragma unroll
-    for (int b = 0; b < ROW; b += 2) {
-      const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
-      const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
-      sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
-    }
-    store_f32_row<ROW>(c.R + row_off, nr);
-    store_f32_row<ROW>(c.S + row_off, ns);
-    store_bf16_pairs<ROW>(c.sig + row_off, sg);
-    wave_lds_sync();  // the next region's staging overwrites s_v
+    wave_lds_sync();  // the next pair's staging overwrites s_v
   }
 }
 
