@@ -161,32 +161,47 @@ __device__ __forceinline__ uint32_t colex6_tb(const uint32_t* tb, uint32_t a, ui
 // the rank multiset (counts c_0..c_12, sum 7, each <= 4) is ranked
 // lexicographically, index = sum_i off[i][k_i][c_i] with k_i the cards left
 // before rank i (prefix counts from one multiply of packed 3-bit fields).
-__device__ __forceinline__ uint32_t arith_rank7(const uint32_t c[7], const uint16_t* __restrict__ nonflush,
-                                                const uint16_t* __restrict__ flush, const uint16_t* off) {
-  uint64_t cnt = 0, cmask = 0;
-  uint32_t scnt = 0;
-#pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    const uint32_t s = (c[k] * 79u) >> 10;  // c / 13 for c < 52
-    const uint32_t r = c[k] - 13u * s;
-    cnt += 1ull << (3 * r);
-    scnt += 1u << (4 * s);
-    cmask |= 1ull << c[k];
-  }
-  const uint32_t fl = (scnt + 0x3333u) & 0x8888u;  // a nibble >= 5
+//
+// The card sums (packed rank counts, packed suit counts, card mask) are exact
+// integer sums, so a set of cards shared by several hands (a board) is summed
+// once (rank7_add) and each hand adds its own cards before rank7_finish.
+struct Rank7Sums {
+  uint64_t cnt = 0, cmask = 0;  // 3-bit count per rank; bit per card
+  uint32_t scnt = 0;            // 4-bit count per suit
+};
+
+__device__ __forceinline__ void rank7_add(Rank7Sums& a, uint32_t card) {
+  const uint32_t s = (card * 79u) >> 10;  // card / 13 for card < 52
+  const uint32_t r = card - 13u * s;
+  a.cnt += 1ull << (3 * r);
+  a.scnt += 1u << (4 * s);
+  a.cmask |= 1ull << card;
+}
+
+__device__ __forceinline__ uint32_t rank7_finish(const Rank7Sums& a, const uint16_t* __restrict__ nonflush,
+                                                 const uint16_t* __restrict__ flush, const uint16_t* off) {
+  const uint32_t fl = (a.scnt + 0x3333u) & 0x8888u;  // a nibble >= 5
   if (fl) {
     const uint32_t s = (uint32_t)__builtin_ctz(fl) >> 2;
-    return flush[(uint32_t)(cmask >> (13 * s)) & 0x1FFFu];
+    return flush[(uint32_t)(a.cmask >> (13 * s)) & 0x1FFFu];
   }
-  const uint64_t pre = cnt * 0x0000001249249249ull;  // field i = c_0 + .. + c_i
+  const uint64_t pre = a.cnt * 0x0000001249249249ull;  // field i = c_0 + .. + c_i
   uint32_t idx = 0;
 #pragma unroll
   for (int r = 0; r < 13; ++r) {
     const uint32_t before = r == 0 ? 0u : (uint32_t)(pre >> (3 * (r - 1))) & 7u;
-    const uint32_t cr = (uint32_t)(cnt >> (3 * r)) & 7u;
+    const uint32_t cr = (uint32_t)(a.cnt >> (3 * r)) & 7u;
     idx += off[(r * 8 + (7 - before)) * 5 + cr];
   }
   return nonflush[idx];
+}
+
+__device__ __forceinline__ uint32_t arith_rank7(const uint32_t c[7], const uint16_t* __restrict__ nonflush,
+                                                const uint16_t* __restrict__ flush, const uint16_t* off) {
+  Rank7Sums a;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) rank7_add(a, c[k]);
+  return rank7_finish(a, nonflush, flush, off);
 }
 
 }  // namespace mccfr

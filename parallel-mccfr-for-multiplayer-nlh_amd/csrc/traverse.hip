@@ -162,13 +162,18 @@ __global__ __launch_bounds__(kDealWG) void k_deal(LaunchCfg lc, const DevSeg* __
       sc.root_tidx[(size_t)pl * L + t] = colex6_tb(s_binom, h0, h1, bc[0], bc[1], bc[2], bc[3]);
     }
   }
+  Rank7Sums board;  // the five board cards, shared by every seat
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rank7_add(board, bc[k]);
 #pragma unroll
   for (int pl = 0; pl < kMaxPlayers; ++pl) {
     if (pl < N) {
       const uint32_t h0 = dk[2 * pl], h1 = dk[2 * pl + 1];
       const uint32_t hb = lut[h0 * 52 + h1];
-      const uint32_t c7[7] = {h0, h1, bc[0], bc[1], bc[2], bc[3], bc[4]};
-      const uint32_t r7 = arith_rank7(c7, in.at.nonflush, in.at.flush, s_off);
+      Rank7Sums hand = board;
+      rank7_add(hand, h0);
+      rank7_add(hand, h1);
+      const uint32_t r7 = rank7_finish(hand, in.at.nonflush, in.at.flush, s_off);
       sc.root_bkt[(size_t)pl * L + t] = hb | (bf[pl] << 8) | (kTurnUnknown << 16) | (bucket_of_rank(r7) << 24);
       sc.root_rank[(size_t)pl * L + t] = (uint16_t)r7;
     }
@@ -564,6 +569,30 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   }
 }
 
+// Exclusive prefix of v over a 256-thread workgroup (every thread calls it);
+// the workgroup total in *total when total is not null.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t x = s_w[k];
+    if (k < w) before += x;
+    tot += x;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return before + incl - v;
+}
+
 // ---------------------------------------------------------------------------
 // Showdown payoffs (cfr_gpu.cu:694-752): side pots in ascending levels over
 // every seat's investment, each level shared by the best live ranks that
@@ -574,7 +603,15 @@ template <bool TRACE>
 __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                   const DevGame* __restrict__ tenants, DevScratch sc,
                                                   mccfr_trace* __restrict__ trace) {
+  static_assert(kEmitLanes == 256, "a showdown workgroup is one emission block");
+  __shared__ uint32_t s_w[4];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  {
+    // the emission block's record count (the walk has set every lane's count)
+    uint32_t total = 0;
+    (void)block_excl_scan_256(t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u, s_w, &total);
+    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
+  }
   if (t >= lc.lanes) return;
   const uint32_t si = find_seg(segs, lc.n_segs, uniform(t & ~63u));
   const uint32_t st = sc.sd_state[t];
@@ -636,45 +673,28 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
 // Record emission: lane-ordered (key, (action | mask << 8, utility)) pairs.
 // Blocks of kEmitLanes lanes, one lane per thread.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += o;
-  }
-  if (lane == 63) s_w[w] = incl;
-  __syncthreads();
-  uint32_t before = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t x = s_w[k];
-    if (k < w) before += x;
-    tot += x;
-  }
-  __syncthreads();
-  if (total) *total = tot;
-  return before + incl - v;
-}
-
 __device__ __forceinline__ uint32_t count4(uint32_t w) {
   return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
 }
 
-__global__ __launch_bounds__(256) void k_emit_count(LaunchCfg lc, DevScratch sc) {
-  __shared__ uint32_t s_w[4];
-  const uint32_t t = blockIdx.x * kEmitLanes + threadIdx.x;
-  const uint32_t c = t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u;
-  uint32_t total = 0;
-  (void)block_excl_scan_256(c, s_w, &total);
-  if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
-}
-
-// Exclusive scan of the block counts in place (one workgroup); resets the
-// launch counters.
-__global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, DevScratch sc) {
+// Exclusive scan of the block counts in place (one workgroup; k_showdown wrote
+// the counts); resets the launch counters and fills the per-segment update
+// context of the table update.
+__global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                    const DevGame* __restrict__ tenants, DevScratch sc) {
   __shared__ uint32_t s_w[16];
+  for (uint32_t i = threadIdx.x; i < lc.n_segs; i += blockDim.x) {
+    const DevSeg& S = segs[i];
+    const DevGame& G = tenants[S.tenant];
+    SegCtx c;
+    c.R = G.R;
+    c.S = G.S;
+    c.sig = G.sig;
+    c.tw = (float)S.t_weight;
+    c.num_actions = (int16_t)G.num_actions;
+    c.cfr_plus = (int16_t)(G.use_cfr_plus != 0);
+    sc.seg_ctx[i] = c;
+  }
   const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
   const uint32_t per = (nblk + 1023) / 1024;
   const uint32_t b0 = min(threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
@@ -933,22 +953,6 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // lane a holding action a's FP64 sums.  sigma comes from the pre-launch
 // regret row, which nothing has modified yet: only the owner writes a slot.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_seg_ctx(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                 const DevGame* __restrict__ tenants, SegCtx* __restrict__ out) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lc.n_segs; i += gridDim.x * blockDim.x) {
-    const DevSeg& S = segs[i];
-    const DevGame& G = tenants[S.tenant];
-    SegCtx c;
-    c.R = G.R;
-    c.S = G.S;
-    c.sig = G.sig;
-    c.tw = (float)S.t_weight;
-    c.num_actions = (int16_t)G.num_actions;
-    c.cfr_plus = (int16_t)(G.use_cfr_plus != 0);
-    out[i] = c;
-  }
-}
-
 // Finds the runs: one wave per 64 sorted records, head flags by ballot.  A
 // run that ends inside its wave with at most kShortRun records is listed in
 // the wave's region for k_reduce_rows; the others (longer, or running past the
@@ -1411,8 +1415,7 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits) {
   const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
-  hipLaunchKernelGGL(k_emit_count, dim3(nblk), dim3(256), 0, s, lc, sc);
-  hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, sc);
+  hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
   hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
   // at most lanes * kMaxDepth records; the sort kernels loop over the tiles
   const uint64_t max_rec = (uint64_t)lc.lanes * kMaxDepth;
@@ -1432,7 +1435,6 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint32_t* keys = sc.key[passes & 1];
   const uint2* vals = sc.val[passes & 1];
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
-  hipLaunchKernelGGL(k_seg_ctx, dim3((lc.n_segs + 255) / 256), dim3(256), 0, s, lc, segs, tenants, sc.seg_ctx);
   hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
   hipLaunchKernelGGL(k_reduce_cand, dim3(rgrid / 64 + 1), dim3(256), 0, s, sc, keys);
 #define MCCFR_REDUCE(R)                                                                                \
