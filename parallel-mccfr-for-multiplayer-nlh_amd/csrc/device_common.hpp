@@ -5,6 +5,7 @@
 // CPU restatement under oracle/.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/mccfr/mccfr.h"
@@ -93,12 +94,15 @@ struct LaunchCfg {
 // Counters the kernels of one launch write (reset by the launch itself).
 struct LaunchCounters {
   uint32_t n_rec;        // update-player records emitted
-  uint32_t n_long;       // runs handed to the long-run reduction
-  uint32_t n_pieces;     // pieces of the long runs
   uint32_t n_cand;       // runs whose end the per-wave pass could not see
+  uint32_t n_long;       // runs handed to the long-run reduction
+  uint32_t n_pieces;     // pieces of the long runs (with n_long: one 64-bit word, bumped together)
   uint32_t n_runs;       // short runs listed for the row-parallel reduction
   uint32_t pad[3];
 };
+static_assert(offsetof(LaunchCounters, n_long) % 8 == 0 &&
+                  offsetof(LaunchCounters, n_pieces) == offsetof(LaunchCounters, n_long) + 4,
+              "n_long and n_pieces form one little-endian 64-bit word");
 
 // What the table update needs of a segment, gathered once per launch.
 struct SegCtx {

@@ -954,10 +954,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // regret row, which nothing has modified yet: only the owner writes a slot.
 // ---------------------------------------------------------------------------
 // Finds the runs: one wave per 64 sorted records, head flags by ballot.  A
-// run that ends inside its wave with at most kShortRun records is listed in
-// the wave's region for k_reduce_rows; the others (longer, or running past the
-// wave: at most 4 per wave) in the wave's region for k_reduce_cand.  Plain
-// stores, no atomics: every region has one writer.
+// run that ends inside its wave or among the next kShortRun records, with at
+// most kShortRun records, is listed in the wave's region for k_reduce_rows;
+// the longer ones (at most 4 per wave) in the wave's region for
+// k_reduce_cand.  Plain stores, no atomics: every region has one writer.
 __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const int lane = threadIdx.x & 63;
@@ -970,9 +970,18 @@ __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __r
     const uint32_t key = valid ? keys[i] : 0u;
     const bool head = valid && (i == 0 || keys[i - 1] != key);
     const uint64_t heads = __ballot(head);
+    // run ends among the next kShortRun records (a head there, or the end of
+    // the records), so a short run crossing into the next region is closed here
+    const uint32_t xi = base + 64u + (uint32_t)lane;
+    const uint32_t xkey = (lane < (int)kShortRun && xi < n) ? keys[xi] : 0u;
+    // the record before extension lane l: lane 63's record for l = 0, else extension lane l - 1
+    const uint32_t xprev = __shfl(lane == 63 ? key : xkey, lane == 0 ? 63 : lane - 1, 64);
+    const uint64_t xends = __ballot(lane < (int)kShortRun && xi <= n && (xi == n || xkey != xprev)) &
+                           ((1ull << kShortRun) - 1ull);
     const uint64_t later = heads & ~((2ull << lane) - 1ull);  // heads of higher lanes
-    const uint32_t e = later ? base + (uint32_t)__builtin_ctzll(later) : min(base + 64u, n);
-    const bool closed = later != 0 || e == n;
+    const uint32_t e = later ? base + (uint32_t)__builtin_ctzll(later)
+                             : (xends ? base + 64u + (uint32_t)__builtin_ctzll(xends) : min(base + 64u, n));
+    const bool closed = later != 0 || xends != 0 || e == n;
     const bool is_short = head && closed && e - i <= kShortRun;
     const bool defer = head && !is_short;
     const uint64_t sm = __ballot(is_short);
@@ -989,14 +998,24 @@ __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __r
 // One thread per wave region's deferred runs: each run's end by a short
 // forward scan, then exponential and binary search (the keys ascend).  Short
 // runs join the region's run list; long ones are cut into kPiece-record
-// pieces.
+// pieces.  A wave reserves the list space of all its long runs and pieces with
+// one 64-bit atomic (the long-run and piece counters share a word).
 __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const uint32_t n_waves = (n + 63) / 64;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_waves; w += gridDim.x * blockDim.x) {
-    const uint32_t nc = sc.n_cand[w];
-    uint32_t nr = sc.n_runs[w];
-    for (uint32_t c = 0; c < nc; ++c) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w0 < n_waves; w0 += stride) {
+    const uint32_t w = w0 + (uint32_t)lane;
+    const uint32_t nc = w < n_waves ? sc.n_cand[w] : 0u;
+    uint32_t nr = nc ? sc.n_runs[w] : 0u;
+    uint32_t lstart[4], llen[4];  // long runs of the region (llen 0: none)
+    uint32_t nl = 0, np = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      llen[c] = 0;
+      lstart[c] = 0;
+      if ((uint32_t)c >= nc) continue;
       const uint32_t i = sc.cand[(size_t)w * 4 + c];
       const uint32_t key = keys[i];
       // the next kShortRun keys, loaded together: e = first index whose key differs
@@ -1023,14 +1042,37 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
         if (keys[mid] == key) lo = mid + 1;
         else hi = mid;
       }
-      const uint32_t len = lo - i, np = (len + kPiece - 1) / kPiece;
-      const uint32_t r = atomicAdd(&sc.cnt->n_long, 1u);
-      const uint32_t p0 = atomicAdd(&sc.cnt->n_pieces, np);
-      sc.long_runs[r] = make_uint4(i, len, p0, np);
-      for (uint32_t j = 0; j < np; ++j)
-        sc.piece[p0 + j] = make_uint4(i + j * kPiece, min(lo, i + (j + 1) * kPiece), key, r);
+      lstart[c] = i;
+      llen[c] = lo - i;
+      ++nl;
+      np += (lo - i + kPiece - 1) / kPiece;
     }
     if (nc) sc.n_runs[w] = nr;
+    // the wave's exclusive prefix of (pieces << 32 | long runs), one atomic
+    const uint64_t mine = ((uint64_t)np << 32) | nl;
+    uint64_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    const uint64_t total = __shfl(incl, 63, 64);
+    uint64_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(reinterpret_cast<unsigned long long*>(&sc.cnt->n_long), total);
+    base = __shfl(base, 0, 64);
+    const uint64_t excl = base + incl - mine;
+    uint32_t r = (uint32_t)excl, p0 = (uint32_t)(excl >> 32);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (llen[c] == 0) continue;
+      const uint32_t i = lstart[c], end = i + llen[c], npc = (llen[c] + kPiece - 1) / kPiece;
+      const uint32_t key = keys[i];
+      sc.long_runs[r] = make_uint4(i, llen[c], p0, npc);
+      for (uint32_t j = 0; j < npc; ++j)
+        sc.piece[p0 + j] = make_uint4(i + j * kPiece, min(end, i + (j + 1) * kPiece), key, r);
+      ++r;
+      p0 += npc;
+    }
   }
 }
 
