@@ -61,7 +61,10 @@ struct DevGame {
   uint8_t upd_seats[kMaxPlayers];  // update-player schedule
   int32_t n_upd;
 
-  // tables (slot-major rows of `row` entries)
+  // tables: slot s's regrets and strategy sums share one interleaved row,
+  // R[s * 2 * row + a] and S[s * 2 * row + a] (S = R + row), so the update
+  // reads and writes both with one access; the BF16 snapshot the walk gathers
+  // is a dense [slot][row] table of its own
   float* R;                   // regrets
   float* S;                   // strategy sums
   uint16_t* sig;              // BF16 regret-matched strategy snapshot

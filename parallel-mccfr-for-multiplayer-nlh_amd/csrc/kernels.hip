@@ -3,6 +3,9 @@
 //   (the launch pipeline -- deal, walk, record sort, table update -- is in traverse.hip)
 //   k_rm_sweep<ROW>   full-table regret matching (after tables are uploaded).
 //   k_normalize<ROW>  average strategy (kernel_normalize_strategy, cfr_gpu.cu:1029-1058).
+//   k_clamp<ROW>, k_plane<ROW>  CFR+ clamp of every regret; one plane of the
+//                     interleaved regret / strategy-sum rows to or from a dense buffer.
+//   (regret and strategy-sum rows are interleaved: slot s at [s * 2 * ROW], S = R + ROW)
 //   k_eval7_walk      7-card rank via the rank-walk table (eval7_gpu, cfr_gpu.cu:126-137).
 //   k_eval7_arith     7-card rank via flush table + perfect hash of the rank multiset.
 #include <hip/hip_runtime.h>
@@ -19,7 +22,7 @@ __global__ __launch_bounds__(256) void k_rm_sweep(const float* __restrict__ R, u
   for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
        slot += (uint64_t)gridDim.x * blockDim.x) {
     float r[ROW];
-    load_f32_row<ROW>(R + slot * ROW, r);
+    load_f32_row<ROW>(R + slot * 2 * ROW, r);
     float sum = 0.0f;
 #pragma unroll
     for (int a = 0; a < ROW; ++a) sum += fmaxf(0.0f, r[a]);
@@ -43,7 +46,7 @@ __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ S, 
   for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
        slot += (uint64_t)gridDim.x * blockDim.x) {
     float s[ROW];
-    load_f32_row<ROW>(S + slot * ROW, s);
+    load_f32_row<ROW>(S + slot * 2 * ROW, s);
     float total = 0.0f;
 #pragma unroll
     for (int a = 0; a < ROW; ++a) total += s[a];
@@ -84,14 +87,32 @@ __global__ void k_advance_segments(DevSeg* segs, const DevGame* __restrict__ ten
 }
 
 // kernel_clamp_regrets (cfr_gpu.cu:1018-1022) over a whole table.
-__global__ __launch_bounds__(256) void k_clamp(float4* __restrict__ R, size_t n4) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-    float4 v = R[i];
-    v.x = fmaxf(0.0f, v.x);
-    v.y = fmaxf(0.0f, v.y);
-    v.z = fmaxf(0.0f, v.z);
-    v.w = fmaxf(0.0f, v.w);
-    R[i] = v;
+template <int ROW>
+__global__ __launch_bounds__(256) void k_clamp(float* __restrict__ R, uint64_t slots) {
+  for (uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < slots;
+       slot += (uint64_t)gridDim.x * blockDim.x) {
+    float r[ROW];
+    load_f32_row<ROW>(R + slot * 2 * ROW, r);
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) r[a] = fmaxf(0.0f, r[a]);
+    store_f32_row<ROW>(R + slot * 2 * ROW, r);
+  }
+}
+
+// One plane (regrets or strategy sums) of slots [slot0, slot0 + n) between the
+// interleaved device rows and a dense [n][ROW] staging buffer.
+template <int ROW, bool TO_DENSE>
+__global__ __launch_bounds__(256) void k_plane(float* __restrict__ plane, float* __restrict__ dense, uint64_t slot0,
+                                               uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    float v[ROW];
+    if (TO_DENSE) {
+      load_f32_row<ROW>(plane + (slot0 + i) * 2 * ROW, v);
+      store_f32_row<ROW>(dense + i * ROW, v);
+    } else {
+      load_f32_row<ROW>(dense + i * ROW, v);
+      store_f32_row<ROW>(plane + (slot0 + i) * 2 * ROW, v);
+    }
   }
 }
 
@@ -202,8 +223,8 @@ __global__ __launch_bounds__(256) void k_randomize(float* __restrict__ R, float*
     for (int a = 0; a < ROW; ++a) r[a] = a < num_actions ? u01(rng.next()) : 0.0f;
 #pragma unroll
     for (int v = 0; v < ROW / 4; ++v) {
-      reinterpret_cast<float4*>(R + slot * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
-      reinterpret_cast<float4*>(S + slot * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+      reinterpret_cast<float4*>(R + slot * 2 * ROW)[v] = make_float4(r[4 * v], r[4 * v + 1], r[4 * v + 2], r[4 * v + 3]);
+      reinterpret_cast<float4*>(S + slot * 2 * ROW)[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 }
@@ -267,8 +288,22 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 void launch_zero(hipStream_t s, void* p, size_t bytes) {  // bytes: a multiple of 16
   hipLaunchKernelGGL(k_zero, dim3(grid_for(bytes / 16, 256, 1u << 16)), dim3(256), 0, s, (uint4*)p, bytes / 16);
 }
-void launch_clamp(hipStream_t s, float* R, size_t elems) {  // elems: a multiple of 4
-  hipLaunchKernelGGL(k_clamp, dim3(grid_for(elems / 4, 256, 1u << 16)), dim3(256), 0, s, (float4*)R, elems / 4);
+void launch_clamp(hipStream_t s, int row, float* R, uint64_t slots) {
+  const unsigned grid = grid_for(slots, 256, 65536);
+  if (row == 8)
+    hipLaunchKernelGGL(k_clamp<8>, dim3(grid), dim3(256), 0, s, R, slots);
+  else
+    hipLaunchKernelGGL(k_clamp<16>, dim3(grid), dim3(256), 0, s, R, slots);
+}
+void launch_plane(hipStream_t s, int row, float* plane, float* dense, uint64_t slot0, uint64_t n, bool to_dense) {
+  const unsigned grid = grid_for(n, 256, 65536);
+  if (row == 8) {
+    if (to_dense) hipLaunchKernelGGL((k_plane<8, true>), dim3(grid), dim3(256), 0, s, plane, dense, slot0, n);
+    else hipLaunchKernelGGL((k_plane<8, false>), dim3(grid), dim3(256), 0, s, plane, dense, slot0, n);
+  } else {
+    if (to_dense) hipLaunchKernelGGL((k_plane<16, true>), dim3(grid), dim3(256), 0, s, plane, dense, slot0, n);
+    else hipLaunchKernelGGL((k_plane<16, false>), dim3(grid), dim3(256), 0, s, plane, dense, slot0, n);
+  }
 }
 void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots) {
   const unsigned grid = grid_for(slots, 256, 65536);

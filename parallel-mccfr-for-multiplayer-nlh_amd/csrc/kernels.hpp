@@ -103,7 +103,9 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
 void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);
 void launch_zero(hipStream_t s, void* p, size_t bytes);
-void launch_clamp(hipStream_t s, float* R, size_t elems);
+void launch_clamp(hipStream_t s, int row, float* R, uint64_t slots);
+// plane: a tenant's regret or strategy-sum pointer (rows at stride 2 * row)
+void launch_plane(hipStream_t s, int row, float* plane, float* dense, uint64_t slot0, uint64_t n, bool to_dense);
 void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, size_t stride,
                        uint16_t* out, size_t n);
 void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size_t stride,

@@ -402,9 +402,9 @@ int tenant_create(mccfr_ctx* c, const mccfr_game_cfg* cfg, const mccfr_root_stat
     HIPX(c, hipMalloc(&t.owned, e * 10));
     mem = t.owned;
   }
-  t.hg.R = static_cast<float*>(mem);
-  t.hg.S = t.hg.R + e;
-  t.hg.sig = reinterpret_cast<uint16_t*>(t.hg.S + e);
+  t.hg.R = static_cast<float*>(mem);  // interleaved rows: [slot][regrets | strategy sums]
+  t.hg.S = t.hg.R + t.row;
+  t.hg.sig = reinterpret_cast<uint16_t*>(t.hg.R + 2 * e);
   launch_zero(c->stream, mem, e * 10);
   launch_set_tenant(c->stream, c->d_tenants + k, t.hg);
   hipError_t err = hipGetLastError();
@@ -449,7 +449,7 @@ int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* hos
   for (uint32_t i = 0; i < n; ++i) {
     Tenant& t = c->tenants[segs[i].tenant];
     if (t.clamp_pending) {  // the reference clamps every slot after every launch (cfr_gpu.cu:1018-1022)
-      launch_clamp(c->stream, t.hg.R, t.elems());
+      launch_clamp(c->stream, t.row, t.hg.R, t.slots);
       t.clamp_pending = false;
     }
     t.traversals += segs[i].n;
@@ -968,6 +968,33 @@ int mccfr_tenant_table_shape(const mccfr_ctx* c, uint32_t id, uint64_t* slots, u
   return MCCFR_OK;
 }
 
+// One plane (regrets or strategy sums) of a tenant between its interleaved
+// device rows and dense host rows [slot][row], through a device staging buffer
+// of at most 64 MB.  to_host: device -> host; else host -> device (the host
+// buffer is only read).
+static int plane_transfer(mccfr_ctx* c, const Tenant& t, float* plane, float* host, bool to_host) {
+  const uint64_t chunk = std::min<uint64_t>(t.slots, (64ull << 20) / ((uint64_t)t.row * sizeof(float)));
+  float* stage = nullptr;
+  HIPX(c, hipMalloc(&stage, chunk * t.row * sizeof(float)));
+  hipError_t e = hipSuccess;
+  for (uint64_t s0 = 0; s0 < t.slots && e == hipSuccess; s0 += chunk) {
+    const uint64_t n = std::min<uint64_t>(chunk, t.slots - s0);
+    const size_t bytes = n * t.row * sizeof(float);
+    if (to_host) {
+      launch_plane(c->stream, t.row, plane, stage, s0, n, true);
+      e = hipMemcpyAsync(host + s0 * t.row, stage, bytes, hipMemcpyDeviceToHost, c->stream);
+    } else {
+      e = hipMemcpyAsync(stage, host + s0 * t.row, bytes, hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) launch_plane(c->stream, t.row, plane, stage, s0, n, false);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the stage is reused by the next chunk
+  }
+  (void)hipFree(stage);
+  if (e != hipSuccess) return fail(c, MCCFR_E_HIP, std::string("table transfer: ") + hipGetErrorString(e));
+  return MCCFR_OK;
+}
+
 int mccfr_download_tables(mccfr_ctx* c, float* R, float* S, uint16_t* sig, size_t elems, int layout) {
   return mccfr_tenant_download(c, 0, R, S, sig, elems, layout);
 }
@@ -981,21 +1008,25 @@ int mccfr_tenant_download(mccfr_ctx* c, uint32_t id, float* R, float* S, uint16_
   int rc = rm_sweep_if_dirty(c, *t);
   if (rc) return rc;
   HIPX(c, hipStreamSynchronize(c->stream));
-  auto get = [&](auto* host, const auto* dev) -> int {
-    using T = std::remove_pointer_t<decltype(host)>;
+  auto get = [&](float* host, float* plane) -> int {
     if (!host) return MCCFR_OK;
-    if (layout == 0) {
-      HIPX(c, hipMemcpy(host, dev, want * sizeof(T), hipMemcpyDeviceToHost));
-    } else {
-      std::vector<T> tmp(want);
-      HIPX(c, hipMemcpy(tmp.data(), dev, want * sizeof(T), hipMemcpyDeviceToHost));
-      to_soa(tmp.data(), host, t->slots, t->row);
-    }
+    if (layout == 0) return plane_transfer(c, *t, plane, host, true);
+    std::vector<float> tmp(want);
+    if ((rc = plane_transfer(c, *t, plane, tmp.data(), true))) return rc;
+    to_soa(tmp.data(), host, t->slots, t->row);
     return MCCFR_OK;
   };
   if ((rc = get(R, t->hg.R))) return rc;
   if ((rc = get(S, t->hg.S))) return rc;
-  if ((rc = get(sig, t->hg.sig))) return rc;
+  if (sig) {
+    if (layout == 0) {
+      HIPX(c, hipMemcpy(sig, t->hg.sig, want * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<uint16_t> tmp(want);
+      HIPX(c, hipMemcpy(tmp.data(), t->hg.sig, want * sizeof(uint16_t), hipMemcpyDeviceToHost));
+      to_soa(tmp.data(), sig, t->slots, t->row);
+    }
+  }
   return MCCFR_OK;
 }
 
@@ -1010,16 +1041,12 @@ int mccfr_tenant_upload(mccfr_ctx* c, uint32_t id, const float* R, const float* 
   if (elems != want) return fail(c, MCCFR_E_ARG, "elems must equal slots * row_actions");
   if (layout != 0 && layout != 1) return fail(c, MCCFR_E_ARG, "layout must be 0 (rows) or 1 (planes)");
   HIPX(c, hipStreamSynchronize(c->stream));
-  auto put = [&](const float* host, float* dev) -> int {
+  auto put = [&](const float* host, float* plane) -> int {
     if (!host) return MCCFR_OK;
-    if (layout == 0) {
-      HIPX(c, hipMemcpy(dev, host, want * sizeof(float), hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> tmp(want);
-      to_aos(host, tmp.data(), t->slots, t->row);
-      HIPX(c, hipMemcpy(dev, tmp.data(), want * sizeof(float), hipMemcpyHostToDevice));
-    }
-    return MCCFR_OK;
+    if (layout == 0) return plane_transfer(c, *t, plane, const_cast<float*>(host), false);
+    std::vector<float> tmp(want);
+    to_aos(host, tmp.data(), t->slots, t->row);
+    return plane_transfer(c, *t, plane, tmp.data(), false);
   };
   int rc;
   if ((rc = put(R, t->hg.R))) return rc;

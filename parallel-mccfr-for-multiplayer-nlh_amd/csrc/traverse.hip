@@ -1116,7 +1116,7 @@ struct RowAcc {
 // Lane a of a ROW-lane group: R <- R + dR (CFR+ clamp), S <- S + dS, and the
 // slot's BF16 strategy re-matched from the new row (sum in action order).
 template <int ROW>
-__device__ __forceinline__ void apply_lane(const SegCtx& c, size_t row_off, int a, int group_lane0, float r_a,
+__device__ __forceinline__ void apply_lane(const SegCtx& c, size_t slot, int a, int group_lane0, float r_a,
                                            float s_a, double dr, double ds) {
   float nr = (float)((double)r_a + dr);
   if (c.cfr_plus) nr = fmaxf(0.0f, nr);
@@ -1125,9 +1125,9 @@ __device__ __forceinline__ void apply_lane(const SegCtx& c, size_t row_off, int 
 #pragma unroll
   for (int b = 0; b < ROW; ++b) sum += fmaxf(0.0f, __shfl(nr, group_lane0 + b, 64));
   const float x = (sum > 1e-7f) ? fmaxf(0.0f, nr) / sum : 0.0f;
-  gbl(c.R)[row_off + a] = nr;
-  gbl(c.S)[row_off + a] = ns;
-  gbl(c.sig)[row_off + a] = f32_to_bf16(x);
+  gbl(c.R)[slot * 2 * ROW + a] = nr;
+  gbl(c.S)[slot * 2 * ROW + a] = ns;
+  gbl(c.sig)[slot * ROW + a] = f32_to_bf16(x);
 }
 
 __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, uint32_t key, uint32_t slot_bits,
@@ -1178,10 +1178,10 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
       const uint4 run = sc.runs[j < nr0 ? (size_t)r0 * 64 + j : (size_t)(r0 + 1) * 64 + (j - nr0)];
       const uint32_t slot = run.z & slot_mask;
       const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
-      const size_t row_off = (size_t)slot * ROW;
+      const size_t rs_off = (size_t)slot * 2 * ROW;  // interleaved regret / strategy-sum row
       const int A = c.num_actions;
       float rr[ROW];
-      load_f32_row<ROW>(c.R + row_off, rr);
+      load_f32_row<ROW>(c.R + rs_off, rr);
       double dr[ROW];
 #pragma unroll
       for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
           if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
       }
       float ss[ROW];
-      load_f32_row<ROW>(c.S + row_off, ss);
+      load_f32_row<ROW>(c.S + rs_off, ss);
       double ds[ROW];
 #pragma unroll
       for (int b = 0; b < ROW; ++b) ds[b] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
@@ -1250,9 +1250,9 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
         const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
         sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
       }
-      store_f32_row<ROW>(c.R + row_off, nr);
-      store_f32_row<ROW>(c.S + row_off, ns);
-      store_bf16_pairs<ROW>(c.sig + row_off, sg);
+      store_f32_row<ROW>(c.R + rs_off, nr);
+      store_f32_row<ROW>(c.S + rs_off, ns);
+      store_bf16_pairs<ROW>(c.sig + (size_t)slot * ROW, sg);
     }
     wave_lds_sync();  // the next pair's staging overwrites s_v
   }
@@ -1291,9 +1291,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
     uint32_t slot;
     const SegCtx c = ctx_of_key(sc.seg_ctx, pc.z, lc.slot_bits, &slot);
     const int A = c.num_actions;
-    const size_t row_off = (size_t)slot * ROW;
     float rr[ROW];
-    load_f32_row<ROW>(c.R + row_off, rr);
+    load_f32_row<ROW>(c.R + (size_t)slot * 2 * ROW, rr);
     // phase 1: one lane per record
     uint32_t m_vm = 0;
     float m_sum = 0.0f, m_uni = 0.0f;
@@ -1373,7 +1372,6 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc
     const uint4 run = sc.long_runs[r];
     uint32_t slot;
     const SegCtx c = ctx_of_key(sc.seg_ctx, keys[run.x], lc.slot_bits, &slot);
-    const size_t row_off = (size_t)slot * ROW;
     double dr = 0.0, ds = 0.0;
     for (uint32_t j = (uint32_t)g; j < run.w; j += kGroups) {
       dr += sc.partial[(size_t)(run.z + j) * 32 + a];
@@ -1386,9 +1384,9 @@ __global__ __launch_bounds__(256) void k_reduce_runs(LaunchCfg lc, DevScratch sc
       ts += __shfl(ds, k * ROW + a, 64);
     }
     if (g == 0) {
-      const float r_a = gbl(c.R)[row_off + a];
-      const float s_a = gbl(c.S)[row_off + a];
-      apply_lane<ROW>(c, row_off, a, 0, r_a, s_a, tr, ts);
+      const float r_a = gbl(c.R)[(size_t)slot * 2 * ROW + a];
+      const float s_a = gbl(c.S)[(size_t)slot * 2 * ROW + a];
+      apply_lane<ROW>(c, slot, a, 0, r_a, s_a, tr, ts);
     }
   }
 }
