@@ -187,9 +187,14 @@ __device__ __forceinline__ uint32_t menu_word(const DevGame& g, int a) {
   return (uint32_t)g.kind[a] | ((uint32_t)g.num[a] << 8) | ((uint32_t)g.den[a] << 16);
 }
 
-// chips_for over the wave's LDS copy of the menu (divergent action index).
-__device__ __forceinline__ int chips_lds(const DevGame& g, const uint32_t* menu, int a, int pot, int stack,
-                                         int to_call, int current_bet, int lfr) {
+// ceil(2^32 / den) for den >= 2 (0 otherwise): num * pot / den is then one
+// high multiply plus one correction (chips_lds).
+__device__ __forceinline__ uint32_t den_reciprocal(uint32_t den) { return den >= 2 ? 0xFFFFFFFFu / den + 1u : 0u; }
+
+// chips_for over the wave's LDS copy of the menu (divergent action index);
+// rcp[a] = den_reciprocal(den of entry a).
+__device__ __forceinline__ int chips_lds(const DevGame& g, const uint32_t* menu, const uint32_t* rcp, int a, int pot,
+                                         int stack, int to_call, int current_bet, int lfr) {
   if (pot <= 0) pot = 1;
   const uint32_t m = menu[a];
   const int k = (int)(m & 0xFFu);
@@ -201,7 +206,12 @@ __device__ __forceinline__ int chips_lds(const DevGame& g, const uint32_t* menu,
   const int maxrt = current_bet + (head > 0 ? head : 0);
   int target = mrt;
   if (k == MCCFR_ACT_POT_FRAC) {
-    int inc = (int)__umul24((m >> 8) & 0xFFu, (uint32_t)pot) / (int)(m >> 16);
+    // x / den for x = num * pot < 2^26 and 1 <= den < 2^8: q = hi32(x * ceil(2^32 / den))
+    // is x / den or one more (x * e / 2^32 < 1 for e = ceil(2^32 / den) - 2^32 / den < 1)
+    const uint32_t x = __umul24((m >> 8) & 0xFFu, (uint32_t)pot), den = m >> 16;
+    uint32_t q = den >= 2 ? __umulhi(x, rcp[a]) : x;
+    if (q * den > x) --q;
+    int inc = (int)q;
     inc = inc > 1 ? inc : 1;
     target = current_bet + inc > mrt ? current_bet + inc : mrt;
   }
@@ -277,6 +287,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   __shared__ uint32_t s_ps[kRegPending * WG];    // pending update nodes: key
   __shared__ uint32_t s_pa[kRegPending * WG];    //                       action | legal mask << 8
   __shared__ uint32_t s_menu[WG / 64][MCCFR_MAX_ACTIONS];  // kind | num << 8 | den << 16 of the wave's tenant
+  __shared__ uint32_t s_rcp[WG / 64][MCCFR_MAX_ACTIONS];   // den_reciprocal of each entry's den
 
   const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
   const uint32_t base =
@@ -286,8 +297,10 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   const DevSeg& Sg = segs[si];
   const DevGame& G = tenants[uniform(Sg.tenant)];
   uint32_t* menu = s_menu[wv];
+  uint32_t* rcp = s_rcp[wv];
   if (lane < MCCFR_MAX_ACTIONS) {
     menu[lane] = menu_word(G, lane);
+    rcp[lane] = G.kind[lane] == MCCFR_ACT_POT_FRAC ? den_reciprocal(G.den[lane]) : 0u;
   }
   __builtin_amdgcn_wave_barrier();
 
@@ -548,7 +561,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
     qmask &= ~(1u << player);
     qpos = wrap(player + 1, N);
-    const int chips = chips_lds(G, menu, act, pot, stk, to_call, current_bet, lfr);
+    const int chips = chips_lds(G, menu, rcp, act, pot, stk, to_call, current_bet, lfr);
     ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
     if (rules.fold_mask >> act & 1u) {
       folded |= 1u << player;
