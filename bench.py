@@ -265,7 +265,8 @@ def large_table_bench(args):
                       "table_update": float(upd_ms[N:].mean())},
         "launch": {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
                    "table_atomics": 0,
-                   "counter_atomics": 2 * long_runs},
+                   # one 64-bit counter atomic per k_reduce_cand wave (4,096 records) holding a long run
+                   "counter_atomics_max": min(long_runs, -(-recs // 4096))},
     }
     for name in ("large_table_l2.json", "large_table_traffic.json"):
         path = os.path.join(ROOT, "profiles", name)

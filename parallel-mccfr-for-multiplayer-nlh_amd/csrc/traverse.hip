@@ -11,20 +11,22 @@
 //                 one path.  Lanes keep the game state in LDS and record the
 //                 update-player nodes (key, sampled action, legal mask) and
 //                 the terminal utility.
+//   k_showdown    side-pot payoffs of the traversals that reached a showdown;
+//                 counts each emission block's records.
 //   k_emit_*      records in lane order -> (key, payload) arrays; the payload
 //                 carries the record's action word and utility, so the table
 //                 update reads records in sorted order without gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
 //                 (key = segment << slot_bits | slot), 10-bit digits.
 //   k_runs        every run of equal keys is one table slot of one tenant;
-//                 runs are listed (k_reduce_cand finds the ends the per-wave
-//                 pass cannot see).  k_reduce_rows: a group of ROW lanes owns
-//                 a short run, lane a summing action a's contributions
-//                 (cfr_gpu.cu:784-837) in FP64 in record order, then applies
-//                 them: regrets += sum (CFR+ clamp), strategy sums += sum, BF16
-//                 snapshot re-matched.  Runs longer than kShortRun are cut
-//                 into kPiece-record pieces (k_reduce_piece, one wave
-//                 each) whose sums k_reduce_runs adds in piece order.
+//                 runs are listed (k_reduce_cand finds the ends of the long
+//                 ones).  k_reduce_rows: one lane owns a short run and sums
+//                 every action's contributions (cfr_gpu.cu:784-837) in FP64
+//                 in record order, then applies them: regrets += sum (CFR+
+//                 clamp), strategy sums += sum, BF16 snapshot re-matched.
+//                 Runs longer than kShortRun are cut into kPiece-record
+//                 pieces (k_reduce_piece, one wave each) whose sums
+//                 k_reduce_runs adds in piece order.
 // No atomics touch the tables, and every sum has a fixed order, so a tenant's
 // result does not depend on which other tenants share its launches.
 // Untouched slots need no sweep: clamp and regret matching are per-slot and
@@ -319,7 +321,6 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   rules.max_raises = G.max_raises;
   rules.bb = G.bb;
   const int N = G.N;
-  const int A = G.num_actions;
   const int hist_bits = G.hist_bits;
   const int n_passive = G.n_passive;
   uint64_t passive = 0;  // fallback preference list, 4 bits per entry
@@ -686,9 +687,6 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
 // Record emission: lane-ordered (key, (action | mask << 8, utility)) pairs.
 // Blocks of kEmitLanes lanes, one lane per thread.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t count4(uint32_t w) {
-  return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
-}
 
 // Exclusive scan of the block counts in place (one workgroup; k_showdown wrote
 // the counts); resets the launch counters and fills the per-segment update
@@ -1089,43 +1087,6 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
   }
 }
 
-// Contributions of one slot's records for one action (lane a of the group):
-// sigma and q from the pre-launch regret row rr (cfr_gpu.cu:784-801),
-// recomputed only when the legal mask changes (hash collisions can put info
-// sets with different masks in one slot).
-template <int ROW>
-struct RowAcc {
-  double dr = 0.0, ds = 0.0;
-  uint32_t cached_vm = 0;
-  float sum_r = 0.0f, uni = 0.0f;
-
-  __device__ __forceinline__ void add(const float (&rr)[ROW], float r_a, int a, int A, float tw, uint32_t av,
-                                      float u) {
-    const uint32_t act = av & 0xFFu, vm = av >> 8;
-    if (vm != cached_vm) {
-      cached_vm = vm;
-      float s = 0.0f;
-#pragma unroll
-      for (int b = 0; b < ROW; ++b) s += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
-      sum_r = s;
-      uni = 1.0f / (float)__popc(vm);
-    }
-    const bool pos = sum_r > 1e-7f;
-    float r_act = rr[0];
-#pragma unroll
-    for (int b = 1; b < ROW; ++b) r_act = ((uint32_t)b == act) ? rr[b] : r_act;
-    const float q = fmaxf(pos ? fmaxf(0.0f, r_act) / sum_r : uni, 0.05f);
-    const bool legal = a < A && (vm >> a & 1u);
-    const float sig_a = pos ? (legal ? fmaxf(0.0f, r_a) : 0.0f) / sum_r : (legal ? uni : 0.0f);
-    const float rd_sampled = u / q - u;
-    const float rd_other = 0.0f - u;
-    if (legal) {
-      dr += (double)((uint32_t)a == act ? rd_sampled : rd_other);
-      ds += (double)(tw * sig_a);
-    }
-  }
-};
-
 // Lane a of a ROW-lane group: R <- R + dR (CFR+ clamp), S <- S + dS, and the
 // slot's BF16 strategy re-matched from the new row (sum in action order).
 template <int ROW>
@@ -1276,7 +1237,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 // pair of divisions per record instead of one per record and action), staged
 // in LDS with the record's action word and utility.  Then its 64 / ROW groups
 // take the records in a fixed stride, lane a summing action a's contributions
-// in FP64 (RowAcc's expressions; sigma cached per legal mask), and lane a of
+// in FP64 (sigma cached per legal mask), and lane a of
 // the first group adds the groups' sums in group order (shuffles); the
 // piece's FP64 sums are stored.  A piece is defined from its run's first
 // record, so the sums depend only on the run's records, not on where the run
