@@ -193,6 +193,9 @@ __device__ __forceinline__ uint32_t rank7_finish(const Rank7Sums& a, const uint1
     const uint32_t cr = (uint32_t)(a.cnt >> (3 * r)) & 7u;
     idx += off[(r * 8 + (7 - before)) * 5 + cr];
   }
+  // keep the sum in 32 bits: otherwise the compiler folds each term into the
+  // 64-bit address (13 64-bit adds instead of one)
+  asm volatile("" : "+v"(idx));
   return nonflush[idx];
 }
 

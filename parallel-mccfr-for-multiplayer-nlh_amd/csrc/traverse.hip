@@ -1006,11 +1006,12 @@ __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __r
   }
 }
 
-// One thread per wave region's deferred runs: each run's end by a short
-// forward scan, then exponential and binary search (the keys ascend).  Short
-// runs join the region's run list; long ones are cut into kPiece-record
-// pieces.  A wave reserves the list space of all its long runs and pieces with
-// one 64-bit atomic (the long-run and piece counters share a word).
+// One thread per wave region's deferred runs (k_runs defers only runs of more
+// than kShortRun records): each run's end by a forward scan of kShortRun keys,
+// then exponential and binary search (the keys ascend); the run is cut into
+// kPiece-record pieces.  A wave reserves the list space of all its long runs
+// and pieces with one 64-bit atomic (the long-run and piece counters share a
+// word).
 __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const uint32_t n_waves = (n + 63) / 64;
@@ -1019,7 +1020,6 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
   for (uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w0 < n_waves; w0 += stride) {
     const uint32_t w = w0 + (uint32_t)lane;
     const uint32_t nc = w < n_waves ? sc.n_cand[w] : 0u;
-    uint32_t nr = nc ? sc.n_runs[w] : 0u;
     uint32_t lstart[4], llen[4];  // long runs of the region (llen 0: none)
     uint32_t nl = 0, np = 0;
 #pragma unroll
@@ -1036,29 +1036,27 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
         const uint32_t idx = i + (uint32_t)k;
         if (idx >= n || keys[idx] != key) e = idx;
       }
-      if (e - i <= kShortRun) {
-        sc.runs[(size_t)w * 64 + nr++] = make_uint4(i, e - i, key, 0u);
-        continue;
-      }
-      e = i + kShortRun + 1;  // keys[i .. i + kShortRun] all equal key
-      // keys[e - 1] == key: bracket the end, then bisect
-      uint32_t lo = e, step = 32;
-      while (lo + step < n && keys[lo + step - 1] == key) {
-        lo += step;
-        step <<= 1;
-      }
-      uint32_t hi = min(lo + step, n);
-      while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        if (keys[mid] == key) lo = mid + 1;
-        else hi = mid;
+      if (e - i > kShortRun) {
+        e = i + kShortRun + 1;  // keys[i .. i + kShortRun] all equal key
+        // keys[e - 1] == key: bracket the end, then bisect
+        uint32_t lo = e, step = 32;
+        while (lo + step < n && keys[lo + step - 1] == key) {
+          lo += step;
+          step <<= 1;
+        }
+        uint32_t hi = min(lo + step, n);
+        while (lo < hi) {
+          const uint32_t mid = lo + ((hi - lo) >> 1);
+          if (keys[mid] == key) lo = mid + 1;
+          else hi = mid;
+        }
+        e = lo;
       }
       lstart[c] = i;
-      llen[c] = lo - i;
+      llen[c] = e - i;
       ++nl;
-      np += (lo - i + kPiece - 1) / kPiece;
+      np += (e - i + kPiece - 1) / kPiece;
     }
-    if (nc) sc.n_runs[w] = nr;
     // the wave's exclusive prefix of (pieces << 32 | long runs), one atomic
     const uint64_t mine = ((uint64_t)np << 32) | nl;
     uint64_t incl = mine;
@@ -1452,8 +1450,8 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
   hipLaunchKernelGGL(k_reduce_cand, dim3(rgrid / 64 + 1), dim3(256), 0, s, sc, keys);
 #define MCCFR_REDUCE(R)                                                                                \
-  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);               \
-  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);              \
+  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);              \
+  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);             \
   hipLaunchKernelGGL(k_reduce_runs<R>, dim3(2048), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
     MCCFR_REDUCE(8);
