@@ -1122,7 +1122,9 @@ __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, ui
 // exact in FP64, and equal to the record-by-record sum for a run of one mask
 // (the rare earlier stretches of a run wait in LDS).
 // A lane per run keeps a wave's row gathers in flight together; row accesses
-// are latency-bound, not bandwidth-bound.
+// are latency-bound, not bandwidth-bound.  8-wide rows are held in registers
+// through the record loop (96 registers, 5 waves per SIMD; variants with 6
+// waves measured the same); 16-wide rows are re-read (below).
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                      const uint2* __restrict__ vals) {
@@ -1150,81 +1152,191 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
       const uint4 run = sc.runs[j < nr0 ? (size_t)r0 * 64 + j : (size_t)(r0 + 1) * 64 + (j - nr0)];
       const uint32_t slot = run.z & slot_mask;
       const SegCtx c = one_seg ? c0 : sc.seg_ctx[(uint32_t)((uint64_t)run.z >> lc.slot_bits)];
-      const size_t rs_off = (size_t)slot * 2 * ROW;  // interleaved regret / strategy-sum row
-      const int A = c.num_actions;
-      float rr[ROW];
-      load_f32_row<ROW>(c.R + rs_off, rr);
-      double dr[ROW];
+      if constexpr (ROW <= 8) {
+        const size_t rs_off = (size_t)slot * 2 * ROW;  // interleaved regret / strategy-sum row
+        const int A = c.num_actions;
+        float rr[ROW];
+        load_f32_row<ROW>(c.R + rs_off, rr);
+        double dr[ROW];
 #pragma unroll
-      for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
-      // the current mask's sum of positive regrets, uniform share and q per action
-      uint32_t cur_vm = 0, cnt = 0;
-      float sum_r = 0.0f, uni = 0.0f, qv[ROW];
+        for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
+        // the current mask's sum of positive regrets, uniform share and q per action
+        uint32_t cur_vm = 0, cnt = 0;
+        float sum_r = 0.0f, uni = 0.0f, qv[ROW];
 #pragma unroll
-      for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
-      // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
-      auto stretch_ds = [&](int b) -> double {
-        const bool legal = b < A && (cur_vm >> b & 1u);
-        const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
-        return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
-      };
-      bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
-      for (uint32_t k = 0; k < run.y; ++k) {
-        const uint2 v = s_v[wv][run.x - base + k];
-        const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
-        const float u = __uint_as_float(v.y);
-        if (vm != cur_vm) {
-          if (cnt) {
+        for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
+        // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
+        auto stretch_ds = [&](int b) -> double {
+          const bool legal = b < A && (cur_vm >> b & 1u);
+          const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
+          return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
+        };
+        bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
+        for (uint32_t k = 0; k < run.y; ++k) {
+          const uint2 v = s_v[wv][run.x - base + k];
+          const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+          const float u = __uint_as_float(v.y);
+          if (vm != cur_vm) {
+            if (cnt) {
 #pragma unroll
-            for (int b = 0; b < ROW; ++b) s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
-            spilled = true;
+              for (int b = 0; b < ROW; ++b) s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
+              spilled = true;
+            }
+            cnt = 0;
+            cur_vm = vm;
+            float t = 0.0f;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
+            sum_r = t;
+            uni = 1.0f / (float)__popc(vm);
+            const bool pos = t > 1e-7f;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
           }
-          cnt = 0;
-          cur_vm = vm;
-          float t = 0.0f;
+          ++cnt;
+          float q = qv[0];
 #pragma unroll
-          for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
-          sum_r = t;
-          uni = 1.0f / (float)__popc(vm);
-          const bool pos = t > 1e-7f;
+          for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+          const float rd_sampled = u / q - u;
+          const float rd_other = 0.0f - u;
 #pragma unroll
-          for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
+          for (int b = 0; b < ROW; ++b)
+            if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
         }
-        ++cnt;
-        float q = qv[0];
+        float ss[ROW];
+        load_f32_row<ROW>(c.S + rs_off, ss);
+        double ds[ROW];
 #pragma unroll
-        for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-        const float rd_sampled = u / q - u;
-        const float rd_other = 0.0f - u;
+        for (int b = 0; b < ROW; ++b) ds[b] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
+        // R <- R + dR (CFR+ clamp), S <- S + dS, sigma re-matched from the new row
+        float nr[ROW], ns[ROW];
+        float sum = 0.0f;
 #pragma unroll
-        for (int b = 0; b < ROW; ++b)
-          if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+        for (int b = 0; b < ROW; ++b) {
+          nr[b] = (float)((double)rr[b] + dr[b]);
+          if (c.cfr_plus) nr[b] = fmaxf(0.0f, nr[b]);
+          ns[b] = (float)((double)ss[b] + ds[b]);
+          sum += fmaxf(0.0f, nr[b]);
+        }
+        uint32_t sg[ROW / 2];
+#pragma unroll
+        for (int b = 0; b < ROW; b += 2) {
+          const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
+          const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
+          sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+        }
+        store_f32_row<ROW>(c.R + rs_off, nr);
+        store_f32_row<ROW>(c.S + rs_off, ns);
+        store_bf16_pairs<ROW>(c.sig + (size_t)slot * ROW, sg);
+      } else {
+        const size_t rs_off = (size_t)slot * 2 * ROW;  // interleaved regret / strategy-sum row
+        const int A = c.num_actions;
+        double dr[ROW];
+#pragma unroll
+        for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
+        // the current mask's sum of positive regrets, uniform share and q per
+        // action.  The 16-wide regret row is read again where it is needed (at
+        // a mask change and for the apply) rather than held through the
+        // record loop: 176 -> 114 registers, 2 -> 4 waves per SIMD.
+        uint32_t cur_vm = 0, cnt = 0;
+        float sum_r = 0.0f, uni = 0.0f, qv[ROW];
+#pragma unroll
+        for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
+        // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
+        auto stretch_ds = [&](int b, float r_b) -> double {
+          const bool legal = b < A && (cur_vm >> b & 1u);
+          const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, r_b) : 0.0f) / sum_r : (legal ? uni : 0.0f);
+          return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
+        };
+        bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
+        for (uint32_t k = 0; k < run.y; ++k) {
+          const uint2 v = s_v[wv][run.x - base + k];
+          const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+          const float u = __uint_as_float(v.y);
+          if (vm != cur_vm) {
+            float rr[ROW];
+            load_f32_row<ROW>(c.R + rs_off, rr);
+            if (cnt) {
+#pragma unroll
+              for (int b = 0; b < ROW; ++b)
+                s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b, rr[b]);
+              spilled = true;
+            }
+            cnt = 0;
+            cur_vm = vm;
+            float t = 0.0f;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
+            sum_r = t;
+            uni = 1.0f / (float)__popc(vm);
+            const bool pos = t > 1e-7f;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
+          }
+          ++cnt;
+          float q = qv[0];
+#pragma unroll
+          for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+          const float rd_sampled = u / q - u;
+          const float rd_other = 0.0f - u;
+#pragma unroll
+          for (int b = 0; b < ROW; ++b)
+            if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+        }
+        asm volatile("" ::: "memory");  // the rows below are loaded here, not kept from the loop
+        // R <- R + dR (CFR+ clamp), S <- S + dS four actions at a time, loads of
+        // a chunk after the previous chunk's stores (only the new regrets stay
+        // live, for the re-match); sigma re-matched from the new row (sum in
+        // action order)
+        float rr[ROW], ss[ROW];
+        float nr[ROW];
+        float sum = 0.0f;
+        const auto qr = gbl(reinterpret_cast<f32x4*>(c.R + rs_off));
+        const auto qs = gbl(reinterpret_cast<f32x4*>(c.S + rs_off));
+#pragma unroll
+        for (int q4 = 0; q4 < ROW / 4; ++q4) {
+          const f32x4 r4 = qr[q4], s4 = qs[q4];
+          rr[4 * q4] = r4.x;
+          rr[4 * q4 + 1] = r4.y;
+          rr[4 * q4 + 2] = r4.z;
+          rr[4 * q4 + 3] = r4.w;
+          ss[4 * q4] = s4.x;
+          ss[4 * q4 + 1] = s4.y;
+          ss[4 * q4 + 2] = s4.z;
+          ss[4 * q4 + 3] = s4.w;
+          float ns[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int b = 4 * q4 + j;
+            const double ds = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b, rr[b]);
+            float x = (float)((double)rr[b] + dr[b]);
+            if (c.cfr_plus) x = fmaxf(0.0f, x);
+            nr[b] = x;
+            ns[j] = (float)((double)ss[b] + ds);
+            sum += fmaxf(0.0f, x);
+          }
+          f32x4 o;
+          o.x = nr[4 * q4];
+          o.y = nr[4 * q4 + 1];
+          o.z = nr[4 * q4 + 2];
+          o.w = nr[4 * q4 + 3];
+          qr[q4] = o;
+          f32x4 o2;
+          o2.x = ns[0];
+          o2.y = ns[1];
+          o2.z = ns[2];
+          o2.w = ns[3];
+          qs[q4] = o2;
+        }
+        uint32_t sg[ROW / 2];
+#pragma unroll
+        for (int b = 0; b < ROW; b += 2) {
+          const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
+          const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
+          sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
+        }
+        store_bf16_pairs<ROW>(c.sig + (size_t)slot * ROW, sg);
       }
-      float ss[ROW];
-      load_f32_row<ROW>(c.S + rs_off, ss);
-      double ds[ROW];
-#pragma unroll
-      for (int b = 0; b < ROW; ++b) ds[b] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
-      // R <- R + dR (CFR+ clamp), S <- S + dS, sigma re-matched from the new row
-      float nr[ROW], ns[ROW];
-      float sum = 0.0f;
-#pragma unroll
-      for (int b = 0; b < ROW; ++b) {
-        nr[b] = (float)((double)rr[b] + dr[b]);
-        if (c.cfr_plus) nr[b] = fmaxf(0.0f, nr[b]);
-        ns[b] = (float)((double)ss[b] + ds[b]);
-        sum += fmaxf(0.0f, nr[b]);
-      }
-      uint32_t sg[ROW / 2];
-#pragma unroll
-      for (int b = 0; b < ROW; b += 2) {
-        const float x0 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b]) / sum : 0.0f;
-        const float x1 = (sum > 1e-7f) ? fmaxf(0.0f, nr[b + 1]) / sum : 0.0f;
-        sg[b / 2] = (uint32_t)f32_to_bf16(x0) | ((uint32_t)f32_to_bf16(x1) << 16);
-      }
-      store_f32_row<ROW>(c.R + rs_off, nr);
-      store_f32_row<ROW>(c.S + rs_off, ns);
-      store_bf16_pairs<ROW>(c.sig + (size_t)slot * ROW, sg);
     }
     wave_lds_sync();  // the next pair's staging overwrites s_v
   }
