@@ -194,33 +194,32 @@ __device__ __forceinline__ uint32_t menu_word(const DevGame& g, int a) {
 __device__ __forceinline__ uint32_t den_reciprocal(uint32_t den) { return den >= 2 ? 0xFFFFFFFFu / den + 1u : 0u; }
 
 // chips_for over the wave's LDS copy of the menu (divergent action index);
-// rcp[a] = den_reciprocal(den of entry a).
+// rcp[a] = den_reciprocal(den of entry a).  Branch-free: the call, all-in and
+// raise amounts are all formed and the action's kind selects one.
 __device__ __forceinline__ int chips_lds(const DevGame& g, const uint32_t* menu, const uint32_t* rcp, int a, int pot,
                                          int stack, int to_call, int current_bet, int lfr) {
-  if (pot <= 0) pot = 1;
+  pot = pot <= 0 ? 1 : pot;
   const uint32_t m = menu[a];
   const int k = (int)(m & 0xFFu);
-  if (k == MCCFR_ACT_FOLD || k == MCCFR_ACT_CHECK) return 0;
-  if (k == MCCFR_ACT_CALL || k == MCCFR_ACT_CHECK_CALL) return to_call < stack ? to_call : stack;
-  if (k == MCCFR_ACT_ALL_IN) return stack;
+  const int call_chips = to_call < stack ? to_call : stack;
   const int mrt = min_raise_to(current_bet, lfr, g.bb);
   const int head = stack - to_call;
   const int maxrt = current_bet + (head > 0 ? head : 0);
-  int target = mrt;
-  if (k == MCCFR_ACT_POT_FRAC) {
-    // x / den for x = num * pot < 2^26 and 1 <= den < 2^8: q = hi32(x * ceil(2^32 / den))
-    // is x / den or one more (x * e / 2^32 < 1 for e = ceil(2^32 / den) - 2^32 / den < 1)
-    const uint32_t x = __umul24((m >> 8) & 0xFFu, (uint32_t)pot), den = m >> 16;
-    uint32_t q = den >= 2 ? __umulhi(x, rcp[a]) : x;
-    if (q * den > x) --q;
-    int inc = (int)q;
-    inc = inc > 1 ? inc : 1;
-    target = current_bet + inc > mrt ? current_bet + inc : mrt;
-  }
+  // x / den for x = num * pot < 2^26 and 1 <= den < 2^8: q = hi32(x * ceil(2^32 / den))
+  // is x / den or one more (x * e / 2^32 < 1 for e = ceil(2^32 / den) - 2^32 / den < 1);
+  // entries other than pot fractions have den 1
+  const uint32_t x = __umul24((m >> 8) & 0xFFu, (uint32_t)pot), den = m >> 16;
+  uint32_t q = den >= 2 ? __umulhi(x, rcp[a]) : x;
+  q = q * den > x ? q - 1u : q;
+  const int inc = (int)q > 1 ? (int)q : 1;
+  int target = (k == MCCFR_ACT_POT_FRAC && current_bet + inc > mrt) ? current_bet + inc : mrt;
   target = target < maxrt ? target : maxrt;
   const int add = target - current_bet;
-  const int chips = to_call + (add > 0 ? add : 0);
-  return chips < stack ? chips : stack;
+  const int raise_chips = min(to_call + (add > 0 ? add : 0), stack);
+  return (k == MCCFR_ACT_FOLD || k == MCCFR_ACT_CHECK) ? 0
+         : (k == MCCFR_ACT_CALL || k == MCCFR_ACT_CHECK_CALL) ? call_chips
+         : k == MCCFR_ACT_ALL_IN ? stack
+                                 : raise_chips;
 }
 
 // ---------------------------------------------------------------------------
@@ -245,38 +244,48 @@ constexpr uint32_t kStackMask = 0x7FFFu;  // chips behind: bits 0..14; bet: 15..
 struct Rules {
   uint32_t check_mask, call_mask, fold_mask, allin_mask, raise_mask;
   int max_raises, bb;
+  // the first four raise entries, wave-uniform: num | den << 8 | (1 << a) << 16
+  // (a min-raise as num 0, den 1: always of legal size); further raise
+  // entries are read from the LDS menu
+  uint32_t fast[4];
+  uint32_t raise_rest;
 };
 
+// Branch-free: every lane evaluates every case and the result is selected
+// (a wave's lanes sit in different cases; early returns made divergent
+// regions).  Same outcomes as valid_mask: facing a bet it cannot cover, a seat
+// may fold or go all in; with no chips behind, or at the raise cap, it may only
+// check / fold / call; otherwise the raises of legal size and all-in are added.
 __device__ __forceinline__ uint32_t rules_mask(const Rules& g, const uint32_t* menu, int pot, int stack, int to_call,
                                                int current_bet, int lfr, int raises) {
-  if (pot <= 0) pot = 1;
-  uint32_t mask;
-  if (to_call == 0) {
-    mask = g.check_mask;
-  } else if (to_call >= stack) {
-    return g.fold_mask | g.allin_mask;
-  } else {
-    mask = g.fold_mask | g.call_mask;
-  }
+  pot = pot <= 0 ? 1 : pot;
   const int head = stack - to_call;
-  if (head <= 0) return mask;
-  if (g.max_raises > 0 && raises >= g.max_raises) return mask;
-  const int mrt = min_raise_to(current_bet, lfr, g.bb);
-  if (current_bet + head >= mrt) {
-    // max(current_bet + max(1, num*pot/den), mrt) <= current_bet + head
-    //   <=> num*pot < (head + 1)*den, given mrt <= current_bet + head
-    // (pot < 2^18, head < 2^15, num and den < 2^8: 24-bit multiplies suffice)
-    uint32_t rm = g.raise_mask;
-    while (rm) {
-      const int a = __builtin_ctz(rm);
-      rm &= rm - 1;
-      const uint32_t m = menu[a];
-      if ((m & 0xFFu) == MCCFR_ACT_RAISE_MIN ||
-          __umul24((m >> 8) & 0xFFu, (uint32_t)pot) < __umul24((uint32_t)(head + 1), m >> 16))
-        mask |= 1u << a;
-    }
+  const uint32_t base = to_call == 0 ? g.check_mask : (g.fold_mask | g.call_mask);
+  // max(current_bet + max(1, num*pot/den), mrt) <= current_bet + head
+  //   <=> num*pot < (head + 1)*den, given mrt <= current_bet + head
+  // (pot < 2^18, head < 2^15, num and den < 2^8: 24-bit multiplies suffice;
+  // for head <= 0 the bits are discarded below)
+  uint32_t rbits = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t f = g.fast[i];
+    const bool ok = __umul24(f & 0xFFu, (uint32_t)pot) < __umul24((uint32_t)(head + 1), (f >> 8) & 0xFFu);
+    rbits |= ok ? f >> 16 : 0u;  // an unused slot has no bit
   }
-  return mask | g.allin_mask;
+  uint32_t rm = g.raise_rest;  // wave-uniform
+  while (rm) {
+    const int a = __builtin_ctz(rm);
+    rm &= rm - 1;
+    const uint32_t m = menu[a];
+    const bool ok = (m & 0xFFu) == MCCFR_ACT_RAISE_MIN ||
+                    __umul24((m >> 8) & 0xFFu, (uint32_t)pot) < __umul24((uint32_t)(head + 1), m >> 16);
+    rbits |= ok ? 1u << a : 0u;
+  }
+  const bool capped = g.max_raises > 0 && raises >= g.max_raises;
+  const int mrt = min_raise_to(current_bet, lfr, g.bb);
+  const uint32_t open = (current_bet + head >= mrt ? rbits : 0u) | g.allin_mask;
+  const uint32_t mask = (head > 0 && !capped) ? (base | open) : base;
+  return (to_call != 0 && to_call >= stack) ? (g.fold_mask | g.allin_mask) : mask;
 }
 
 template <int ROW, int WG, bool TRACE>
@@ -320,6 +329,21 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   rules.raise_mask = G.raise_mask;
   rules.max_raises = G.max_raises;
   rules.bb = G.bb;
+  {
+    uint32_t rm = rules.raise_mask;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      rules.fast[i] = 0u;
+      if (rm) {
+        const int a = __builtin_ctz(rm);
+        rm &= rm - 1;
+        const bool min_raise = G.kind[a] == MCCFR_ACT_RAISE_MIN;
+        rules.fast[i] = (min_raise ? 0u : (uint32_t)G.num[a]) | ((min_raise ? 1u : (uint32_t)G.den[a]) << 8) |
+                        ((1u << a) << 16);
+      }
+    }
+    rules.raise_rest = rm;
+  }
   const int N = G.N;
   const int hist_bits = G.hist_bits;
   const int n_passive = G.n_passive;
