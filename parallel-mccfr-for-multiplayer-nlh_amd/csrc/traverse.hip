@@ -540,21 +540,27 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
     float sg[ROW];
     load_bf16_row<ROW>(sig + (size_t)slot * ROW, sg);
-    // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot
+    // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot: running sums
+    // of the legal actions' sigma in action order (an illegal action adds 0,
+    // which leaves a sum unchanged; legal actions are menu indices below A)
+    float cum[ROW];
     float total = 0.0f;
 #pragma unroll
-    for (int a = 0; a < ROW; ++a)  // legal actions are menu indices, all below A
-      if (vm >> a & 1u) total += sg[a];
+    for (int a = 0; a < ROW; ++a) {
+      total += (vm >> a & 1u) ? sg[a] : 0.0f;
+      cum[a] = total;
+    }
     int act = -1;
     if (total >= 1e-7f) {
+      // the first legal action whose running sum reaches r.  r > 0 and the
+      // sums do not decrease, so that action's index is the number of sums
+      // below r (an action that adds nothing is never the first to reach r),
+      // and r <= total keeps it below ROW.
       const float r = u01(rng.next()) * total;
-      float cum = 0.0f;
+      int below = 0;
 #pragma unroll
-      for (int a = 0; a < ROW; ++a) {
-        const bool valid = vm >> a & 1u;
-        if (valid) cum += sg[a];
-        if (valid && act < 0 && r <= cum) act = a;
-      }
+      for (int a = 0; a < ROW; ++a) below += cum[a] < r ? 1 : 0;
+      act = below;
     }
     if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508, 520-521)
       for (int i = 0; i < n_passive; ++i) {
