@@ -346,15 +346,11 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   }
   const int N = G.N;
   const int hist_bits = G.hist_bits;
-  const int n_passive = G.n_passive;
-  uint64_t passive = 0;  // fallback preference list, 4 bits per entry
-  for (int i = 0; i < n_passive; ++i) passive |= (uint64_t)(G.passive_order[i] & 15u) << (4 * i);
   const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
   const uint16_t* __restrict__ sig = G.sig;
   const uint32_t seg_key = uniform((uint32_t)((uint64_t)si << lc.slot_bits));
   const uint32_t lane0 = Sg.lane0, seg_n = Sg.n;
   const int up = Sg.update_player;
-  const int up_stack0 = G.stack[up];
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
 
@@ -493,7 +489,9 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
     }
     const uint32_t alive = allmask & ~folded;
     if (__popc(alive) == 1) {
-      const int inv_up = up_stack0 - (int)(sb[up * WG] & kStackMask);
+      // the tenant's fields used on rare paths are read where they are used,
+      // not held in scalar registers through the walk
+      const int inv_up = G.stack[up] - (int)(sb[up * WG] & kStackMask);
       u = ((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
       status = 1;
       continue;
@@ -563,8 +561,8 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
       act = below;
     }
     if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508, 520-521)
-      for (int i = 0; i < n_passive; ++i) {
-        const int a = (int)((passive >> (4 * i)) & 15u);
+      for (int i = 0; i < G.n_passive; ++i) {
+        const int a = (int)(G.passive_order[i] & 15u);
         if (vm >> a & 1u) {
           act = a;
           break;
