@@ -779,15 +779,22 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
   __shared__ uint2 s_val[kEmitStage];
   const uint32_t t = blockIdx.x * kEmitLanes + threadIdx.x;
   const uint32_t c = t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u;
+  const size_t L = sc.cap_lanes;
+  // a lane's first kAhead records and its utility are loaded before the scan,
+  // so their latency overlaps it (most lanes hold at most kAhead records)
+  constexpr uint32_t kAhead = 4;
+  uint32_t ak[kAhead], aa[kAhead];
+#pragma unroll
+  for (uint32_t k = 0; k < kAhead; ++k) {
+    ak[k] = k < c ? sc.node_slot[(size_t)k * L + t] : 0u;
+    aa[k] = k < c ? sc.node_av[(size_t)k * L + t] : 0u;
+  }
+  const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
   uint32_t total = 0;
   uint32_t pos = block_excl_scan_256(c, s_w, &total);
   const uint32_t gbase = sc.blk_cnt[blockIdx.x];
   const bool staged = total <= kEmitStage;  // block-uniform
-  const size_t L = sc.cap_lanes;
-  const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
-  for (uint32_t k = 0; k < c; ++k, ++pos) {
-    const uint32_t key = sc.node_slot[(size_t)k * L + t];
-    const uint2 v = make_uint2(sc.node_av[(size_t)k * L + t], ub);
+  auto put = [&](uint32_t key, uint2 v) {
     if (staged) {
       s_key[pos] = key;
       s_val[pos] = v;
@@ -795,7 +802,12 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
       sc.key[0][gbase + pos] = key;
       sc.val[0][gbase + pos] = v;
     }
-  }
+    ++pos;
+  };
+#pragma unroll
+  for (uint32_t k = 0; k < kAhead; ++k)
+    if (k < c) put(ak[k], make_uint2(aa[k], ub));
+  for (uint32_t k = kAhead; k < c; ++k) put(sc.node_slot[(size_t)k * L + t], make_uint2(sc.node_av[(size_t)k * L + t], ub));
   if (staged) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < total; i += 256) {
