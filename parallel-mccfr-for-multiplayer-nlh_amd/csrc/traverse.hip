@@ -648,29 +648,34 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
   static_assert(kEmitLanes == 256, "a showdown workgroup is one emission block");
   __shared__ uint32_t s_w[4];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  {
-    // the emission block's record count (the walk has set every lane's count)
-    uint32_t total = 0;
-    (void)block_excl_scan_256(t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u, s_w, &total);
-    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
+  const bool in = t < lc.lanes;
+  // every load is issued before the block scan (its barriers), so their
+  // latencies overlap it; lanes return only after the scan
+  const uint32_t cnt = in ? (uint32_t)sc.node_count[t] : 0u;
+  const uint32_t st = in ? (uint32_t)sc.sd_state[t] : 0u;
+  const bool showdown = (st & kShowdownFlag) != 0;
+  int N = 0, up = 0;
+  if (uniform(t & ~63u) < lc.lanes) {  // wave-uniform: the wave's segment
+    const DevSeg& Sg = segs[find_seg(segs, lc.n_segs, uniform(t & ~63u))];
+    N = tenants[uniform(Sg.tenant)].N;
+    up = Sg.update_player;
   }
-  if (t >= lc.lanes) return;
-  const uint32_t si = find_seg(segs, lc.n_segs, uniform(t & ~63u));
-  const uint32_t st = sc.sd_state[t];
-  if (!(st & kShowdownFlag)) return;
-  const DevSeg& Sg = segs[si];
-  const DevGame& G = tenants[uniform(Sg.tenant)];
-  const int N = G.N;
-  const int up = Sg.update_player;
-  const uint32_t folded = st & ~kShowdownFlag;
   const size_t L = sc.cap_lanes;
   int inv[kMaxPlayers];
   uint32_t rkv[kMaxPlayers];
 #pragma unroll
   for (int p = 0; p < kMaxPlayers; ++p) {
-    inv[p] = p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
-    rkv[p] = p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+    inv[p] = showdown && p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
+    rkv[p] = showdown && p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
   }
+  {
+    // the emission block's record count (the walk has set every lane's count)
+    uint32_t total = 0;
+    (void)block_excl_scan_256(cnt, s_w, &total);
+    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
+  }
+  if (!showdown) return;
+  const uint32_t folded = st & ~kShowdownFlag;
   int inv_up = inv[0];
   uint32_t rk_up = rkv[0];
 #pragma unroll
