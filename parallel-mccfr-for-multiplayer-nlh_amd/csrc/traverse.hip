@@ -867,13 +867,19 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
   for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
     const uint32_t tile = xcd_tile(it, nt);
     if (tile >= nt) continue;  // workgroup-uniform
+    uint32_t kd[kSortItems];  // the tile's keys, loaded before the counts are cleared
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
+      kd[i] = idx < n ? keys[idx] : 0u;
+    }
     for (uint32_t b = tx; b < kSortBins; b += kSortWG) s_h[b] = 0;
     __syncthreads();
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
       const bool valid = idx < n;
-      const uint32_t d = valid ? (keys[idx] >> shift) & (kSortBins - 1u) : 0u;
+      const uint32_t d = valid ? (kd[i] >> shift) & (kSortBins - 1u) : 0u;
       const uint64_t peers = digit_peers(d, valid);  // one LDS add per distinct digit of the wave
       if (valid && (peers & lt) == 0) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
     }
@@ -928,28 +934,16 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
   const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
   // first output position of each digit: exclusive scan of the digit totals
-  // (thread tx owns digits tx*kPerThread .. +kPerThread-1)
+  // (thread tx owns digits tx*kPerThread .. +kPerThread-1), formed at the
+  // workgroup's first tile
   uint32_t digit_base[kPerThread];
-  {
-    uint32_t sum = 0;
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-      digit_base[q] = sum;
-      sum += bin_total[tx * kPerThread + q];
-    }
-    const uint32_t b0 = block_excl_scan_256(sum, s_w, nullptr);
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) digit_base[q] += b0;
-  }
+  bool based = false;
   const uint32_t items = xcd_items(nt);
   for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
     const uint32_t tile = xcd_tile(it, nt);
     if (tile >= nt) continue;  // workgroup-uniform
-#pragma unroll
-    for (int k = 0; k < kWaves; ++k)
-#pragma unroll
-      for (int q = 0; q < kPerThread; ++q) s_wc[k][tx * kPerThread + q] = 0;
-    __syncthreads();
+    // the tile's loads first (keys, values, this thread's digits' tile
+    // offsets), so their latency overlaps the scans and the ranking set-up;
     // wave w holds keys [wbase, wbase + 64 * kSortItems) of the tile, item-major
     uint32_t kk[kSortItems], off[kSortItems];
     uint2 vv[kSortItems];
@@ -961,6 +955,26 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
       kk[i] = valid ? kin[idx] : 0u;
       vv[i] = valid ? vin[idx] : make_uint2(0u, 0u);
     }
+    uint32_t toff[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) toff[q] = hist[(size_t)(tx * kPerThread + q) * max_tiles + tile];
+    if (!based) {  // workgroup-uniform
+      uint32_t sum = 0;
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) {
+        digit_base[q] = sum;
+        sum += bin_total[tx * kPerThread + q];
+      }
+      const uint32_t b0 = block_excl_scan_256(sum, s_w, nullptr);
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) digit_base[q] += b0;
+      based = true;
+    }
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k)
+#pragma unroll
+      for (int q = 0; q < kPerThread; ++q) s_wc[k][tx * kPerThread + q] = 0;
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
@@ -985,7 +999,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
         s_wc[k][dd] = run;
         run += c;
       }
-      s_gb[dd] = digit_base[q] + hist[(size_t)dd * max_tiles + tile];
+      s_gb[dd] = digit_base[q] + toff[q];
     }
     __syncthreads();
 #pragma unroll
