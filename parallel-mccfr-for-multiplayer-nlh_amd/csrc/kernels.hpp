@@ -16,7 +16,8 @@ constexpr int kRegPending = 4;
 constexpr int kSortWG = 256;
 constexpr int kSortItems = 8;
 constexpr uint32_t kSortTile = kSortWG * kSortItems;
-constexpr uint32_t kDigitBits = 10;
+constexpr uint32_t kDigitBits = 10;     // widest radix digit (a launch spreads its key bits evenly over passes)
+constexpr uint32_t kMinDigitBits = 8;   // narrowest (a scatter thread owns whole digits)
 constexpr uint32_t kSortBins = 1u << kDigitBits;
 
 // Scratch of one launch, sized for `cap_lanes` lanes.  Node lists are

@@ -845,20 +845,23 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t i, uint32_t nt) {
 __device__ __forceinline__ uint32_t xcd_items(uint32_t nt) { return (nt + kXcds - 1) / kXcds * kXcds; }
 
 // Lanes of the wave whose digit equals this lane's, among `valid` lanes.
+template <uint32_t BITS>
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
   uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int b = 0; b < (int)kDigitBits; ++b) {
+  for (int b = 0; b < (int)BITS; ++b) {
     const uint64_t bal = __ballot((d >> b) & 1u);
     peers &= ((d >> b) & 1u) ? bal : ~bal;
   }
   return peers;
 }
 
+template <uint32_t BITS>
 __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict__ keys,
                                                      const LaunchCounters* __restrict__ cnt, uint32_t shift,
                                                      uint32_t* __restrict__ hist, uint32_t max_tiles) {
-  __shared__ uint32_t s_h[kSortBins];
+  constexpr uint32_t Bins = 1u << BITS;
+  __shared__ uint32_t s_h[Bins];
   const uint32_t n = cnt->n_rec;
   const uint32_t nt = n_tiles_of(n);
   const int tx = threadIdx.x, lane = tx & 63;
@@ -873,18 +876,18 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
       const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
       kd[i] = idx < n ? keys[idx] : 0u;
     }
-    for (uint32_t b = tx; b < kSortBins; b += kSortWG) s_h[b] = 0;
+    for (uint32_t b = tx; b < Bins; b += kSortWG) s_h[b] = 0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = tile * kSortTile + (uint32_t)(i * kSortWG + tx);
       const bool valid = idx < n;
-      const uint32_t d = valid ? (kd[i] >> shift) & (kSortBins - 1u) : 0u;
-      const uint64_t peers = digit_peers(d, valid);  // one LDS add per distinct digit of the wave
+      const uint32_t d = valid ? (kd[i] >> shift) & (Bins - 1u) : 0u;
+      const uint64_t peers = digit_peers<BITS>(d, valid);  // one LDS add per distinct digit of the wave
       if (valid && (peers & lt) == 0) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t b = tx; b < kSortBins; b += kSortWG) hist[(size_t)b * max_tiles + tile] = s_h[b];
+    for (uint32_t b = tx; b < Bins; b += kSortWG) hist[(size_t)b * max_tiles + tile] = s_h[b];
     __syncthreads();
   }
 }
@@ -918,16 +921,18 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, ui
   if (threadIdx.x == 0) bin_total[blockIdx.x] = carry;
 }
 
+template <uint32_t BITS>
 __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint2* __restrict__ vin,
                                                         uint32_t* __restrict__ kout, uint2* __restrict__ vout,
                                                         const LaunchCounters* __restrict__ cnt, uint32_t shift,
                                                         const uint32_t* __restrict__ hist,
                                                         const uint32_t* __restrict__ bin_total, uint32_t max_tiles) {
-  static_assert(kSortWG == 256 && kSortBins % kSortWG == 0, "whole digits per thread");
+  constexpr uint32_t Bins = 1u << BITS;
+  static_assert(kSortWG == 256 && Bins % kSortWG == 0, "whole digits per thread");
   constexpr int kWaves = kSortWG / 64;
-  constexpr int kPerThread = (int)(kSortBins / kSortWG);  // digits owned by each thread
-  __shared__ uint32_t s_wc[kWaves][kSortBins];  // per-wave digit counts, then per-wave bases
-  __shared__ uint32_t s_gb[kSortBins];          // global offset of (digit, tile)
+  constexpr int kPerThread = (int)(Bins / kSortWG);  // digits owned by each thread
+  __shared__ uint32_t s_wc[kWaves][Bins];  // per-wave digit counts, then per-wave bases
+  __shared__ uint32_t s_gb[Bins];          // global offset of (digit, tile)
   __shared__ uint32_t s_w[4];
   const uint32_t n = cnt->n_rec;
   const uint32_t nt = n_tiles_of(n);
@@ -979,8 +984,8 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       const bool valid = idx < n;
-      const uint32_t d = (kk[i] >> shift) & (kSortBins - 1u);
-      const uint64_t peers = digit_peers(d, valid);
+      const uint32_t d = (kk[i] >> shift) & (Bins - 1u);
+      const uint64_t peers = digit_peers<BITS>(d, valid);
       // all lanes read the wave's count before the group's first lane bumps it
       const uint32_t before = s_wc[w][d];
       off[i] = before + (uint32_t)__popcll(peers & lt);
@@ -1006,7 +1011,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       if (idx < n) {
-        const uint32_t d = (kk[i] >> shift) & (kSortBins - 1u);
+        const uint32_t d = (kk[i] >> shift) & (Bins - 1u);
         const uint32_t pos = s_gb[d] + s_wc[w][d] + off[i];
         kout[pos] = kk[i];
         vout[pos] = vv[i];
@@ -1605,15 +1610,27 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
   // a multiple of the XCD count (the tile order assumes it)
   const uint32_t sgrid = std::min<uint32_t>((std::max<uint32_t>(max_rec_tiles, 1u) + 7u) / 8u * 8u, 2048u);
+  // as few passes of at most kDigitBits as the key needs, the bits spread
+  // evenly over them (26-bit keys: 3 passes of 9 bits, 512 digits each)
   const uint32_t passes = key_bits == 0 ? 1u : (key_bits + kDigitBits - 1u) / kDigitBits;
+  const uint32_t bits = std::max(kMinDigitBits, std::min(kDigitBits, (key_bits + passes - 1u) / passes));
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t shift = kDigitBits * p;
+    const uint32_t shift = bits * p;
     const uint32_t* kin = sc.key[p & 1];
     const uint2* vin = sc.val[p & 1];
-    hipLaunchKernelGGL(k_rs_hist, dim3(sgrid), dim3(kSortWG), 0, s, kin, sc.cnt, shift, sc.hist, sc.max_tiles);
-    hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(256), 0, s, sc.hist, sc.bin_total, sc.cnt, sc.max_tiles);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(sgrid), dim3(kSortWG), 0, s, kin, vin, sc.key[(p + 1) & 1],
-                       sc.val[(p + 1) & 1], sc.cnt, shift, sc.hist, sc.bin_total, sc.max_tiles);
+#define MCCFR_PASS(B)                                                                                     \
+  hipLaunchKernelGGL(k_rs_hist<B>, dim3(sgrid), dim3(kSortWG), 0, s, kin, sc.cnt, shift, sc.hist, sc.max_tiles);  \
+  hipLaunchKernelGGL(k_rs_scan, dim3(1u << B), dim3(256), 0, s, sc.hist, sc.bin_total, sc.cnt, sc.max_tiles);   \
+  hipLaunchKernelGGL(k_rs_scatter<B>, dim3(sgrid), dim3(kSortWG), 0, s, kin, vin, sc.key[(p + 1) & 1],            \
+                     sc.val[(p + 1) & 1], sc.cnt, shift, sc.hist, sc.bin_total, sc.max_tiles)
+    if (bits == 8) {
+      MCCFR_PASS(8);
+    } else if (bits == 9) {
+      MCCFR_PASS(9);
+    } else {
+      MCCFR_PASS(10);
+    }
+#undef MCCFR_PASS
   }
   const uint32_t* keys = sc.key[passes & 1];
   const uint2* vals = sc.val[passes & 1];
