@@ -932,7 +932,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
   constexpr int kWaves = kSortWG / 64;
   constexpr int kPerThread = (int)(Bins / kSortWG);  // digits owned by each thread
   __shared__ uint32_t s_wc[kWaves][Bins];  // per-wave digit counts, then per-wave bases
-  __shared__ uint32_t s_gb[Bins];          // global offset of (digit, tile)
+  __shared__ uint32_t s_ls[Bins];          // tile-local start of each digit
+  __shared__ uint32_t s_adj[Bins];         // global position of (digit, tile) minus its tile-local start
+  __shared__ uint32_t s_k[kSortTile];      // the tile in digit order
+  __shared__ uint2 s_v[kSortTile];
   __shared__ uint32_t s_w[4];
   const uint32_t n = cnt->n_rec;
   const uint32_t nt = n_tiles_of(n);
@@ -994,6 +997,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    // per digit: the waves' bases within the digit, the digit's count in
+    // the tile, then the tile-local digit starts (a scan over the digits)
+    uint32_t lstart[kPerThread];
+    uint32_t tsum = 0;
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
       const uint32_t dd = (uint32_t)(tx * kPerThread + q);
@@ -1004,18 +1011,37 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
         s_wc[k][dd] = run;
         run += c;
       }
-      s_gb[dd] = digit_base[q] + toff[q];
+      lstart[q] = tsum;
+      tsum += run;
+    }
+    const uint32_t lb = block_excl_scan_256(tsum, s_w, nullptr);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const uint32_t dd = (uint32_t)(tx * kPerThread + q);
+      s_ls[dd] = lb + lstart[q];
+      s_adj[dd] = digit_base[q] + toff[q] - (lb + lstart[q]);  // >= 0: earlier digits' keys all precede
     }
     __syncthreads();
+    // the tile in digit order in LDS (equal digits keep their order), then
+    // written out with consecutive threads on consecutive positions of each
+    // digit's output segment
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
       const uint32_t idx = wbase + (uint32_t)(i * 64 + lane);
       if (idx < n) {
         const uint32_t d = (kk[i] >> shift) & (Bins - 1u);
-        const uint32_t pos = s_gb[d] + s_wc[w][d] + off[i];
-        kout[pos] = kk[i];
-        vout[pos] = vv[i];
+        const uint32_t lp = s_ls[d] + s_wc[w][d] + off[i];
+        s_k[lp] = kk[i];
+        s_v[lp] = vv[i];
       }
+    }
+    __syncthreads();
+    const uint32_t tcount = min(kSortTile, n - tile * kSortTile);
+    for (uint32_t j = (uint32_t)tx; j < tcount; j += kSortWG) {
+      const uint32_t key = s_k[j];
+      const uint32_t pos = s_adj[(key >> shift) & (Bins - 1u)] + j;
+      kout[pos] = key;
+      vout[pos] = s_v[j];
     }
     __syncthreads();
   }
