@@ -742,8 +742,21 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
   const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
   const uint32_t per = (nblk + 1023) / 1024;
   const uint32_t b0 = min(threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
+  // up to kHeld counts per thread are loaded together into registers (launches
+  // of up to 2^22 lanes); longer ranges loop
+  constexpr uint32_t kHeld = 16;
+  const bool held = per <= kHeld;  // workgroup-uniform
+  uint32_t hv[kHeld];
   uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; ++b) sum += sc.blk_cnt[b];
+  if (held) {
+#pragma unroll
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      hv[k] = b0 + k < b1 ? sc.blk_cnt[b0 + k] : 0u;
+      sum += hv[k];
+    }
+  } else {
+    for (uint32_t b = b0; b < b1; ++b) sum += sc.blk_cnt[b];
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t incl = sum;
 #pragma unroll
@@ -760,10 +773,18 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
     tot += x;
   }
   uint32_t run = before + incl - sum;
-  for (uint32_t b = b0; b < b1; ++b) {
-    const uint32_t c = sc.blk_cnt[b];
-    sc.blk_cnt[b] = run;
-    run += c;
+  if (held) {
+#pragma unroll
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      if (b0 + k < b1) sc.blk_cnt[b0 + k] = run;
+      run += hv[k];
+    }
+  } else {
+    for (uint32_t b = b0; b < b1; ++b) {
+      const uint32_t c = sc.blk_cnt[b];
+      sc.blk_cnt[b] = run;
+      run += c;
+    }
   }
   if (threadIdx.x == 0) {
     sc.cnt->n_rec = tot;
