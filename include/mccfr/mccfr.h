@@ -240,8 +240,8 @@ int mccfr_counters(const mccfr_ctx* ctx, uint64_t* traversals, uint64_t* launche
 
 /* Kernel timing of the last `n` launches issued with timing on (ms, device
  * time between HIP events recorded on the context stream): deal (k_deal),
- * walk (k_walk) and table update (record emission, sort and reduce).  Any
- * output pointer may be NULL. */
+ * walk (k_walk) and table update (showdown payoffs, record emission, sort and
+ * reduce).  Any output pointer may be NULL. */
 int mccfr_set_timing(mccfr_ctx* ctx, int enabled);
 int mccfr_kernel_times(mccfr_ctx* ctx, float* deal_ms, float* walk_ms, float* update_ms, uint32_t n);
 

@@ -95,9 +95,11 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 // One launch (traverse.hip): deal and walk of every segment's traversals
 // (launch_walk), then records -> sort -> owner-computes table update
 // (launch_reduce).  `trace` (device, indexed by lane) may be null.
+// after_deal / after_walk (may be null): recorded after k_deal and after k_walk
+// (before k_showdown), for per-kernel timing.
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
-                 hipEvent_t after_deal = nullptr);
+                 hipEvent_t after_deal = nullptr, hipEvent_t after_walk = nullptr);
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits);
 

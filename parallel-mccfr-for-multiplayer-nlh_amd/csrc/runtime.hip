@@ -441,8 +441,7 @@ int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* hos
   if ((rc = stage_segments(c, p.segs))) return rc;
   if ((rc = record_event(c, 0))) return rc;
   launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape,
-              host_trace ? c->dtrace : nullptr, timing_event(c, 1));
-  if ((rc = record_event(c, 2))) return rc;
+              host_trace ? c->dtrace : nullptr, timing_event(c, 1), timing_event(c, 2));
   launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
   if ((rc = record_event(c, 3))) return rc;
   uint64_t total = 0;

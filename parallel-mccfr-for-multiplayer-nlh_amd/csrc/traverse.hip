@@ -1631,7 +1631,7 @@ void walk_dispatch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const
 
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
-                 hipEvent_t after_deal) {
+                 hipEvent_t after_deal, hipEvent_t after_walk) {
   hipLaunchKernelGGL(k_deal, dim3(lc.lanes / kDealWG), dim3(kDealWG), 0, s, lc, segs, tenants, in, sc);
   if (after_deal) (void)hipEventRecord(after_deal, s);
   DevScratch ws = sc;
@@ -1640,6 +1640,7 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
     walk_dispatch<8>(s, lc, segs, tenants, ws, shape, trace);
   else
     walk_dispatch<16>(s, lc, segs, tenants, ws, shape, trace);
+  if (after_walk) (void)hipEventRecord(after_walk, s);
   const dim3 grid((lc.lanes + 255) / 256);
   if (trace)
     hipLaunchKernelGGL(k_showdown<true>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
