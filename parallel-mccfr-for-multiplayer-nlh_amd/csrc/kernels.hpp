@@ -11,8 +11,9 @@ namespace mccfr {
 // written to the lane's node list in HBM when they are reached.
 constexpr int kRegPending = 4;
 
-// Radix sort of the launch's records: 10-bit digits (26-30-bit keys in three
-// passes), tiles of kSortWG threads x kSortItems keys.
+// Radix sort of the launch's records: ceil(key_bits / 10) passes with the key
+// bits spread evenly over them (26-bit keys: three 9-bit digits), tiles of
+// kSortWG threads x kSortItems keys.
 constexpr int kSortWG = 256;
 constexpr int kSortItems = 8;
 constexpr uint32_t kSortTile = kSortWG * kSortItems;

@@ -17,7 +17,8 @@
 //                 carries the record's action word and utility, so the table
 //                 update reads records in sorted order without gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
-//                 (key = segment << slot_bits | slot), 10-bit digits.
+//                 (key = segment << slot_bits | slot), the key bits spread
+//                 evenly over ceil(key_bits / 10) passes.
 //   k_runs        every run of equal keys is one table slot of one tenant;
 //                 runs are listed (k_reduce_cand finds the ends of the long
 //                 ones).  k_reduce_rows: one lane owns a short run and sums
@@ -1071,9 +1072,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // Owner-computes table update (cfr_gpu.cu:784-837 for the contributions,
 // :1018-1022 and :318-351 for clamp and re-matching).  Every run of equal
-// keys is one slot of one tenant with exactly one owner: a group of ROW lanes,
-// lane a holding action a's FP64 sums.  sigma comes from the pre-launch
-// regret row, which nothing has modified yet: only the owner writes a slot.
+// keys is one slot of one tenant with exactly one owner (a lane for a short
+// run, the wave of k_reduce_runs for a long one) holding the actions' FP64
+// sums.  sigma comes from the pre-launch regret row, which nothing has
+// modified yet: only the owner writes a slot.
 // ---------------------------------------------------------------------------
 // Finds the runs: one wave per 64 sorted records, head flags by ballot.  A
 // run that ends inside its wave or among the next kShortRun records, with at
