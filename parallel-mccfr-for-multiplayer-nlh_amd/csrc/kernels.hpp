@@ -46,8 +46,10 @@ struct DevScratch {
   // its short runs at runs[64 w ..] and the runs it defers at cand[4 w ..].
   uint4* runs;           // [max_waves][64] short runs: first record, records, key
   uint32_t* n_runs;      // [max_waves] short runs of each region
-  uint32_t* cand;        // [max_waves][4] first records of deferred runs
+  uint2* cand;           // [max_waves][4] deferred runs: first record, end (0: past the region)
   uint32_t* n_cand;      // [max_waves] deferred runs of each region
+  uint8_t* head0;        // [max_waves + kHeadPad] lane of each region's first run head (64: none)
+  uint8_t* head1;        // [head1_bytes] per 64 regions: 0 if one of them holds a run head, else 64
   SegCtx* seg_ctx;       // [kMaxSegs] table pointers and weights of the launch's segments
   uint4* long_runs;      // [max_long] long runs: first record, records, first piece, pieces
   uint4* piece;          // [max_pieces] pieces of long runs: first record, end, key, long run
@@ -59,6 +61,12 @@ struct DevScratch {
 constexpr uint32_t kEmitLanes = 256;   // lanes per emission block (one per thread)
 constexpr uint32_t kShortRun = 16;     // longer runs are split into pieces reduced by whole workgroups
 constexpr uint32_t kPiece = 256;       // records per piece of a long run (one wave each)
+constexpr uint32_t kHeadPad = 128;     // head0 / head1 bytes past the last entry (whole 64-byte reads)
+// Size of head1 for `cap_lanes` lanes, in whole dwords.
+__host__ __device__ inline uint32_t head1_bytes(uint32_t cap_lanes) {
+  const uint64_t regions = (uint64_t)cap_lanes * kMaxDepth / 64 + 1;
+  return (uint32_t)((regions / 64 + 1 + kHeadPad + 3) & ~3ull);
+}
 
 // Tables of the arithmetic 7-card evaluator (built on the host).
 constexpr int kNonFlushClasses = 49205;  // rank multisets of 7 cards, <= 4 per rank

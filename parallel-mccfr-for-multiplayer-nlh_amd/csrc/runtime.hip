@@ -87,8 +87,10 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   const size_t max_waves = recs / 64 + 1;
   if (e == hipSuccess) e = hipMalloc(&s.runs, max_waves * 64 * sizeof(uint4));
   if (e == hipSuccess) e = hipMalloc(&s.n_runs, max_waves * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.cand, max_waves * 4 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.cand, max_waves * 4 * sizeof(uint2));
   if (e == hipSuccess) e = hipMalloc(&s.n_cand, max_waves * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s.head0, max_waves + kHeadPad);
+  if (e == hipSuccess) e = hipMalloc(&s.head1, head1_bytes(cap));
   if (e == hipSuccess) e = hipMalloc(&s.seg_ctx, (size_t)kMaxSegs * sizeof(SegCtx));
   if (e == hipSuccess) e = hipMalloc(&s.long_runs, max_long * sizeof(uint4));
   if (e == hipSuccess) e = hipMalloc(&s.piece, max_pieces * sizeof(uint4));
@@ -100,7 +102,7 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
 
 void free_scratch(DevScratch& s) {
   void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_tidx, s.sd_inv, s.sd_state, s.root_dealer,
-                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.n_cand,
+                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.n_cand, s.head0, s.head1,
                   s.runs, s.n_runs,
                   s.seg_ctx, s.long_runs,
                   s.piece, s.partial, s.cnt};

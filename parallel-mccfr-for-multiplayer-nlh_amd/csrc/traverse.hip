@@ -787,6 +787,9 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
       run += c;
     }
   }
+  // k_runs marks the 64-region groups that hold a run head
+  for (uint32_t i = threadIdx.x; i < head1_bytes(sc.cap_lanes) / 4; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(sc.head1)[i] = 0x40404040u;
   if (threadIdx.x == 0) {
     sc.cnt->n_rec = tot;
     sc.cnt->n_long = 0;
@@ -1111,20 +1114,59 @@ __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __r
     const uint64_t sm = __ballot(is_short);
     const uint64_t dm = __ballot(defer);
     if (is_short) sc.runs[(size_t)w * 64 + (uint32_t)__popcll(sm & lt)] = make_uint4(i, e - i, key, 0u);
-    if (defer) sc.cand[(size_t)w * 4 + (uint32_t)__popcll(dm & lt)] = i;
+    if (defer) sc.cand[(size_t)w * 4 + (uint32_t)__popcll(dm & lt)] = make_uint2(i, closed ? e : 0u);
     if (lane == 0) {
       sc.n_runs[w] = (uint32_t)__popcll(sm);
       sc.n_cand[w] = (uint32_t)__popcll(dm);
+      sc.head0[w] = heads ? (uint8_t)__builtin_ctzll(heads) : (uint8_t)64;
+      if (heads) sc.head1[w >> 6] = 0;
     }
   }
 }
 
+// The first of the 64 bytes at base + a (a a multiple of 64) from index
+// `from` on that is below 64, an index at or past `lim` counting as one:
+// its offset from a (64 if none) and, in *val, the byte.
+__device__ __forceinline__ uint32_t first_head64(const uint8_t* __restrict__ base, uint32_t a, uint32_t from,
+                                                 uint32_t lim, uint32_t* val) {
+  const uint4* p = reinterpret_cast<const uint4*>(base + a);
+  uint32_t wd[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 v = p[q];
+    wd[4 * q] = v.x;
+    wd[4 * q + 1] = v.y;
+    wd[4 * q + 2] = v.z;
+    wd[4 * q + 3] = v.w;
+  }
+  uint32_t hit = 64;
+  *val = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const uint32_t rb = a + 4u * (uint32_t)q;  // index of the word's byte 0
+    uint32_t hm = ~wd[q] & 0x40404040u;       // bytes 0..63 (64 has bit 6 set)
+    hm &= from > rb ? (from - rb >= 4u ? 0u : 0xFFFFFFFFu << (8u * (from - rb))) : 0xFFFFFFFFu;
+    hm |= lim > rb ? (lim - rb >= 4u ? 0u : 0x40404040u & (0xFFFFFFFFu << (8u * (lim - rb)))) : 0x40404040u;
+    if (hit == 64u && hm != 0u) {
+      const uint32_t k = (uint32_t)__builtin_ctz(hm) >> 3;
+      hit = 4u * (uint32_t)q + k;
+      *val = (wd[q] >> (8u * k)) & 0xFFu;
+    }
+  }
+  return hit;
+}
+
 // One thread per wave region's deferred runs (k_runs defers only runs of more
-// than kShortRun records): each run's end by a forward scan of kShortRun keys,
-// then exponential and binary search (the keys ascend); the run is cut into
-// kPiece-record pieces.  A wave reserves the list space of all its long runs
-// and pieces with one 64-bit atomic (the long-run and piece counters share a
-// word).
+// than kShortRun records, and passes on the end of those that end within the
+// region or the next kShortRun records).  Any other deferred run ends at the
+// first run head of a later region, or at the end of the records: found in
+// the region's group of 64 regions (head0, one byte each, one 64-byte read),
+// else through the first later group that holds a head (head1, one byte per
+// group) and that group's regions.  Each run is cut into kPiece-record
+// pieces.  A wave reserves the list space of all its long runs and pieces
+// with one 64-bit atomic (the long-run and piece counters share a word); a
+// lane writes the pieces of its runs of up to 4 pieces, the wave those of
+// longer runs.
 __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const uint32_t n_waves = (n + 63) / 64;
@@ -1133,37 +1175,51 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
   for (uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w0 < n_waves; w0 += stride) {
     const uint32_t w = w0 + (uint32_t)lane;
     const uint32_t nc = w < n_waves ? sc.n_cand[w] : 0u;
-    uint32_t lstart[4], llen[4];  // long runs of the region (llen 0: none)
+    // the region's 4 candidate slots, loaded beside their count (slots past
+    // the count are not used)
+    uint2 cds[4];
+    if (w < n_waves) {
+      const uint4* cp = reinterpret_cast<const uint4*>(sc.cand + (size_t)w * 4);
+      const uint4 c01 = cp[0], c23 = cp[1];
+      cds[0] = make_uint2(c01.x, c01.y);
+      cds[1] = make_uint2(c01.z, c01.w);
+      cds[2] = make_uint2(c23.x, c23.y);
+      cds[3] = make_uint2(c23.z, c23.w);
+    }
+    uint32_t lstart[4], llen[4], lkey[4];  // long runs of the region (llen 0: none)
     uint32_t nl = 0, np = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       llen[c] = 0;
       lstart[c] = 0;
+      lkey[c] = 0;
       if ((uint32_t)c >= nc) continue;
-      const uint32_t i = sc.cand[(size_t)w * 4 + c];
-      const uint32_t key = keys[i];
-      // the next kShortRun keys, loaded together: e = first index whose key differs
-      uint32_t e = i + 1 + kShortRun;
-#pragma unroll
-      for (int k = (int)kShortRun; k >= 1; --k) {
-        const uint32_t idx = i + (uint32_t)k;
-        if (idx >= n || keys[idx] != key) e = idx;
-      }
-      if (e - i > kShortRun) {
-        e = i + kShortRun + 1;  // keys[i .. i + kShortRun] all equal key
-        // keys[e - 1] == key: bracket the end, then bisect
-        uint32_t lo = e, step = 32;
-        while (lo + step < n && keys[lo + step - 1] == key) {
-          lo += step;
-          step <<= 1;
+      const uint32_t i = cds[c].x;
+      uint32_t e = cds[c].y;
+      lkey[c] = keys[i];  // in flight during the search
+      if (e == 0) {
+        uint32_t v;
+        const uint32_t a = (w + 1) & ~63u;
+        const uint32_t h = first_head64(sc.head0, a, w + 1, n_waves, &v);
+        if (h < 64u) {
+          e = a + h >= n_waves ? n : (a + h) * 64u + v;
+        } else {
+          const uint32_t n_groups = (n_waves + 63) / 64;
+          uint32_t g = (a >> 6) + 1, ga, gh;
+          for (;;) {  // each pass moves g on by at least one; groups from n_groups on are hits
+            ga = g & ~63u;
+            gh = first_head64(sc.head1, ga, g, n_groups, &v);
+            if (gh < 64u) break;
+            g = ga + 64u;
+          }
+          const uint32_t grp = ga + gh;
+          if (grp >= n_groups) {
+            e = n;
+          } else {  // the group holds a head (or regions past the records)
+            const uint32_t h2 = first_head64(sc.head0, grp * 64u, grp * 64u, n_waves, &v);
+            e = grp * 64u + h2 >= n_waves ? n : (grp * 64u + h2) * 64u + v;
+          }
         }
-        uint32_t hi = min(lo + step, n);
-        while (lo < hi) {
-          const uint32_t mid = lo + ((hi - lo) >> 1);
-          if (keys[mid] == key) lo = mid + 1;
-          else hi = mid;
-        }
-        e = lo;
       }
       lstart[c] = i;
       llen[c] = e - i;
@@ -1184,16 +1240,33 @@ __global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32
     base = __shfl(base, 0, 64);
     const uint64_t excl = base + incl - mine;
     uint32_t r = (uint32_t)excl, p0 = (uint32_t)(excl >> 32);
+    uint32_t rc[4], pc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
+      rc[c] = r;
+      pc[c] = p0;
       if (llen[c] == 0) continue;
       const uint32_t i = lstart[c], end = i + llen[c], npc = (llen[c] + kPiece - 1) / kPiece;
-      const uint32_t key = keys[i];
       sc.long_runs[r] = make_uint4(i, llen[c], p0, npc);
-      for (uint32_t j = 0; j < npc; ++j)
-        sc.piece[p0 + j] = make_uint4(i + j * kPiece, min(end, i + (j + 1) * kPiece), key, r);
+      if (npc <= 4)
+        for (uint32_t j = 0; j < npc; ++j)
+          sc.piece[p0 + j] = make_uint4(i + j * kPiece, min(end, i + (j + 1) * kPiece), lkey[c], r);
       ++r;
       p0 += npc;
+    }
+    // the pieces of runs of more than 4 pieces, written by the whole wave
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint64_t m = __ballot(llen[c] > 4 * kPiece);
+      while (m) {
+        const int src = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t i = __shfl(lstart[c], src, 64), len = __shfl(llen[c], src, 64);
+        const uint32_t p = __shfl(pc[c], src, 64), rr = __shfl(rc[c], src, 64), key = __shfl(lkey[c], src, 64);
+        const uint32_t npc = (len + kPiece - 1) / kPiece;
+        for (uint32_t j = (uint32_t)lane; j < npc; j += 64)
+          sc.piece[p + j] = make_uint4(i + j * kPiece, min(i + len, i + (j + 1) * kPiece), key, rr);
+      }
     }
   }
 }
