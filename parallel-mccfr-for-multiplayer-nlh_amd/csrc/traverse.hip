@@ -687,9 +687,31 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
     }
   const bool up_in = !(folded >> up & 1u);
   float pay = -(float)inv_up;
+  // The update player wins exactly the layers above M, the largest investment
+  // of a live seat with a better rank (0 if none), up to its own investment;
+  // with no live seat of equal rank investing more than M, it wins each of
+  // them alone, and their pots add up to sum_p min(inv_p, inv_up) -
+  // min(inv_p, M).  Every share is then a whole number of chips below 2^24,
+  // so the loop's FP32 sums are exact and equal this one.  A tie takes the
+  // loop (split shares round).
+  int M = 0;
+  bool tie = false;
+#pragma unroll
+  for (int p = 0; p < kMaxPlayers; ++p) {
+    const bool live = !(folded >> p & 1u) && p != up;
+    M = (live && rkv[p] > rk_up && inv[p] > M) ? inv[p] : M;
+  }
+  int won = 0;
+#pragma unroll
+  for (int p = 0; p < kMaxPlayers; ++p) {
+    const bool live = !(folded >> p & 1u) && p != up;
+    tie = tie || (live && rkv[p] == rk_up && inv[p] > M);
+    won += min(inv[p], inv_up) - min(inv[p], M);
+  }
+  if (up_in && !tie && M < inv_up) pay += (float)won;
   int prev = 0;
 #pragma unroll
-  for (int it = 0; it < kMaxPlayers; ++it) {
+  for (int it = 0; it < kMaxPlayers && up_in && tie; ++it) {
     int lvl = 0x7FFFFFFF;
 #pragma unroll
     for (int p = 0; p < kMaxPlayers; ++p)
