@@ -246,7 +246,7 @@ int mccfr_set_timing(mccfr_ctx* ctx, int enabled);
 int mccfr_kernel_times(mccfr_ctx* ctx, float* deal_ms, float* walk_ms, float* update_ms, uint32_t n);
 
 /* Work of the last launch (waits for it): update-player records sorted,
- * runs (table slots) longer than 16 records, and their 1024-record pieces. */
+ * runs (table slots) longer than 16 records, and their 256-record pieces. */
 int mccfr_last_launch_stats(mccfr_ctx* ctx, uint32_t* records, uint32_t* long_runs, uint32_t* pieces);
 
 /* Device copy of `bytes` bytes between two fresh buffers, `reps` times

@@ -120,6 +120,7 @@ def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
             tid += n
         R0, S0, sig0 = ctx.download()
         tr = ctx.run_launch(0, 2, tid, n, trace=True)
+        recs, _, _ = ctx.last_launch_stats()
         R1, S1, _ = ctx.download()
         otr = oracle.replay_trace(cfg, hand_table, golden_lut, R0, sig0, 0, 2, tid, n)
         for f in ("path_hash", "decisions", "updates", "terminal"):
@@ -127,5 +128,6 @@ def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
         assert np.array_equal(tr["utility"].view(np.uint32), otr["utility"].view(np.uint32))
         mass = float(S1.astype(np.float64).sum() - S0.astype(np.float64).sum())
         nodes = int(tr["updates"].astype(np.int64).sum())
+        assert recs == nodes  # every update node emitted once
         assert abs(mass - 2.0 * nodes) <= 1e-4 * 2.0 * nodes
         assert (R1 >= 0).all()
