@@ -870,7 +870,7 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, one kDigitBits-bit digit per pass: tile histograms, a
+// Stable LSD radix sort, one digit of at most kDigitBits bits per pass: tile histograms, a
 // scan of the (digit, tile) counts, and a scatter that ranks each key within
 // its tile by wave-level digit matching (one ballot per digit bit) so equal
 // digits keep their order.
