@@ -73,6 +73,26 @@ def test_stepwise_parity(name, start, hand_table, golden_lut):
         assert (R >= 0).all()
 
 
+EDGE_CASES = {
+    # one traversal: a launch of 256 lanes with 255 padding lanes
+    "one_traversal": (lambda: mccfr.reference_cfg(num_players=3, starting_stack=400, table_log2=14, seed=31), 1, 3),
+    # a ragged count: a partial last wave and emission block
+    "ragged_257": (lambda: mccfr.cfg_6p_100bb(table_log2=16, seed=32), 257, 6),
+    # 16-slot tables: ~6,000 records per slot, so runs span several 4,096-record
+    # region groups, one 8-bit radix pass sorts the 4-bit keys, and the last run
+    # ends at the last record
+    "sixteen_slots": (lambda: mccfr.cfg_6p_100bb(table_log2=4, seed=33), 1 << 16, 6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_edge_shapes_parity(case, hand_table, golden_lut):
+    make, n, launches = EDGE_CASES[case]
+    cfg = make()
+    R, S, _ = run_stepwise(cfg, hand_table, golden_lut, n=n, launches=launches, random_seed=7)
+    assert np.count_nonzero(S) > 0
+
+
 def test_train_graph_equals_launch_sequence(hand_table, golden_lut):
     """mccfr_train (captured hipGraph, device-advanced launch state) gives the
     tables of the same launches issued one by one and checked against the oracle."""
