@@ -26,9 +26,9 @@
 //                         all-in 2-player game, chip conservation, and the
 //                         sum_a sigma(a)*rdelta(a) = 0 invariant.
 //   orc_cpu_baseline      same contributions as orc_replay_launch (test).
-// Rows marked "parity unpinned" in DESIGN.md: cuRAND streams and
-// __float2bfloat16 (replaced by the project's own Philox and RNE definitions),
-// eval5/eval6 against the reference's downloaded HandRanks.dat (absent).
+// Parity unpinned: cuRAND streams and __float2bfloat16 (replaced by the
+// project's own Philox and RNE definitions), and eval5/eval6 against the
+// reference's downloaded HandRanks.dat (absent here).
 #include "oracle.h"
 
 #include <algorithm>
