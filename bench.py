@@ -407,6 +407,17 @@ def main():
                           "per_traversal": {"decisions": float(tr["decisions"].mean()),
                                             "update_nodes": float(tr["updates"].mean()),
                                             "update_legal_actions": float(tr["update_legal"].mean())}}
+    # the walk's actual bound: vector issue, from the committed counter profile
+    # (VALU instructions per launch) over this run's walk time
+    ipath = os.path.join(ROOT, "profiles", "walk_6max_100bb_issue.json")
+    if os.path.exists(ipath):
+        with open(ipath) as f:
+            iss = json.load(f)
+        slots = float(walk_ms.mean()) * 1e-3 * iss["clock_hz"] * iss["simds"] / iss["cycles_per_instruction"]
+        result["roofline"]["issue"] = {"valu_instructions_per_launch": iss["valu_instructions_per_launch"],
+                                       "issue_frac": iss["valu_instructions_per_launch"] / slots,
+                                       "lanes_active_per_instruction": iss["lanes_active_per_instruction"],
+                                       "source": iss["source"]}
     result["launch"] = {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
                         "path_bytes_per_launch": path_bytes(tr, N, ctx.row),
                         "path_gbs": path_bytes(tr, N, ctx.row) / (elapsed / args.steps) / 1e9}
