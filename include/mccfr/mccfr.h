@@ -111,6 +111,17 @@ int mccfr_set_hand_table(mccfr_ctx* ctx, const int32_t* hr, size_t words);
 int mccfr_load_hand_table(mccfr_ctx* ctx, const char* path);
 int mccfr_use_generated_hand_table(mccfr_ctx* ctx);
 
+/* Host-only hand-table checks (no context, no device), the ones
+ * mccfr_set_hand_table and mccfr_load_hand_table apply before any device work.
+ * The reference's loader checks the size and the royal flush
+ * (hand_eval.cpp:59-86, cfr_gpu.cu:270-299); these also reject a table with
+ * any word outside [0, words - 53], so that no walk over an accepted table
+ * can read outside it.  The file form checks the file size before it
+ * allocates.  MCCFR_OK, MCCFR_E_TABLE (wrong size, word out of range, royal
+ * flush not 7462), MCCFR_E_IO (cannot open or read), MCCFR_E_ARG (null). */
+int mccfr_validate_hand_table(const int32_t* hr, size_t words);
+int mccfr_validate_hand_table_file(const char* path);
+
 /* Preflop bucket table (52*52 bytes, [h0*52 + h1]) the trainer uses
  * (cfr_gpu.cu:1164-1173 over abstraction.cpp:10-59). */
 int mccfr_preflop_lut(uint8_t* out_2704);
@@ -383,7 +394,11 @@ int mccfr_server_submit(mccfr_server* srv, const mccfr_game_cfg* cfg, const mccf
  * in flight.  Reports requests running and waiting. */
 int mccfr_server_step(mccfr_server* srv, int wait, uint32_t* running, uint32_t* waiting);
 
-/* state: 0 waiting, 1 running, 2 done (tables final), 3 released. */
+/* state: 0 waiting, 1 running, 2 done (tables final), 3 released, 4 failed
+ * (its launch no longer fits the context's launch scratch after
+ * mccfr_set_max_batch shrank it: the step that finds it returns MCCFR_E_STATE
+ * naming the request, and the request stops once its launches in flight have
+ * finished; release it to free its tables). */
 int mccfr_server_poll(mccfr_server* srv, uint64_t request, int* state, uint64_t* traversals);
 
 /* A done request's tenant (for mccfr_tenant_download and the like) and its
@@ -391,7 +406,7 @@ int mccfr_server_poll(mccfr_server* srv, uint64_t request, int* state, uint64_t*
 int mccfr_server_tenant(mccfr_server* srv, uint64_t request, uint32_t* tenant);
 int mccfr_server_result(mccfr_server* srv, uint64_t request, float* strategy, size_t elems, int layout);
 
-/* Frees a done request's tables. */
+/* Frees a done or failed request's tables. */
 int mccfr_server_release(mccfr_server* srv, uint64_t request);
 
 #ifdef __cplusplus

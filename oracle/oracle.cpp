@@ -21,10 +21,21 @@
 //   orc_preflop_lut, orc_valid_mask, orc_chips
 //       golden fixtures written by the reference's abstraction.cpp (oracle/_ref).
 //   orc_deal              permutation invariants (distinct cards) + uniformity.
-//   orc_replay_launch     first-launch closed form (all-zero tables: passive
-//                         fallback everywhere, no draws), brute force of an
-//                         all-in 2-player game, chip conservation, and the
-//                         sum_a sigma(a)*rdelta(a) = 0 invariant.
+//   orc_replay_launch, orc_walk_cards (the Walker: rules, apply, queue rebuild,
+//   raise cap, history, side pots, depth guard, update-node contributions)
+//       nine hand-worked forced-play sequences (tests/forced_play.py: every
+//       decision's player, history, legal actions, chips, pot, bet to match,
+//       last full raise, raise count and queue, the terminal utilities and the
+//       contributions written as literals; one-hot strategy rows force the
+//       play), covering open / re-raise / call queue order, a short all-in
+//       that keeps the last full raise, min-raises, the raise cap on the 4- and
+//       10-action menus, 3- and 4-bit history and its 30-bit mask, a three-way
+//       all-in with side pots, heads-up order, fold-around and the depth guard;
+//       brute force of the 2-player games with stacks <= the big blind (all-in
+//       from the blinds; {fold, all-in} for the small blind) against the
+//       brute-force evaluator; the first-launch closed form (all-zero tables:
+//       passive fallback everywhere, no draws); chip conservation; the
+//       sum_a sigma(a)*rdelta(a) = 0 invariant.
 //   orc_cpu_baseline      same contributions as orc_replay_launch (test).
 // Parity unpinned: cuRAND streams and __float2bfloat16 (replaced by the
 // project's own Philox and RNE definitions), and eval5/eval6 against the
@@ -458,6 +469,27 @@ struct Walker {
   float tw;  // strategy-sum weight (float)iteration, cfr_gpu.cu:828
   uint32_t path_hash = 2166136261u;
   int decisions = 0, updates = 0, terminal = 0, update_legal = 0;
+  std::vector<orc_decision>* log = nullptr;  // decision log (orc_walk_cards)
+
+  void log_decision(const Game& g, int player, int a, uint32_t vm, uint32_t slot, uint32_t hist, int pot_before) {
+    if (!log) return;
+    orc_decision d{};
+    d.player = (uint8_t)player;
+    d.street = (uint8_t)g.street;
+    d.action = (uint8_t)a;
+    d.is_update = player == up;
+    d.legal = vm;
+    d.history = hist;
+    d.slot = slot;
+    d.chips = g.pot - pot_before;
+    d.pot = g.pot;
+    d.current_bet = g.current_bet;
+    d.lfr = g.lfr;
+    d.raises = g.raises;
+    d.n_queue = g.n_to_act;
+    for (int i = 0; i < g.n_to_act; ++i) d.queue |= (uint32_t)g.to_act[i] << (4 * i);
+    log->push_back(d);
+  }
 
   void hash_step(uint32_t slot, int a) {
     for (int i = 0; i < 4; ++i) path_hash = (path_hash ^ ((slot >> (8 * i)) & 0xFFu)) * 16777619u;
@@ -606,9 +638,12 @@ struct Walker {
       const int sampled = sample(slot, vm);
       const float q = std::fmax(strat[sampled], 0.05f);
       hash_step(slot, sampled);
+      const uint32_t hist = g.ab;
+      const int pot_before = g.pot;
       --g.n_to_act;
       for (int i = 0; i < g.n_to_act; ++i) g.to_act[i] = g.to_act[i + 1];
       if (apply(g, player, sampled)) rebuild_queue(g, player);
+      log_decision(g, player, sampled, vm, slot, hist, pot_before);
       const float child_ev = traverse(g, depth + 1);
       const float ev = child_ev;
       for (int a = 0; a < A; ++a) {
@@ -620,9 +655,12 @@ struct Walker {
     }
     const int chosen = sample(slot, vm);
     hash_step(slot, chosen);
+    const uint32_t hist = g.ab;
+    const int pot_before = g.pot;
     --g.n_to_act;
     for (int i = 0; i < g.n_to_act; ++i) g.to_act[i] = g.to_act[i + 1];
     if (apply(g, player, chosen)) rebuild_queue(g, player);
+    log_decision(g, player, chosen, vm, slot, hist, pot_before);
     return traverse(g, depth + 1);
   }
 };
@@ -655,11 +693,22 @@ void deal_cards(const mccfr_game_cfg& cfg, const mccfr_root_state* root, Stream&
 // postflop root: stacks and investments as given, seats outside active_mask
 // folded, no bets, queue from the seat after the dealer (as at a street
 // change, cfr_gpu.cu:676-686).
+void setup_dealt(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                 int preflop_dealer, Game& g);
+
 void setup_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
                 Stream& rng, Game& g) {
+  deal_cards(cfg, root, rng, g);
+  // draw 2N+5 of the preflop root (cfr_gpu.cu:963); a postflop root names its dealer
+  const int dealer = (root && root->street > 0) ? root->dealer : (int)(rng.next() % (uint32_t)cfg.num_players);
+  setup_dealt(cfg, root, hr, lut, dealer, g);
+}
+
+// Buckets, stacks, blinds and the first queue of a dealt hand (g.hole, g.comm set).
+void setup_dealt(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                 int preflop_dealer, Game& g) {
   const int N = cfg.num_players;
   g.N = N;
-  deal_cards(cfg, root, rng, g);
   for (int p = 0; p < N; ++p) g.hb[p] = lut[g.hole[2 * p] * 52 + g.hole[2 * p + 1]];
   for (int p = 0; p < N; ++p) {
     g.bbk[p][0] = 0;
@@ -703,7 +752,7 @@ void setup_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, const i
   }
   g.folded = 0; g.all_in = 0; g.pot = 0;
   g.ab = 0; g.street = 0;
-  g.dealer = (int)(rng.next() % (uint32_t)N);
+  g.dealer = preflop_dealer;
   const int sb_pos = (N == 2) ? g.dealer : (g.dealer + 1) % N;
   const int bb_pos = (N == 2) ? (g.dealer + 1) % N : (g.dealer + 2) % N;
   const int sb_amt = std::min(cfg.small_blind, g.stacks[sb_pos]);
@@ -1331,6 +1380,66 @@ int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* ro
     SinkNull sink;
     for (uint32_t i = 0; i < n; ++i)
       run_traversal(*cfg, root, hr, lut, T, sink, up, tw, first_tid + i, tag, trace + i);
+  }
+  return 0;
+}
+
+int orc_walk_cards(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                   const uint8_t* hole, const uint8_t* board, int dealer, const float* R, const uint16_t* sigma,
+                   int up, int64_t t_weight, uint64_t tid, uint32_t tag, mccfr_trace* trace, orc_decision* log,
+                   int log_cap, int* n_log, uint32_t* c_slot, uint32_t* c_action, float* c_rd, float* c_sd,
+                   int c_cap, int* n_contrib) {
+  if (!cfg_ok(cfg) || !hr || !lut || !R || !hole || !board) return -1;
+  const int N = cfg->num_players;
+  if (up < 0 || up >= N || dealer < 0 || dealer >= N) return -1;
+  const int row = row_width(*cfg);
+  const uint64_t slots = 1ull << cfg->table_log2;
+  std::vector<uint16_t> own;
+  if (!sigma) {
+    own.resize(slots * row);
+    orc_regret_matching(R, own.data(), slots, row);
+    sigma = own.data();
+  }
+  TablesAoS T{R, sigma, row};
+  Game g;
+  for (int i = 0; i < 2 * N; ++i) g.hole[i] = hole[i];
+  for (int k = 0; k < 5; ++k) g.comm[k] = board[k];
+  const bool postflop = root && root->street > 0;
+  setup_dealt(*cfg, root, hr, lut, postflop ? root->dealer : dealer, g);
+  // the walk's draws follow the deal's (2N + 5 - fixed cards, then the
+  // preflop dealer draw), as in a replayed launch
+  Stream rng(cfg->seed, tid, tag);
+  const int n_draws = 2 * N + 5 - (postflop ? root->street + 2 : 0) + (postflop ? 0 : 1);
+  for (int i = 0; i < n_draws; ++i) (void)rng.next();
+  std::vector<Contribution> contrib;
+  SinkList sink{&contrib};
+  std::vector<orc_decision> dec;
+  Walker<TablesAoS, SinkList> w{*cfg, hr, T, sink, rng, up, (float)t_weight};
+  w.log = &dec;
+  const float u = w.traverse(g, 0);
+  if (trace) {
+    trace->utility = u;
+    trace->path_hash = w.path_hash;
+    trace->decisions = (uint16_t)w.decisions;
+    trace->updates = (uint8_t)w.updates;
+    trace->terminal = (uint8_t)w.terminal;
+    trace->update_legal = (uint16_t)w.update_legal;
+    trace->reserved = 0;
+  }
+  if (n_log) *n_log = (int)dec.size();
+  if (n_contrib) *n_contrib = (int)contrib.size();
+  if (log) {
+    if ((int)dec.size() > log_cap) return -1;
+    std::copy(dec.begin(), dec.end(), log);
+  }
+  if (c_slot || c_action || c_rd || c_sd) {
+    if ((int)contrib.size() > c_cap) return -1;
+    for (size_t i = 0; i < contrib.size(); ++i) {
+      if (c_slot) c_slot[i] = contrib[i].slot;
+      if (c_action) c_action[i] = contrib[i].a;
+      if (c_rd) c_rd[i] = contrib[i].rd;
+      if (c_sd) c_sd[i] = contrib[i].sd;
+    }
   }
   return 0;
 }

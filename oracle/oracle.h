@@ -84,6 +84,41 @@ int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* ro
 /* Hole cards (2N) and full board (5) of traversal tid from a root. */
 void orc_deal_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, uint64_t tid, uint32_t tag,
                    uint8_t* hole_out, uint8_t* board_out);
+/* One decision node of a walk, as the walk saw it and left it (the state
+ * fields are those after the action was applied and the queue rebuilt). */
+typedef struct orc_decision {
+  uint8_t player;
+  uint8_t street;
+  uint8_t action;
+  uint8_t is_update;    /* 1 when player == the update player */
+  uint32_t legal;       /* legal-action mask at the node */
+  uint32_t history;     /* action-history bits before the action */
+  uint32_t slot;        /* info-set slot */
+  int32_t chips;        /* chips the action put in */
+  int32_t pot;          /* pot after the action */
+  int32_t current_bet;  /* street's bet to match after the action */
+  int32_t lfr;          /* last full raise after the action */
+  int32_t raises;       /* aggressive actions this street after the action */
+  int32_t n_queue;      /* seats left to act after the action ... */
+  uint32_t queue;       /* ... in order, 4 bits each from the low bits */
+} orc_decision;
+
+/* One traversal with given cards, dealer and tables (the walk of
+ * orc_replay_launch_root after its deal: cfr_gpu.cu:954-981 and 644-860).
+ * hole: 2N cards seat by seat; board: 5 cards (a postflop root's fixed cards
+ * must be its first street + 2); dealer: the preflop button (ignored for a
+ * postflop root, which names its own).  The sampling draws come from stream
+ * (cfg->seed, tid, tag) after the deal's draws, as in a replayed launch.
+ * trace: one record or NULL; log: up to log_cap decisions, *n_log their
+ * number; contributions (slot, action, regret delta, strategy-sum delta) of
+ * the update player's nodes: up to c_cap entries, *n_contrib their number.
+ * Returns 0, or -1 on a bad argument / a capacity exceeded. */
+int orc_walk_cards(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,
+                   const uint8_t* hole, const uint8_t* board, int dealer, const float* R, const uint16_t* sigma,
+                   int update_player, int64_t t_weight, uint64_t tid, uint32_t tag, mccfr_trace* trace,
+                   orc_decision* log, int log_cap, int* n_log, uint32_t* c_slot, uint32_t* c_action, float* c_rd,
+                   float* c_sd, int c_cap, int* n_contrib);
+
 /* Regret matching of slot-major rows (kernel_regret_matching, cfr_gpu.cu:318-351). */
 void orc_regret_matching(const float* R, uint16_t* sigma, uint64_t slots, int row);
 

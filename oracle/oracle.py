@@ -51,6 +51,8 @@ _SIGS = {
     "orc_eval_scripted": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
     "orc_eval_scripted_action": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int, C.c_uint64, C.c_int64, _P]),
     "orc_eval_buckets": (C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
+    "orc_walk_cards": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32,
+                                 _P, _P, C.c_int, C.POINTER(C.c_int), _P, _P, _P, _P, C.c_int, C.POINTER(C.c_int)]),
     "orc_cpu_baseline": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_uint32, C.c_int, C.c_int64,
                                       C.c_uint64, C.c_int]),
 }
@@ -124,6 +126,35 @@ def replay_launch(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, tra
     if rc != 0:
         raise RuntimeError("orc_replay_launch_root rejected its arguments")
     return dR, dS, aR, aS, tr
+
+
+DECISION_DTYPE = np.dtype([("player", "u1"), ("street", "u1"), ("action", "u1"), ("is_update", "u1"),
+                           ("legal", "<u4"), ("history", "<u4"), ("slot", "<u4"), ("chips", "<i4"), ("pot", "<i4"),
+                           ("current_bet", "<i4"), ("lfr", "<i4"), ("raises", "<i4"), ("n_queue", "<i4"),
+                           ("queue", "<u4")])
+
+
+def walk_cards(cfg, hr, lut, hole, board, dealer, R, sigma, up, t_weight=1, tid=0, tag=0, root=None, cap=64):
+    """One traversal with given cards and dealer: (trace record, decision log,
+    contributions as (slot, action, regret delta, strategy-sum delta) arrays)."""
+    hole = np.ascontiguousarray(hole, np.uint8)
+    board = np.ascontiguousarray(board, np.uint8)
+    R = np.ascontiguousarray(R, np.float32)
+    sig_p = None if sigma is None else p(np.ascontiguousarray(sigma, np.uint16))
+    tr = np.zeros(1, dtype=TRACE_DTYPE)
+    log = np.zeros(cap, dtype=DECISION_DTYPE)
+    cs = np.zeros(cap * 16, np.uint32)
+    ca = np.zeros(cap * 16, np.uint32)
+    crd = np.zeros(cap * 16, np.float32)
+    csd = np.zeros(cap * 16, np.float32)
+    n_log, n_c = C.c_int(), C.c_int()
+    rc = lib().orc_walk_cards(C.byref(cfg), _root_ptr(root), p(hr), p(lut), p(hole), p(board), dealer, p(R), sig_p,
+                              up, t_weight, tid, tag, p(tr), p(log), cap, C.byref(n_log), p(cs), p(ca), p(crd),
+                              p(csd), cap * 16, C.byref(n_c))
+    if rc != 0:
+        raise RuntimeError("orc_walk_cards rejected its arguments")
+    k = n_c.value
+    return tr[0], log[: n_log.value], (cs[:k], ca[:k], crd[:k], csd[:k])
 
 
 def deal_root(cfg, root, tid, tag=0):

@@ -44,16 +44,66 @@ const std::vector<int32_t>* generated_table() {
   return tbl;
 }
 
-bool table_ok(const int32_t* hr, size_t words) {  // hand_eval.cpp:59-86
-  if (words != kHandTableWords) return false;
+// Hand-table check (hand_eval.cpp:59-86: size, then As Ks Qs Js Ts 2c 3d must
+// rank 7462), plus a range check the reference does not make: every word must
+// lie in [0, words - 53].  A walk reads hr[p + c] for card codes c in 1..52
+// with p the previous word, so with every word in range no walk over this table
+// -- the rank walks, the bucket builder's 5- and 6-card walks that decode a
+// state word, the harness's walks -- can read outside it, whatever its
+// contents.  Valid tables pass: state words are state bases (<= words - 53) and
+// rank words are below 0xA000.  On failure *why names the check.
+int table_check(const int32_t* hr, size_t words, const char** why) {
+  if (words != kHandTableWords) {
+    *why = "expected 32,487,834 words";
+    return MCCFR_E_TABLE;
+  }
+  const uint32_t hi = (uint32_t)(words - 53);
+  // one linear pass; an out-of-range word sets a bit of `bad`
+  uint32_t bad = 0;
+  for (size_t i = 0; i < words; ++i) bad |= (uint32_t)hr[i] > hi;
+  if (bad) {
+    *why = "a word lies outside [0, 32,487,781]";
+    return MCCFR_E_TABLE;
+  }
   const uint8_t royal[7] = {51, 50, 49, 48, 47, 0, 14};  // As Ks Qs Js Ts 2c 3d
   int p = hr[53 + card_rm(royal[0])];
-  for (int i = 1; i < 6; ++i) {
-    if (p < 0 || (size_t)p + 53 > words) return false;
-    p = hr[p + card_rm(royal[i])];
+  for (int i = 1; i < 6; ++i) p = hr[p + card_rm(royal[i])];
+  if (decode_packed((uint32_t)hr[p + card_rm(royal[6])] & 0xFFFFu) != 7462u) {
+    *why = "As Ks Qs Js Ts 2c 3d does not rank 7462";
+    return MCCFR_E_TABLE;
   }
-  if (p < 0 || (size_t)p + 53 > words) return false;
-  return decode_packed((uint32_t)hr[p + card_rm(royal[6])] & 0xFFFFu) == 7462u;
+  return MCCFR_OK;
+}
+
+// Reads a hand-table file into v after checking its size (no allocation for a
+// file of the wrong size), then checks its contents.
+int read_table_file(const char* path, std::vector<int32_t>& v, std::string& msg) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f.is_open()) {
+    msg = std::string("cannot open ") + path;
+    return MCCFR_E_IO;
+  }
+  const std::streamoff bytes = f.tellg();
+  if (bytes != (std::streamoff)(kHandTableWords * sizeof(int32_t))) {
+    msg = std::string("incompatible hand table ") + path + ": expected 129,951,336 bytes";
+    return MCCFR_E_TABLE;
+  }
+  f.seekg(0);
+  try {
+    v.resize(kHandTableWords);
+  } catch (...) {
+    msg = "no host memory for the hand table";
+    return MCCFR_E_NOMEM;
+  }
+  f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(int32_t)));
+  if (!f) {
+    msg = std::string("cannot read ") + path;
+    return MCCFR_E_IO;
+  }
+  const char* why = "";
+  const int rc = table_check(v.data(), v.size(), &why);
+  if (rc != MCCFR_OK) msg = std::string("incompatible hand table ") + path + ": " + why;
+  return rc;
 }
 
 uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
@@ -598,9 +648,8 @@ int mccfr_generate_hand_table(int32_t* hr, size_t words, int threads) {
 
 int mccfr_set_hand_table(mccfr_ctx* c, const int32_t* hr, size_t words) {
   if (!c || !hr) return fail(c, MCCFR_E_ARG, "null argument");
-  if (!table_ok(hr, words))
-    return fail(c, MCCFR_E_TABLE,
-                "incompatible hand table: expected 32,487,834 words with As Ks Qs Js Ts 2c 3d = 7462");
+  const char* why = "";
+  if (table_check(hr, words, &why) != MCCFR_OK) return fail(c, MCCFR_E_TABLE, std::string("incompatible hand table: ") + why);
   if (!c->hr) HIPX(c, hipMalloc(&c->hr, words * sizeof(int32_t)));
   HIPX(c, hipStreamSynchronize(c->stream));
   HIPX(c, hipMemcpy(c->hr, hr, words * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -613,14 +662,26 @@ int mccfr_set_hand_table(mccfr_ctx* c, const int32_t* hr, size_t words) {
 
 int mccfr_load_hand_table(mccfr_ctx* c, const char* path) {
   if (!c || !path) return fail(c, MCCFR_E_ARG, "null argument");
-  std::ifstream f(path, std::ios::binary | std::ios::ate);
-  if (!f.is_open()) return fail(c, MCCFR_E_IO, std::string("cannot open ") + path);
-  const size_t bytes = (size_t)f.tellg();
-  f.seekg(0);
-  std::vector<int32_t> v(bytes / sizeof(int32_t));
-  f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(int32_t)));
-  if (!f) return fail(c, MCCFR_E_IO, std::string("cannot read ") + path);
+  std::vector<int32_t> v;
+  std::string msg;
+  const int rc = read_table_file(path, v, msg);
+  if (rc != MCCFR_OK) return fail(c, rc, msg);
   return mccfr_set_hand_table(c, v.data(), v.size());
+}
+
+int mccfr_validate_hand_table(const int32_t* hr, size_t words) {
+  if (!hr) return fail(nullptr, MCCFR_E_ARG, "null argument");
+  const char* why = "";
+  const int rc = table_check(hr, words, &why);
+  return rc == MCCFR_OK ? rc : fail(nullptr, rc, std::string("incompatible hand table: ") + why);
+}
+
+int mccfr_validate_hand_table_file(const char* path) {
+  if (!path) return fail(nullptr, MCCFR_E_ARG, "null argument");
+  std::vector<int32_t> v;
+  std::string msg;
+  const int rc = read_table_file(path, v, msg);
+  return rc == MCCFR_OK ? rc : fail(nullptr, rc, msg);
 }
 
 int mccfr_use_generated_hand_table(mccfr_ctx* c) {

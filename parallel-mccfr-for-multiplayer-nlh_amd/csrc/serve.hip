@@ -70,7 +70,8 @@ struct Request {
   mccfr_root_state root{};
   int64_t iters = 0;
   uint32_t batch = 0;
-  int state = 0;  // 0 waiting, 1 running, 2 done, 3 released
+  int state = 0;  // 0 waiting, 1 running, 2 done, 3 released, 4 failed
+  bool failed = false;  // its launch no longer fits the context's launch scratch
   uint32_t tenant = 0;
   size_t off = 0;
   uint64_t bytes = 0;
@@ -133,7 +134,7 @@ int retire(mccfr_server* s, bool wait) {
     if (q != hipSuccess) return fail(s->ctx, MCCFR_E_HIP, std::string("server launch: ") + hipGetErrorString(q));
     for (uint64_t id : s->inflight.front().reqs) {
       Request& r = s->reqs[id];
-      if (++r.finished == r.total) r.state = 2;
+      if (++r.finished == r.total) r.state = r.failed ? 4 : 2;
     }
     s->spare.push_back(s->inflight.front().ev);
     s->inflight.pop_front();
@@ -167,20 +168,40 @@ int issue(mccfr_server* s) {
   std::vector<uint64_t> ids;
   uint64_t lanes = 0;
   int row = -1;
+  std::string failed;
   for (uint64_t id : s->running) {
-    const Request& r = s->reqs[id];
+    Request& r = s->reqs[id];
     if (r.issued >= r.total) continue;
     const Tenant& t = c->tenants[r.tenant];
-    if (row < 0) row = t.row;
-    if (t.row != row) continue;  // other row widths go in a later launch
     const HostSeg hs = launch_of(c, r);
     const uint64_t padded = (hs.n + kSegAlign - 1) / kSegAlign * kSegAlign;
+    if (padded > c->sc.cap_lanes) {
+      // the context's launch scratch shrank below this request's launch
+      // (mccfr_set_max_batch): it can never be issued, so it fails rather
+      // than wait for ever; launches already in flight finish first
+      r.failed = true;
+      r.total = r.issued;
+      if (r.finished == r.total) r.state = 4;
+      failed += (failed.empty() ? "" : ", ") + std::to_string(id);
+      continue;
+    }
+    if (row < 0) row = t.row;
+    if (t.row != row) continue;  // other row widths go in a later launch
     if (lanes + padded > c->sc.cap_lanes) continue;
     lanes += padded;
     segs.push_back(hs);
     ids.push_back(id);
   }
-  if (segs.empty()) return MCCFR_OK;
+  const auto report = [&](int rc) {
+    if (failed.empty()) return rc;
+    s->running.erase(std::remove_if(s->running.begin(), s->running.end(),
+                                    [&](uint64_t id) { return s->reqs[id].failed; }),
+                     s->running.end());
+    return rc ? rc
+              : fail(c, MCCFR_E_STATE,
+                     "request(s) " + failed + " failed: launch larger than the context's max_batch");
+  };
+  if (segs.empty()) return report(MCCFR_OK);
   hipEvent_t ev = nullptr;
   if (!s->spare.empty()) {
     ev = s->spare.back();
@@ -191,7 +212,7 @@ int issue(mccfr_server* s) {
   const int rc = issue_launch(c, segs.data(), (uint32_t)segs.size(), nullptr, ev);
   if (rc) {
     s->spare.push_back(ev);
-    return rc;
+    return report(rc);
   }
   for (size_t i = 0; i < ids.size(); ++i) {
     Request& r = s->reqs[ids[i]];
@@ -202,7 +223,7 @@ int issue(mccfr_server* s) {
   s->running.erase(std::remove_if(s->running.begin(), s->running.end(),
                                   [&](uint64_t id) { const Request& r = s->reqs[id]; return r.issued >= r.total; }),
                    s->running.end());
-  return MCCFR_OK;
+  return report(MCCFR_OK);
 }
 
 Request* find(mccfr_server* s, uint64_t id) {
@@ -242,7 +263,7 @@ void mccfr_server_destroy(mccfr_server* s) {
   }
   for (hipEvent_t e : s->spare) (void)hipEventDestroy(e);
   for (auto& kv : s->reqs)
-    if (kv.second.state == 1 || kv.second.state == 2) (void)tenant_destroy(s->ctx, kv.second.tenant);
+    if (kv.second.state == 1 || kv.second.state == 2 || kv.second.state == 4) (void)tenant_destroy(s->ctx, kv.second.tenant);
   (void)hipStreamSynchronize(s->ctx->stream);
   (void)hipFree(s->pool_mem);
   delete s;
@@ -312,7 +333,7 @@ int mccfr_server_result(mccfr_server* s, uint64_t id, float* strategy, size_t el
 int mccfr_server_release(mccfr_server* s, uint64_t id) {
   if (!s) return fail(nullptr, MCCFR_E_ARG, "null server");
   Request* r = find(s, id);
-  if (!r || r->state != 2) return fail(s->ctx, MCCFR_E_STATE, "request is not done");
+  if (!r || (r->state != 2 && r->state != 4)) return fail(s->ctx, MCCFR_E_STATE, "request is not done");
   const int rc = tenant_destroy(s->ctx, r->tenant);
   if (rc) return rc;
   s->pool.release(r->off, r->bytes);

@@ -138,6 +138,8 @@ _SIGS = {
     "mccfr_set_hand_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "mccfr_load_hand_table": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mccfr_use_generated_hand_table": (C.c_int, [C.c_void_p]),
+    "mccfr_validate_hand_table": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "mccfr_validate_hand_table_file": (C.c_int, [C.c_char_p]),
     "mccfr_preflop_lut": (C.c_int, [C.c_void_p]),
     "mccfr_eval7_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_eval7_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
@@ -289,6 +291,19 @@ def cfg_6p_200bb_10(table_log2=28, seed=0) -> GameCfg:
 # Hand table
 # ---------------------------------------------------------------------------
 _table_cache: Optional[np.ndarray] = None
+
+
+def validate_hand_table(hr: np.ndarray) -> int:
+    """Host-only check of a hand table (size, every word in [0, words - 53],
+    royal flush = 7462); returns the status code (0 = accepted)."""
+    hr = np.ascontiguousarray(hr, np.int32)
+    return lib().mccfr_validate_hand_table(_ptr(hr), hr.size)
+
+
+def validate_hand_table_file(path: str) -> int:
+    """Host-only check of a hand-table file (size before any allocation, then
+    the contents); returns the status code (0 = accepted)."""
+    return lib().mccfr_validate_hand_table_file(path.encode())
 
 
 def generate_hand_table(threads: int = 0) -> np.ndarray:
