@@ -145,6 +145,20 @@ int mccfr_eval7_host(mccfr_ctx* ctx, const uint8_t* cards, size_t stride, uint16
 int mccfr_run_launch(mccfr_ctx* ctx, int update_player, int64_t t_weight, uint64_t first_tid,
                      uint32_t n, mccfr_trace* trace);
 
+/* mccfr_run_launch without the trace, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The launch runs on a
+ * static device arena of the library (about 24 MB, held by each device the
+ * library runs on until the process ends) instead of the context's launch
+ * scratch, and calls no hipMalloc, hipFree or other allocating routine: n is
+ * 1..16384 (MCCFR_E_ARG beyond).  Same kernels, same results as
+ * mccfr_run_launch.  Every call on one device shares that device's arena, so
+ * calls that overlap on one device (contexts on different streams) race:
+ * serialize them.  The kernels initialise every arena word they read, so the
+ * previous call's contents need no clearing. */
+int mccfr_run_launch_noalloc(mccfr_ctx* ctx, int update_player, int64_t t_weight, uint64_t first_tid,
+                             uint32_t n);
+
 /* The reference training loop (GPUCFRTrainer::train, cfr_gpu.cu:1244-1378):
  * for iter = 1..iters, for p = 0..N-1, one launch of `batch` traversals,
  * replayed from a captured hipGraph.  Traversal ids continue from the

@@ -63,7 +63,7 @@ constexpr uint32_t kShortRun = 16;     // longer runs are split into pieces redu
 constexpr uint32_t kPiece = 256;       // records per piece of a long run (one wave each)
 constexpr uint32_t kHeadPad = 128;     // head0 / head1 bytes past the last entry (whole 64-byte reads)
 // Size of head1 for `cap_lanes` lanes, in whole dwords.
-__host__ __device__ inline uint32_t head1_bytes(uint32_t cap_lanes) {
+__host__ __device__ constexpr uint32_t head1_bytes(uint32_t cap_lanes) {
   const uint64_t regions = (uint64_t)cap_lanes * kMaxDepth / 64 + 1;
   return (uint32_t)((regions / 64 + 1 + kHeadPad + 3) & ~3ull);
 }

@@ -108,58 +108,83 @@ int read_table_file(const char* path, std::vector<int32_t>& v, std::string& msg)
 
 uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
-// Launch scratch for `cap` lanes (a multiple of kSegAlign).
-hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
-  std::memset(&s, 0, sizeof s);
+// Launch scratch for `cap` lanes (a multiple of kSegAlign): one arena holding
+// every buffer of DevScratch at 256-byte aligned offsets.  The byte counts of
+// the parts, in arena order (scratch_parts), size both the context's arena
+// (one hipMalloc) and the allocation-free entry point's static arena.
+constexpr int kScratchParts = 28;
+struct ScratchParts {
+  size_t b[kScratchParts];
+};
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr ScratchParts scratch_parts(uint32_t cap) {
   const size_t L = cap;
   const size_t recs = L * kMaxDepth;
-  s.cap_lanes = cap;
-  s.max_tiles = (uint32_t)((recs + kSortTile - 1) / kSortTile);
-  hipError_t e = hipMalloc(&s.node_slot, recs * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.node_av, recs * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.node_count, L);
-  if (e == hipSuccess) e = hipMalloc(&s.node_u, L * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(&s.root_bkt, (size_t)kMaxPlayers * L * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.root_rank, (size_t)kMaxPlayers * L * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMalloc(&s.root_tidx, (size_t)kMaxPlayers * L * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.sd_inv, (size_t)kMaxPlayers * L * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMalloc(&s.sd_state, L);
-  if (e == hipSuccess) e = hipMalloc(&s.root_dealer, L);
-  if (e == hipSuccess) e = hipMalloc(&s.blk_cnt, ((L + kEmitLanes - 1) / kEmitLanes) * sizeof(uint32_t));
-  for (int k = 0; k < 2; ++k) {
-    if (e == hipSuccess) e = hipMalloc(&s.key[k], recs * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&s.val[k], recs * sizeof(uint2));
-  }
-  if (e == hipSuccess) e = hipMalloc(&s.hist, (size_t)kSortBins * s.max_tiles * sizeof(uint32_t));
+  const size_t max_tiles = (recs + kSortTile - 1) / kSortTile;
   const size_t max_long = recs / (kShortRun + 1) + 1;
   const size_t max_pieces = recs / kPiece + max_long;
-  if (e == hipSuccess) e = hipMalloc(&s.bin_total, kSortBins * sizeof(uint32_t));
   const size_t max_waves = recs / 64 + 1;
-  if (e == hipSuccess) e = hipMalloc(&s.runs, max_waves * 64 * sizeof(uint4));
-  if (e == hipSuccess) e = hipMalloc(&s.n_runs, max_waves * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.cand, max_waves * 4 * sizeof(uint2));
-  if (e == hipSuccess) e = hipMalloc(&s.n_cand, max_waves * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&s.head0, max_waves + kHeadPad);
-  if (e == hipSuccess) e = hipMalloc(&s.head1, head1_bytes(cap));
-  if (e == hipSuccess) e = hipMalloc(&s.seg_ctx, (size_t)kMaxSegs * sizeof(SegCtx));
-  if (e == hipSuccess) e = hipMalloc(&s.long_runs, max_long * sizeof(uint4));
-  if (e == hipSuccess) e = hipMalloc(&s.piece, max_pieces * sizeof(uint4));
-  if (e == hipSuccess) e = hipMalloc(&s.partial, max_pieces * 32 * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&s.cnt, sizeof(LaunchCounters));
-  if (e == hipSuccess) e = hipMemset(s.cnt, 0, sizeof(LaunchCounters));
-  return e;
+  return ScratchParts{{recs * 4, recs * 4, L, L * 4,                              // node_slot node_av node_count node_u
+                       (size_t)kMaxPlayers * L * 4, (size_t)kMaxPlayers * L * 2,  // root_bkt root_rank
+                       (size_t)kMaxPlayers * L * 4, (size_t)kMaxPlayers * L * 2,  // root_tidx sd_inv
+                       L, L, ((L + kEmitLanes - 1) / kEmitLanes) * 4,            // sd_state root_dealer blk_cnt
+                       recs * 4, recs * 8, recs * 4, recs * 8,                    // key[0] val[0] key[1] val[1]
+                       (size_t)kSortBins * max_tiles * 4, (size_t)kSortBins * 4,  // hist bin_total
+                       max_waves * 64 * 16, max_waves * 4, max_waves * 4 * 8,     // runs n_runs cand
+                       max_waves * 4, max_waves + kHeadPad, head1_bytes(cap),     // n_cand head0 head1
+                       (size_t)kMaxSegs * sizeof(SegCtx), max_long * 16,          // seg_ctx long_runs
+                       max_pieces * 16, max_pieces * 32 * 8, sizeof(LaunchCounters)}};  // piece partial cnt
+}
+constexpr size_t scratch_bytes(uint32_t cap) {
+  const ScratchParts z = scratch_parts(cap);
+  size_t t = 0;
+  for (int i = 0; i < kScratchParts; ++i) t += align256(z.b[i]);
+  return t;
 }
 
-void free_scratch(DevScratch& s) {
-  void* ptrs[] = {s.node_slot, s.node_av, s.node_count, s.node_u, s.root_bkt, s.root_rank, s.root_tidx, s.sd_inv, s.sd_state, s.root_dealer,
-                  s.blk_cnt, s.key[0], s.key[1], s.val[0], s.val[1], s.hist, s.bin_total, s.cand, s.n_cand, s.head0, s.head1,
-                  s.runs, s.n_runs,
-                  s.seg_ctx, s.long_runs,
-                  s.piece, s.partial, s.cnt};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+// Points the buffers of `s` into the arena at `base` (scratch_bytes(cap) bytes).
+void carve_scratch(DevScratch& s, char* base, uint32_t cap) {
   std::memset(&s, 0, sizeof s);
+  const ScratchParts z = scratch_parts(cap);
+  char* p = base;
+  int i = 0;
+  auto take = [&](auto*& ptr) {
+    ptr = reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(p);
+    p += align256(z.b[i++]);
+  };
+  take(s.node_slot), take(s.node_av), take(s.node_count), take(s.node_u), take(s.root_bkt), take(s.root_rank);
+  take(s.root_tidx), take(s.sd_inv), take(s.sd_state), take(s.root_dealer), take(s.blk_cnt);
+  take(s.key[0]), take(s.val[0]), take(s.key[1]), take(s.val[1]), take(s.hist), take(s.bin_total);
+  take(s.runs), take(s.n_runs), take(s.cand), take(s.n_cand), take(s.head0), take(s.head1);
+  take(s.seg_ctx), take(s.long_runs), take(s.piece), take(s.partial), take(s.cnt);
+  s.cap_lanes = cap;
+  s.max_tiles = (uint32_t)(((size_t)cap * kMaxDepth + kSortTile - 1) / kSortTile);
 }
+
+hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
+  std::memset(&s, 0, sizeof s);
+  char* base = nullptr;
+  hipError_t e = hipMalloc(&base, scratch_bytes(cap));
+  if (e != hipSuccess) return e;
+  carve_scratch(s, base, cap);
+  return hipMemset(s.cnt, 0, sizeof(LaunchCounters));
+}
+
+// The arena starts at node_slot.
+void free_scratch(DevScratch& s) {
+  if (s.node_slot) (void)hipFree(s.node_slot);
+  s.node_slot = nullptr;
+}
+
+// Launch scratch of mccfr_run_launch_noalloc: a static arena for launches of
+// up to kNoAllocLanes traversals (scratch_bytes(16384) = about 24 MB), held by
+// every device the library runs on until the process ends.  In host code the
+// name `scratch` is a host-side stand-in, not a device address: the device
+// address is taken once per device (noalloc_scratch) and only that is used.
+constexpr uint32_t kNoAllocLanes = 16384;
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+char* g_noalloc_ptr[kMaxNoAllocDevices] = {};
 
 void drop_graph(mccfr_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -216,7 +241,7 @@ struct Prepared {
   int row = 8;
 };
 
-int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, Prepared& p) {
+int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, const DevScratch& sc, Prepared& p) {
   if (!c->hr) return fail(c, MCCFR_E_STATE, "no hand table loaded");
   if (n == 0 || n > kMaxSegs) return fail(c, MCCFR_E_ARG, "a launch needs 1..4096 segments");
   p.segs.resize(n);
@@ -232,9 +257,9 @@ int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, Prepared& p) {
     if (hs[i].update_player < 0 || hs[i].update_player >= t->cfg.num_players)
       return fail(c, MCCFR_E_ARG, "bad update player");
     const uint32_t padded = round_up(hs[i].n, kSegAlign);
-    if ((uint64_t)lanes + padded > c->sc.cap_lanes)
+    if ((uint64_t)lanes + padded > sc.cap_lanes)
       return fail(c, MCCFR_E_ARG,
-                  "launch needs more lanes than max_batch (segments are padded to 256; see mccfr_set_max_batch)");
+                  "launch needs more lanes than its scratch holds (segments are padded to 256; see mccfr_set_max_batch)");
     DevSeg& d = p.segs[i];
     std::memset(&d, 0, sizeof d);
     d.tenant = hs[i].tenant;
@@ -254,7 +279,7 @@ int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, Prepared& p) {
   p.lc.n_segs = n;
   p.lc.lanes = lanes;
   p.lc.slot_bits = slot_bits;
-  p.lc.cap_lanes = c->sc.cap_lanes;
+  p.lc.cap_lanes = sc.cap_lanes;
   p.key_bits = slot_bits + seg_bits;
   return MCCFR_OK;
 }
@@ -312,6 +337,9 @@ HostSeg next_of(uint32_t id, const Tenant& t, uint32_t n) {
 }
 
 }  // namespace
+
+// The allocation-free launch's arena (see kNoAllocLanes), at file scope.
+static __device__ __align__(256) unsigned char scratch[scratch_bytes(kNoAllocLanes)];
 
 namespace mccfr {
 
@@ -483,18 +511,21 @@ int tenant_destroy(mccfr_ctx* c, uint32_t id) {
   return MCCFR_OK;
 }
 
-int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done) {
+int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done,
+                 const DevScratch* scratch) {
+  const DevScratch& sc = scratch ? *scratch : c->sc;
+  if (scratch && host_trace) return fail(c, MCCFR_E_ARG, "no trace for a launch on outside scratch");
   Prepared p;
-  int rc = prepare(c, segs, n, p);
+  int rc = prepare(c, segs, n, sc, p);
   if (rc) return rc;
   for (uint32_t i = 0; i < n; ++i)
     if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;
   if (host_trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->sc.cap_lanes * sizeof(mccfr_trace)));
   if ((rc = stage_segments(c, p.segs))) return rc;
   if ((rc = record_event(c, 0))) return rc;
-  launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape,
+  launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), sc, c->shape,
               host_trace ? c->dtrace : nullptr, timing_event(c, 1), timing_event(c, 2));
-  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, sc, p.key_bits);
   if ((rc = record_event(c, 3))) return rc;
   uint64_t total = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -760,6 +791,48 @@ int mccfr_run_launch(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t
   return issue_launch(c, &s, 1, trace, nullptr);
 }
 
+// The static arena's device address on the context's device, taken with
+// hipGetSymbolAddress on the first call there (that device made current for
+// the call, the caller's current device restored after it).
+static int noalloc_scratch(mccfr_ctx* c, char** out) {
+  if (c->device < 0 || c->device >= kMaxNoAllocDevices)
+    return fail(c, MCCFR_E_STATE, "device index beyond the allocation-free launch's table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[c->device]) {
+    int cur = 0;
+    HIPX(c, hipGetDevice(&cur));
+    if (cur != c->device) HIPX(c, hipSetDevice(c->device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != c->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr)
+      return fail(c, MCCFR_E_HIP, std::string("static launch scratch: ") + hipGetErrorString(e));
+    g_noalloc_ptr[c->device] = static_cast<char*>(ptr);
+  }
+  *out = g_noalloc_ptr[c->device];
+  return MCCFR_OK;
+}
+
+int mccfr_run_launch_noalloc(mccfr_ctx* c, int update_player, int64_t t_weight, uint64_t first_tid, uint32_t n) {
+  if (!c) return fail(nullptr, MCCFR_E_ARG, "null context");
+  if (n == 0 || n > kNoAllocLanes)
+    return fail(c, MCCFR_E_ARG, "n must be 1..16384 for mccfr_run_launch_noalloc (its static scratch)");
+  char* base = nullptr;
+  const int rc = noalloc_scratch(c, &base);
+  if (rc) return rc;
+  DevScratch sc;
+  carve_scratch(sc, base, kNoAllocLanes);  // host arithmetic on the device address
+  HostSeg s{};
+  s.tenant = 0;
+  s.n = n;
+  s.update_player = update_player;
+  s.t_weight = t_weight;
+  s.first_tid = first_tid;
+  s.iteration = c->tenants[0].next_iter;
+  s.upd_idx = 0;
+  return issue_launch(c, &s, 1, nullptr, nullptr, &sc);
+}
+
 int mccfr_run_segments(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, mccfr_trace* trace) {
   if (!c || (!segs && n_segs)) return fail(c, MCCFR_E_ARG, "null argument");
   std::vector<HostSeg> hs(n_segs);
@@ -795,7 +868,7 @@ int mccfr_train_tenants(mccfr_ctx* c, const uint32_t* tenants, const uint32_t* c
   }
   {
     Prepared p;
-    if ((rc = prepare(c, hs.data(), n, p))) return rc;
+    if ((rc = prepare(c, hs.data(), n, c->sc, p))) return rc;
     for (uint32_t i = 0; i < n; ++i)
       if ((rc = rm_sweep_if_dirty(c, c->tenants[tenants[i]]))) return rc;
     std::vector<uint64_t> key = {(uint64_t)p.row,          (uint64_t)p.lc.lanes, (uint64_t)p.lc.n_segs,
@@ -956,7 +1029,7 @@ int mccfr_autotune(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, int
     hs[i].first_tid = segs[i].first_tid;
   }
   Prepared p;
-  int rc = prepare(c, hs.data(), n_segs, p);
+  int rc = prepare(c, hs.data(), n_segs, c->sc, p);
   if (rc) return rc;
   for (uint32_t i = 0; i < n_segs; ++i)
     if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;

@@ -106,7 +106,10 @@ int tenant_create(mccfr_ctx* c, const mccfr_game_cfg* cfg, const mccfr_root_stat
                   uint32_t* id);
 int tenant_destroy(mccfr_ctx* c, uint32_t id);
 // Issues one batched launch; `done` (may be null) is recorded after it.
-int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done);
+// `scratch`: launch scratch other than the context's (null = the context's);
+// such a launch takes no host trace.
+int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done,
+                 const DevScratch* scratch = nullptr);
 int tenant_normalized(mccfr_ctx* c, uint32_t id, float* out, size_t elems, int layout);
 
 }  // namespace mccfr

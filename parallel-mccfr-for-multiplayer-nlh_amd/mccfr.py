@@ -144,6 +144,7 @@ _SIGS = {
     "mccfr_eval7_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_eval7_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]),
     "mccfr_run_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "mccfr_run_launch_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_uint32]),
     "mccfr_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
     "mccfr_sync": (C.c_int, [C.c_void_p]),
     "mccfr_set_max_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -384,6 +385,10 @@ class Context:
         self.check(lib().mccfr_run_launch(self.h, update_player, t_weight, first_tid, n,
                                           _ptr(tr) if trace else None))
         return tr
+
+    def run_launch_noalloc(self, update_player: int, t_weight: int, first_tid: int, n: int):
+        """mccfr_run_launch on the library's static scratch (n <= 16384, no trace)."""
+        self.check(lib().mccfr_run_launch_noalloc(self.h, update_player, t_weight, first_tid, n))
 
     def train(self, iters: int, batch: int):
         self.check(lib().mccfr_train(self.h, iters, batch))
