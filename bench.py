@@ -22,7 +22,13 @@ The same JSON line carries:
                 traversals/s; and all requests at once, served vs one at a time;
   hand_eval     2^28 uniform 7-card hands through both evaluators;
   cpu_baseline  the oracle's FP32 restatement of the reference launch on a
-                bounded sample, on this host's cores.
+                bounded sample, on the process's share of the host cores and
+                on one thread, plus the reference's table walk over the hand
+                microbenchmark's first 2^24 hands on the host;
+  config1       the reference's CPU-runnable case (3 players, 20bb, 2^22
+                slots, launches of 2^10) on the CPU and through the GPU graph;
+  large_table   12 launches of 2^22 traversals against 2^28-slot 16-wide
+                tables (the large-table stress at size, fewer traversals).
 """
 import argparse
 import ctypes as C
@@ -92,23 +98,95 @@ def hand_bench(ctx, log2n, reps=3):
     return out
 
 
-def cpu_baseline(mccfr, ctx, cfg, launch, tid, batch, launches, threads):
+def host_cpus():
+    """(threads the CPU legs use, CPUs this process may run on).  The GPU box
+    gives a GPU's process a share of its host cores (OMP_NUM_THREADS there,
+    16); the legs use that share, capped by the CPUs visible to the process."""
+    visible = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(share, visible) if share > 0 else visible), visible
+
+
+def cpu_baseline(mccfr, ctx, cfg, launch, tid, batch, launches, threads, launches_1=2):
+    """The oracle's FP32 restatement of the reference launch on the GPU's own
+    table state: on `threads` threads over `launches` launches, and on one
+    thread over `launches_1` launches (bounded samples)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     O = oracle.lib()
-    R, S, _ = ctx.download(layout=1)  # same table state as the GPU, action-major like the reference
-    sig = np.zeros(R.size, np.uint16)
+    R0, S0, _ = ctx.download(layout=1)  # same table state as the GPU, action-major like the reference
     hr = mccfr.generate_hand_table()
     lut = mccfr.preflop_lut()
     N = cfg.num_players
-    secs = O.orc_cpu_baseline(C.byref(cfg), oracle.p(hr), oracle.p(lut), oracle.p(R), oracle.p(S), oracle.p(sig),
-                              launches, batch, launch % N, launch // N + 1, tid, threads)
-    if secs <= 0:
-        return None
-    return {"value": launches * batch / secs, "unit": "traversals/s", "cores": threads, "kind": "port",
-            "sample": f"{launches} launch(es) x {batch} traversals of the same workload from the GPU's "
-                      f"post-warm-up tables (FP32, action-major, full regret-matching and clamp sweeps), "
-                      f"{secs:.1f} s"}
+    out = {}
+    for key, th, nl in (("threads_all", threads, launches), ("threads_1", 1, launches_1)):
+        R, S = R0.copy(), S0.copy()
+        sig = np.zeros(R.size, np.uint16)
+        secs = O.orc_cpu_baseline(C.byref(cfg), oracle.p(hr), oracle.p(lut), oracle.p(R), oracle.p(S),
+                                  oracle.p(sig), nl, batch, launch % N, launch // N + 1, tid, th)
+        if secs <= 0:
+            return None
+        out[key] = {"traversals_per_s": nl * batch / secs, "threads": th, "launches": nl, "seconds": secs}
+    return out
+
+
+def hand_eval_cpu(mccfr, ctx, log2n, threads):
+    """The reference's table walk (hand_eval.cpp:134-143, the oracle's
+    restatement) on the host over the first 2^log2n hands of the GPU
+    microbenchmark's inputs (seed 1), on `threads` threads and on one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n = 1 << log2n
+    dc = ctx.dalloc(n * 8)
+    try:
+        ctx.random_hands_device(1, HAND_TAG, dc, n)
+        cards = np.empty(n * 8, np.uint8)
+        ctx.d2h(cards, dc)
+    finally:
+        ctx.dfree(dc)
+    hr = mccfr.generate_hand_table()
+    out = {"hands": n, "seed": 1}
+    for key, th in (("threads_all", threads), ("threads_1", 1)):
+        ranks = np.empty(n, np.uint16)
+        t0 = time.perf_counter()
+        oracle.lib().orc_walk7_batch(oracle.p(hr), oracle.p(cards), 8, oracle.p(ranks), n, th)
+        dt = time.perf_counter() - t0
+        out[key] = {"hands_per_s": n / dt, "threads": th}
+    return out
+
+
+def config1_bench(mccfr, hr, threads, cpu_launches_all=100, cpu_launches_1=30, gpu_iters=200):
+    """The reference's CPU-runnable case: 3 players, 20bb (400 chips), {F, C/C,
+    pot, all-in}, <= 2 raises per street, 2^22-slot tables, launches of 2^10
+    traversals from fresh (zero) tables and seed 0.  CPU: the oracle's
+    restatement (full sweeps per launch, as the reference) on a bounded sample
+    of the run's first launches; GPU: the captured training graph."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cfg = mccfr.cfg_3p_20bb(table_log2=22, seed=0)
+    B = 1 << 10
+    lut = mccfr.preflop_lut()
+    n = (1 << 22) * 8
+    out = {"workload": "config1: 3 players, 400-chip stacks, {F, C/C, pot, AI}, <=2 raises/street, 2^22 slots, "
+                       "batch 2^10, fresh tables, seed 0"}
+    for key, th, nl in (("cpu_threads_all", threads, cpu_launches_all), ("cpu_threads_1", 1, cpu_launches_1)):
+        R = np.zeros(n, np.float32)
+        S = np.zeros(n, np.float32)
+        sig = np.zeros(n, np.uint16)
+        secs = oracle.lib().orc_cpu_baseline(C.byref(cfg), oracle.p(hr), oracle.p(lut), oracle.p(R), oracle.p(S),
+                                             oracle.p(sig), nl, B, 0, 1, 0, th)
+        out[key] = {"traversals_per_s": nl * B / secs, "threads": th, "launches": nl}
+    with mccfr.Context(cfg, B) as ctx:
+        ctx.set_hand_table(hr)
+        ctx.train(2, B)  # capture + warm-up
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.train(gpu_iters, B)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+    out["gpu"] = {"traversals_per_s": gpu_iters * cfg.num_players * B / dt, "launches": gpu_iters * cfg.num_players,
+                  "ms_per_launch": dt * 1e3 / (gpu_iters * cfg.num_players)}
+    return out
 
 
 def harness_bench(mccfr, ctx, cfg, n_players=6, opponent=3, log2_hands=22, cpu_hands=100_000):
@@ -224,22 +302,21 @@ def serving_bench(mccfr, workloads, hr, n_req, sat_req, budget, max_running):
     return out
 
 
-def large_table_bench(args):
+def large_table_run(mccfr, table_log2, batch_log2, traversals, table_seed):
     """6 players, 200bb (4,000 chips), 10-action menu, <= 4 raises per street,
     2^28-slot tables (2^32 regret + strategy-sum entries, 32 GiB FP32, plus the
-    8 GiB BF16 snapshot), launches of 2^22 traversals, about 1e9 traversals
-    from the captured training graph."""
-    import mccfr
-    cfg = mccfr.cfg_6p_200bb_10(table_log2=args.table_log2 or 28, seed=0)
-    B = 1 << (args.batch_log2 or 22)
+    8 GiB BF16 snapshot), launches of 2^22 traversals from the captured
+    training graph; `traversals` in total (rounded to whole iterations)."""
+    cfg = mccfr.cfg_6p_200bb_10(table_log2=table_log2, seed=0)
+    B = 1 << batch_log2
     N = cfg.num_players
-    iters = max(1, int(round(args.large_traversals / (B * N))))
+    iters = max(1, int(round(traversals / (B * N))))
     ctx = mccfr.Context(cfg, B)
     ctx.use_generated_hand_table()
     slots, row = ctx.slots, ctx.row
     log(f"bench: large_table 2^{cfg.table_log2} slots x {row} actions, batch 2^{B.bit_length() - 1}, "
         f"{iters} iterations x {N} launches")
-    ctx.randomize_tables(args.table_seed)
+    ctx.randomize_tables(table_seed)
     ctx.set_timing(True)
     for L in range(2 * N):  # warm-up, timed per kernel
         ctx.run_launch(L % N, L // N + 1, L * B, B)
@@ -274,6 +351,13 @@ def large_table_bench(args):
             with open(path) as f:
                 result[name[len("large_table_"):-5]] = json.load(f)
     ctx.close()
+    return result
+
+
+def large_table_bench(args):
+    import mccfr
+    result = large_table_run(mccfr, args.table_log2 or 28, args.batch_log2 or 22, args.large_traversals,
+                             args.table_seed)
     print(json.dumps(result), flush=True)
 
 
@@ -288,7 +372,8 @@ def main():
     ap.add_argument("--batch-log2", type=int, default=None, help="default 20 (large_table: 22)")
     ap.add_argument("--table-log2", type=int, default=None, help="default 26 (large_table: 28)")
     ap.add_argument("--hand-log2", type=int, default=28)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cpus)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = the process's host-core share (OMP_NUM_THREADS), capped by the visible CPUs")
     ap.add_argument("--cpu-launches", type=int, default=24)
     ap.add_argument("--batched-launches", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -299,6 +384,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-harness", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the reduced large-table leg")
+    ap.add_argument("--large-leg-launches", type=int, default=12,
+                    help="launches of 2^22 traversals in the default run's large-table leg")
     ap.add_argument("--profile", action="store_true", help="headline launches only (for rocprofv3 runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="with --profile: also run a plain device copy of this size (byte-counter calibration)")
@@ -414,10 +502,13 @@ def main():
         with open(ipath) as f:
             iss = json.load(f)
         slots = float(walk_ms.mean()) * 1e-3 * iss["clock_hz"] * iss["simds"] / iss["cycles_per_instruction"]
-        result["roofline"]["issue"] = {"valu_instructions_per_launch": iss["valu_instructions_per_launch"],
-                                       "issue_frac": iss["valu_instructions_per_launch"] / slots,
-                                       "lanes_active_per_instruction": iss["lanes_active_per_instruction"],
-                                       "source": iss["source"]}
+        # `bound`/`frac` stay the HBM roofline the path is priced against;
+        # what limits the walk in fact is vector instruction issue
+        result["roofline"]["limiter"] = {"kind": "vector-issue",
+                                         "valu_instructions_per_launch": iss["valu_instructions_per_launch"],
+                                         "issue_frac": iss["valu_instructions_per_launch"] / slots,
+                                         "lanes_active_per_instruction": iss["lanes_active_per_instruction"],
+                                         "source": iss["source"]}
     result["launch"] = {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
                         "path_bytes_per_launch": path_bytes(tr, N, ctx.row),
                         "path_gbs": path_bytes(tr, N, ctx.row) / (elapsed / args.steps) / 1e9}
@@ -426,11 +517,22 @@ def main():
         log("bench: hand-evaluation microbenchmark")
         result["hand_eval"] = hand_bench(ctx, args.hand_log2)
 
+    threads, visible = host_cpus()
+    if args.cpu_threads:
+        threads = args.cpu_threads
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline on {threads} threads")
-        result["cpu_baseline"] = cpu_baseline(mccfr, ctx, cfg, state["launch"], state["tid"], B,
-                                              args.cpu_launches, threads)
+        log(f"bench: CPU baseline on {threads} threads and on 1 ({visible} CPUs visible)")
+        cb = cpu_baseline(mccfr, ctx, cfg, state["launch"], state["tid"], B, args.cpu_launches, threads)
+        if cb:
+            result["cpu_baseline"] = {
+                "value": cb["threads_all"]["traversals_per_s"], "unit": "traversals/s", "cores": threads,
+                "kind": "port", "cpus_visible": visible,
+                "sample": f"{cb['threads_all']['launches']} launch(es) x {B} traversals on {threads} threads, "
+                          f"{cb['threads_1']['launches']} on 1 thread, of the same workload from the GPU's "
+                          f"post-warm-up tables (FP32, action-major, full regret-matching and clamp sweeps)",
+                "threads_all": cb["threads_all"], "threads_1": cb["threads_1"]}
+        if cb and not args.no_hand_bench:
+            result["cpu_baseline"]["hand_eval_cpu"] = hand_eval_cpu(mccfr, ctx, 24, threads)
     if not args.no_harness:
         log("bench: strategy-quality harness")
         result["strategy_harness"] = harness_bench(mccfr, ctx, cfg)
@@ -444,6 +546,16 @@ def main():
         result["batched"] = batched_bench(mccfr, workloads, hr, args.batched_launches)
         log(f"bench: batched {result['batched']['batched_traversals_per_s'] / 1e6:.1f} M/s vs plain "
             f"{result['batched']['plain_traversals_per_s'] / 1e6:.1f} M/s")
+
+    if not args.no_cpu_baseline:
+        log("bench: config 1 (3 players, 20bb) on the CPU and the GPU")
+        result["config1"] = config1_bench(mccfr, hr, threads)
+
+    if not args.no_large:
+        log("bench: large table (reduced run)")
+        lt = large_table_run(mccfr, 28, 22, args.large_leg_launches * (1 << 22), args.table_seed)
+        result["large_table"] = {k: lt[k] for k in ("metric", "value", "unit", "steps", "ms_per_step", "config",
+                                                    "kernel_ms", "launch")}
 
     if not args.no_serving:
         log("bench: re-solve serving")

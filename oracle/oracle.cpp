@@ -48,6 +48,7 @@
 #include <chrono>
 #include <cmath>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <thread>
@@ -1382,6 +1383,73 @@ int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* ro
       run_traversal(*cfg, root, hr, lut, T, sink, up, tw, first_tid + i, tag, trace + i);
   }
   return 0;
+}
+
+int64_t orc_replay_sparse(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr,
+                          const uint8_t* lut, const float* R, const uint16_t* sigma, int up, int64_t t_weight,
+                          uint64_t first_tid, uint32_t n, uint32_t tag, int threads, mccfr_trace* trace,
+                          orc_sparse* out) {
+  if (!cfg_ok(cfg) || !hr || !lut || !R || !sigma || !out) return -1;
+  if (up < 0 || up >= cfg->num_players) return -1;
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int row = row_width(*cfg);
+  TablesAoS T{R, sigma, row};
+  const float tw = (float)t_weight;
+  std::vector<std::vector<Contribution>> lists(threads);
+  parallel_for(threads, (int64_t)n, [&](int th, int64_t b, int64_t e) {
+    SinkList sink{&lists[th]};
+    for (int64_t i = b; i < e; ++i)
+      run_traversal(*cfg, root, hr, lut, T, sink, up, tw, first_tid + (uint64_t)i, tag, trace ? trace + i : nullptr);
+  });
+  // concatenate in traversal order (thread t holds an ascending range), then a
+  // stable sort by table index keeps each entry's contributions in that order
+  std::vector<Contribution> all;
+  size_t total = 0;
+  for (auto& l : lists) total += l.size();
+  all.reserve(total);
+  for (auto& l : lists) {
+    all.insert(all.end(), l.begin(), l.end());
+    std::vector<Contribution>().swap(l);
+  }
+  auto idx_of = [row](const Contribution& c) { return (uint64_t)c.slot * (uint64_t)row + c.a; };
+  std::stable_sort(all.begin(), all.end(),
+                   [&](const Contribution& x, const Contribution& y) { return idx_of(x) < idx_of(y); });
+  size_t uniq = 0;
+  for (size_t i = 0; i < all.size(); ++i) uniq += (i == 0 || idx_of(all[i]) != idx_of(all[i - 1]));
+  out->n = (int64_t)uniq;
+  out->idx = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, uniq) * sizeof(uint64_t)));
+  double** cols[4] = {&out->dR, &out->dS, &out->aR, &out->aS};
+  for (double** c : cols) *c = static_cast<double*>(std::malloc(std::max<size_t>(1, uniq) * sizeof(double)));
+  if (!out->idx || !out->dR || !out->dS || !out->aR || !out->aS) {
+    orc_sparse_free(out);
+    return -1;
+  }
+  int64_t k = -1;
+  for (size_t i = 0; i < all.size(); ++i) {
+    const uint64_t x = idx_of(all[i]);
+    if (i == 0 || x != idx_of(all[i - 1])) {
+      ++k;
+      out->idx[k] = x;
+      out->dR[k] = out->dS[k] = out->aR[k] = out->aS[k] = 0.0;
+    }
+    out->dR[k] += (double)all[i].rd;
+    out->dS[k] += (double)all[i].sd;
+    out->aR[k] += std::fabs((double)all[i].rd);
+    out->aS[k] += std::fabs((double)all[i].sd);
+  }
+  return out->n;
+}
+
+void orc_sparse_free(orc_sparse* s) {
+  if (!s) return;
+  std::free(s->idx);
+  std::free(s->dR);
+  std::free(s->dS);
+  std::free(s->aR);
+  std::free(s->aS);
+  s->idx = nullptr;
+  s->dR = s->dS = s->aR = s->aS = nullptr;
+  s->n = 0;
 }
 
 int orc_walk_cards(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr, const uint8_t* lut,

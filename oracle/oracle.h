@@ -84,6 +84,23 @@ int orc_replay_launch_root(const mccfr_game_cfg* cfg, const mccfr_root_state* ro
 /* Hole cards (2N) and full board (5) of traversal tid from a root. */
 void orc_deal_root(const mccfr_game_cfg* cfg, const mccfr_root_state* root, uint64_t tid, uint32_t tag,
                    uint8_t* hole_out, uint8_t* board_out);
+/* Sparse form of a launch's contributions: the touched table entries
+ * (idx = slot * row + action, ascending) and, per entry, the FP64 sums of the
+ * regret and strategy-sum contributions and of their absolute values. */
+typedef struct orc_sparse {
+  int64_t n;
+  uint64_t* idx;
+  double *dR, *dS, *aR, *aS;
+} orc_sparse;
+/* orc_replay_launch_root for full-size tables: traversals split over
+ * `threads` host threads (0 = all), contributions returned sparse (free with
+ * orc_sparse_free); sigma must be given.  Returns the entry count or -1. */
+int64_t orc_replay_sparse(const mccfr_game_cfg* cfg, const mccfr_root_state* root, const int32_t* hr,
+                          const uint8_t* lut, const float* R, const uint16_t* sigma, int update_player,
+                          int64_t t_weight, uint64_t first_tid, uint32_t n, uint32_t tag, int threads,
+                          mccfr_trace* trace, orc_sparse* out);
+void orc_sparse_free(orc_sparse* s);
+
 /* One decision node of a walk, as the walk saw it and left it (the state
  * fields are those after the action was applied and the queue rebuilt). */
 typedef struct orc_decision {

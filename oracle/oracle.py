@@ -51,6 +51,9 @@ _SIGS = {
     "orc_eval_scripted": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
     "orc_eval_scripted_action": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int, C.c_uint64, C.c_int64, _P]),
     "orc_eval_buckets": (C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
+    "orc_replay_sparse": (C.c_int64, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32,
+                                      C.c_uint32, C.c_int, _P, _P]),
+    "orc_sparse_free": (None, [_P]),
     "orc_walk_cards": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int64, C.c_uint64, C.c_uint32,
                                  _P, _P, C.c_int, C.POINTER(C.c_int), _P, _P, _P, _P, C.c_int, C.POINTER(C.c_int)]),
     "orc_cpu_baseline": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_uint32, C.c_int, C.c_int64,
@@ -155,6 +158,31 @@ def walk_cards(cfg, hr, lut, hole, board, dealer, R, sigma, up, t_weight=1, tid=
         raise RuntimeError("orc_walk_cards rejected its arguments")
     k = n_c.value
     return tr[0], log[: n_log.value], (cs[:k], ca[:k], crd[:k], csd[:k])
+
+
+class _Sparse(C.Structure):
+    _fields_ = [("n", C.c_int64), ("idx", C.POINTER(C.c_uint64)), ("dR", C.POINTER(C.c_double)),
+                ("dS", C.POINTER(C.c_double)), ("aR", C.POINTER(C.c_double)), ("aS", C.POINTER(C.c_double))]
+
+
+def replay_sparse(cfg, hr, lut, R, sigma, up, t_weight, first_tid, n, tag=0, root=None, threads=16, trace=True):
+    """Multithreaded FP64 replay of one launch for full-size tables: (idx, dR,
+    dS, aR, aS) over the touched entries (idx = slot * row + action), and the
+    per-traversal records."""
+    R = np.ascontiguousarray(R, np.float32)
+    sigma = np.ascontiguousarray(sigma, np.uint16)
+    tr = np.zeros(n, dtype=TRACE_DTYPE) if trace else None
+    sp = _Sparse()
+    k = lib().orc_replay_sparse(C.byref(cfg), _root_ptr(root), p(hr), p(lut), p(R), p(sigma), up, t_weight,
+                                first_tid, n, tag, threads, p(tr) if trace else None, C.byref(sp))
+    if k < 0:
+        raise RuntimeError("orc_replay_sparse failed")
+    try:
+        out = tuple(np.ctypeslib.as_array(getattr(sp, f), shape=(k,)).copy() if k else np.zeros(0)
+                    for f in ("idx", "dR", "dS", "aR", "aS"))
+    finally:
+        lib().orc_sparse_free(C.byref(sp))
+    return out + (tr,)
 
 
 def deal_root(cfg, root, tid, tag=0):

@@ -127,8 +127,9 @@ def test_randomized_tables_follow_the_stream(hand_table):
 def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
     """6 players, 100bb, 2^26-slot tables, 2^20 traversals per launch, from a
     randomized regret state: after warm-up launches, every lane of a launch is
-    replayed by the oracle (trace-only) from the downloaded pre-launch tables:
-    bit-exact; table-wide strategy-sum mass equals t * (update nodes)."""
+    replayed by the oracle from the downloaded pre-launch tables: bit-exact;
+    every touched table entry within tolerance of the replay's FP64 sums,
+    every other entry unchanged; strategy-sum mass equals t * (update nodes)."""
     cfg = mccfr.cfg_6p_100bb(table_log2=26, seed=0)
     n = 1 << 20
     with mccfr.Context(cfg, n) as ctx:
@@ -142,7 +143,7 @@ def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
         tr = ctx.run_launch(0, 2, tid, n, trace=True)
         recs, _, _ = ctx.last_launch_stats()
         R1, S1, _ = ctx.download()
-        otr = oracle.replay_trace(cfg, hand_table, golden_lut, R0, sig0, 0, 2, tid, n)
+        idx, dR, dS, aR, aS, otr = oracle.replay_sparse(cfg, hand_table, golden_lut, R0, sig0, 0, 2, tid, n)
         for f in ("path_hash", "decisions", "updates", "terminal"):
             assert np.array_equal(tr[f], otr[f]), f
         assert np.array_equal(tr["utility"].view(np.uint32), otr["utility"].view(np.uint32))
@@ -151,3 +152,4 @@ def test_full_size_6max_100bb_lanes_match_oracle(hand_table, golden_lut):
         assert recs == nodes  # every update node emitted once
         assert abs(mass - 2.0 * nodes) <= 1e-4 * 2.0 * nodes
         assert (R1 >= 0).all()
+        helpers.check_launch_sparse(R0, S0, R1, S1, idx, dR, dS, aR, aS)
