@@ -635,7 +635,7 @@ def tenant_footprint(cfg: GameCfg) -> int:
     return int(b.value)
 
 
-WAITING, RUNNING, DONE, RELEASED = range(4)
+WAITING, RUNNING, DONE, RELEASED, FAILED = range(5)
 
 
 class Server:

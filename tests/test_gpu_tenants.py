@@ -319,3 +319,70 @@ def test_server_admits_by_hbm_budget(hand_table):
     out, order = serve(hand_table, reqs, max_running=8, budget=2 * fp + 4096)
     assert sorted(order) == list(range(5))
     assert serve.max_seen == 2  # two tables fit the pool at a time
+
+
+def test_server_admits_requests_that_arrive_while_others_run(hand_table):
+    """Requests submitted between steps while others are on the device, with
+    two running places and a pool for two tables, so later ones wait for a
+    place and for memory: each finishes with the tables it gets alone."""
+    reqs = small_requests(7)
+    for r in reqs:
+        r.cfg.table_log2 = 15
+    fp = mccfr.tenant_footprint(reqs[0].cfg)
+    got = {}
+    submitted_mid_flight = waited = 0
+    with mccfr.Context(small_ref(), 1 << 15) as ctx:
+        ctx.set_hand_table(hand_table)
+        with mccfr.Server(ctx, 2 * fp + 4096, 2) as srv:
+            ids = {}
+            pending = set()
+            nxt = 0
+            for step in range(100000):
+                if nxt < len(reqs) and step % 3 == 0:  # one arrival every third step
+                    r = reqs[nxt]
+                    running_before = sum(srv.poll(ids[i])[0] == mccfr.RUNNING for i in pending)
+                    submitted_mid_flight += running_before > 0
+                    ids[nxt] = srv.submit(r.cfg, r.root, r.iterations, r.batch)
+                    pending.add(nxt)
+                    nxt += 1
+                running, waiting = srv.step(wait=True)
+                assert running <= 2
+                waited += waiting > 0
+                for i in sorted(pending):
+                    if srv.poll(ids[i])[0] == mccfr.DONE:
+                        got[i] = ctx.tenant_download(srv.tenant(ids[i]))[:2]
+                        srv.release(ids[i])
+                        pending.discard(i)
+                if nxt == len(reqs) and not pending:
+                    break
+    assert submitted_mid_flight >= 3 and waited > 0
+    for i, r in enumerate(reqs):
+        alone, _ = serve(hand_table, [r], max_running=1)
+        for k in range(2):
+            assert np.array_equal(got[i][k].view(np.uint32), alone[0][k].view(np.uint32)), (i, k)
+
+
+def test_server_fails_a_request_whose_launch_outgrows_the_scratch(hand_table):
+    """After mccfr_set_max_batch shrinks the context below a running request's
+    launch, the step reports MCCFR_E_STATE, the request ends as failed (state
+    4) and can be released, and the server keeps serving others."""
+    reqs = small_requests(2)
+    with mccfr.Context(small_ref(), 4096) as ctx:
+        ctx.set_hand_table(hand_table)
+        with mccfr.Server(ctx, 1 << 30, 4) as srv:
+            big = srv.submit(reqs[0].cfg, reqs[0].root, 100000, 4096)
+            srv.step(wait=False)
+            assert srv.poll(big)[0] == mccfr.RUNNING
+            ctx.set_max_batch(1024)
+            with pytest.raises(mccfr.MccfrError) as e:
+                for _ in range(4):
+                    srv.step(wait=True)
+            assert e.value.code == mccfr.E_STATE
+            assert srv.poll(big)[0] == 4
+            srv.release(big)
+            ok = srv.submit(reqs[1].cfg, reqs[1].root, 2000, 512)
+            for _ in range(10000):
+                srv.step(wait=True)
+                if srv.poll(ok)[0] == mccfr.DONE:
+                    break
+            assert srv.poll(ok)[0] == mccfr.DONE
