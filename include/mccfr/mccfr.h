@@ -69,7 +69,26 @@ typedef struct mccfr_game_cfg {
   int32_t table_log2;       /* info-set slots = 2^table_log2 (reference: 26) */
   uint64_t seed;            /* Philox key of the traversal streams */
   mccfr_action menu[MCCFR_MAX_ACTIONS];
+  int32_t sampling;         /* MCCFR_SAMPLE_OUTCOME or MCCFR_SAMPLE_EXTERNAL (below) */
+  int32_t reserved;
 } mccfr_game_cfg;
+
+/* Sampling at the update player's nodes.
+ * OUTCOME (0, the reference's, cfr_gpu.cu:768-838): one action is sampled
+ *   from the strategy snapshot and weighted by 1 / max(sigma, 0.05); a
+ *   traversal is one root-to-leaf path.
+ * EXTERNAL (1, textbook external sampling; no counterpart in the reference,
+ *   which labels itself external sampling at cfr_gpu.cu:2 and :627-630 but
+ *   samples): every legal action is explored, in action order; other seats
+ *   sample one action from the snapshot as in OUTCOME.  A node's value is
+ *   v(I) = sum_a sigma(a) v(a) with sigma from the FP32 regrets (as the
+ *   reference's update node forms it), its regret contributions are
+ *   v(a) - v(I) for every legal a, its strategy-sum contributions t * sigma(a)
+ *   (the reference's rule, cfr_gpu.cu:828-836).  Draws follow the walk in
+ *   depth-first order (children in action order); an update node draws
+ *   nothing.  A traversal's utility is its root value. */
+#define MCCFR_SAMPLE_OUTCOME 0
+#define MCCFR_SAMPLE_EXTERNAL 1
 
 /* Fills `cfg` with the reference game: its 8-action menu, no raise cap,
  * 3-bit history and 2^26 slots (cfr_gpu.cuh:17-25, abstraction.h:42-53). */
@@ -80,14 +99,15 @@ void mccfr_game_cfg_reference(mccfr_game_cfg* cfg, int num_players, int starting
 /* Per-traversal record written by mccfr_run_launch when a trace buffer is
  * given (test and debugging aid; costs one 16-byte store per traversal). */
 typedef struct mccfr_trace {
-  float utility;       /* terminal payoff of the update player */
-  uint32_t path_hash;  /* FNV-1a over (slot, action) of every decision on the path */
-  uint16_t decisions;  /* decision nodes on the path */
-  uint8_t updates;     /* update-player decision nodes on the path */
-  uint8_t terminal;    /* 1 fold, 2 showdown, 3 depth limit */
+  float utility;       /* terminal payoff of the update player (EXTERNAL: the root value) */
+  uint32_t path_hash;  /* FNV-1a over (slot, action) of every decision on the path
+                          (EXTERNAL: of every (node, action) entered, depth first) */
+  uint16_t decisions;  /* decision nodes on the path (EXTERNAL: in the explored tree) */
+  uint8_t updates;     /* update-player decision nodes on the path (EXTERNAL: in the tree) */
+  uint8_t terminal;    /* 1 fold, 2 showdown, 3 depth limit (EXTERNAL: of the last leaf) */
   uint16_t update_legal; /* legal actions summed over the update-player nodes */
-  uint16_t reserved;
-} mccfr_trace;
+  uint16_t reserved;     /* EXTERNAL: leaves of the explored tree */
+} mccfr_trace;         /* EXTERNAL: counts are taken modulo their field widths */
 
 typedef struct mccfr_ctx mccfr_ctx;
 

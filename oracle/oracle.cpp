@@ -469,7 +469,7 @@ struct Walker {
   int up;
   float tw;  // strategy-sum weight (float)iteration, cfr_gpu.cu:828
   uint32_t path_hash = 2166136261u;
-  int decisions = 0, updates = 0, terminal = 0, update_legal = 0;
+  int decisions = 0, updates = 0, terminal = 0, update_legal = 0, leaves = 0;
   std::vector<orc_decision>* log = nullptr;  // decision log (orc_walk_cards)
 
   void log_decision(const Game& g, int player, int a, uint32_t vm, uint32_t slot, uint32_t hist, int pot_before) {
@@ -591,12 +591,13 @@ struct Walker {
   // recursion is a single root-to-leaf path; the state is not restored after
   // the child returns because nothing reads it afterwards.
   float traverse(Game& g, int depth) {
-    if (depth >= 20) { terminal = 3; return 0.0f; }
+    if (depth >= 20) { terminal = 3; ++leaves; return 0.0f; }
     int active = 0, winner = -1;
     for (int i = 0; i < g.N; ++i)
       if (!(g.folded >> i & 1u)) { ++active; winner = i; }
     if (active == 1) {
       terminal = 1;
+      ++leaves;
       return winner == up ? (float)(g.pot - g.invested[up]) : (float)(-(int)g.invested[up]);
     }
     if (g.n_to_act == 0) {
@@ -614,6 +615,7 @@ struct Walker {
         if (g.n_to_act > 0) return traverse(g, depth + 1);
       }
       terminal = 2;
+      ++leaves;
       return showdown(g);
     }
     const int player = g.to_act[0];
@@ -636,6 +638,29 @@ struct Walker {
       const int nvalid = __builtin_popcount(vm);
       for (int a = 0; a < A; ++a)
         strat[a] = (sum_r > 1e-7f) ? strat[a] / sum_r : ((vm >> a & 1u) ? 1.0f / (float)nvalid : 0.0f);
+      if (cfg.sampling == MCCFR_SAMPLE_EXTERNAL) {
+        // external sampling: every legal action's subtree, in action order,
+        // from a copy of this node's state; no draw at this node
+        float v[MCCFR_MAX_ACTIONS] = {};
+        for (int a = 0; a < A; ++a) {
+          if (!(vm >> a & 1u)) continue;
+          Game child = g;
+          hash_step(slot, a);
+          const uint32_t hist = child.ab;
+          const int pot_before = child.pot;
+          --child.n_to_act;
+          for (int i = 0; i < child.n_to_act; ++i) child.to_act[i] = child.to_act[i + 1];
+          if (apply(child, player, a)) rebuild_queue(child, player);
+          log_decision(child, player, a, vm, slot, hist, pot_before);
+          v[a] = traverse(child, depth + 1);
+        }
+        float vi = 0.0f;  // v(I) = sum_a sigma(a) v(a), in action order
+        for (int a = 0; a < A; ++a)
+          if (vm >> a & 1u) vi += strat[a] * v[a];
+        for (int a = 0; a < A; ++a)
+          if (vm >> a & 1u) sink.add(slot, a, v[a] - vi, tw * strat[a]);
+        return vi;
+      }
       const int sampled = sample(slot, vm);
       const float q = std::fmax(strat[sampled], 0.05f);
       hash_step(slot, sampled);
@@ -785,7 +810,7 @@ void run_traversal(const mccfr_game_cfg& cfg, const mccfr_root_state* root, cons
     tr->updates = (uint8_t)w.updates;
     tr->terminal = (uint8_t)w.terminal;
     tr->update_legal = (uint16_t)w.update_legal;
-    tr->reserved = 0;
+    tr->reserved = (uint16_t)(cfg.sampling == MCCFR_SAMPLE_EXTERNAL ? w.leaves : 0);
   }
 }
 
@@ -1149,6 +1174,7 @@ bool cfg_ok(const mccfr_game_cfg* c) {
   if (c->num_players < 2 || c->num_players > 6) return false;
   if (c->num_actions < 1 || c->num_actions > MCCFR_MAX_ACTIONS) return false;
   if (c->table_log2 < 4 || c->table_log2 > 32) return false;
+  if (c->sampling != MCCFR_SAMPLE_OUTCOME && c->sampling != MCCFR_SAMPLE_EXTERNAL) return false;
   if (c->history_bits < 1 || c->history_bits > 8) return false;
   for (int a = 0; a < c->num_actions; ++a)
     if (c->menu[a].kind == MCCFR_ACT_POT_FRAC && c->menu[a].den == 0) return false;
@@ -1492,7 +1518,7 @@ int orc_walk_cards(const mccfr_game_cfg* cfg, const mccfr_root_state* root, cons
     trace->updates = (uint8_t)w.updates;
     trace->terminal = (uint8_t)w.terminal;
     trace->update_legal = (uint16_t)w.update_legal;
-    trace->reserved = 0;
+    trace->reserved = (uint16_t)(cfg->sampling == MCCFR_SAMPLE_EXTERNAL ? w.leaves : 0);
   }
   if (n_log) *n_log = (int)dec.size();
   if (n_contrib) *n_contrib = (int)contrib.size();

@@ -31,6 +31,7 @@ struct DevGame {
   int32_t table_log2;
   int32_t use_cfr_plus;
   int32_t use_linear_cfr;
+  int32_t sampling;      // MCCFR_SAMPLE_OUTCOME or MCCFR_SAMPLE_EXTERNAL
   uint32_t seed_lo, seed_hi;
   uint32_t tag;          // Philox counter word 3 (stream family)
   // stack[p]: chips of seat p at the start of the hand, so that invested =
@@ -92,6 +93,7 @@ struct LaunchCfg {
   uint32_t lanes;        // lane space of the launch (sum of aligned segment sizes)
   uint32_t slot_bits;    // record key = segment << slot_bits | slot
   uint32_t cap_lanes;    // lane capacity of the scratch lists
+  uint32_t es;           // 1: the launch's tenants use external sampling (k_walk_es)
 };
 
 // Counters the kernels of one launch write (reset by the launch itself).
@@ -101,7 +103,9 @@ struct LaunchCounters {
   uint32_t n_long;       // runs handed to the long-run reduction
   uint32_t n_pieces;     // pieces of the long runs (with n_long: one 64-bit word, bumped together)
   uint32_t n_runs;       // short runs listed for the row-parallel reduction
-  uint32_t pad[3];
+  uint32_t n_es;         // external-sampling records appended by the walk (may exceed the capacity)
+  uint32_t es_overflow;  // sticky: 1 records beyond capacity, 2 stack beyond its frames (host clears)
+  uint32_t pad;
 };
 static_assert(offsetof(LaunchCounters, n_long) % 8 == 0 &&
                   offsetof(LaunchCounters, n_pieces) == offsetof(LaunchCounters, n_long) + 4,

@@ -55,6 +55,7 @@ struct DevScratch {
   uint4* piece;          // [max_pieces] pieces of long runs: first record, end, key, long run
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
   LaunchCounters* cnt;
+  uint32_t* es_stack;    // external-sampling frames [kEsFrames][words][launch lanes] (not in the arena)
   uint32_t cap_lanes;
   uint32_t max_tiles;
 };
@@ -111,6 +112,9 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
                  hipEvent_t after_deal = nullptr, hipEvent_t after_walk = nullptr);
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits);
+
+// Bytes of external-sampling walk frames for a launch of `lanes` lanes.
+size_t es_stack_bytes(int row, uint32_t lanes);
 
 void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);

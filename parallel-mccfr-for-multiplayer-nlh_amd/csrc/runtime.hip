@@ -201,7 +201,8 @@ void free_all(mccfr_ctx* c) {
   for (auto& t : c->tenants)
     if (t.used && t.owned) (void)hipFree(t.owned);
   void* ptrs[] = {c->d_tenants, c->d_segs, c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets,
-                  c->dtrace, c->buckets.flop5, c->buckets.turn6, c->eval_bk.flop5, c->eval_bk.turn6};
+                  c->dtrace, c->buckets.flop5, c->buckets.turn6, c->eval_bk.flop5, c->eval_bk.turn6,
+                  c->es_stack};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -219,6 +220,42 @@ int rm_sweep_if_dirty(mccfr_ctx* c, Tenant& t) {
   HIPX(c, hipGetLastError());
   t.sigma_dirty = false;
   return MCCFR_OK;
+}
+
+// Frames of the external-sampling walk for a launch of `lanes` lanes (grown,
+// never shrunk; a captured graph holding the old frames is dropped).
+int ensure_es_stack(mccfr_ctx* c, int row, uint32_t lanes) {
+  const size_t want = es_stack_bytes(row, lanes);
+  if (want > c->es_stack_bytes) {
+    HIPX(c, hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    if (c->es_stack) (void)hipFree(c->es_stack);
+    c->es_stack = nullptr;
+    c->es_stack_bytes = 0;
+    const hipError_t e = hipMalloc(&c->es_stack, want);
+    if (e != hipSuccess) {
+      c->es_stack = nullptr;
+      return fail(c, MCCFR_E_NOMEM, std::string("external-sampling frames: ") + hipGetErrorString(e));
+    }
+    c->es_stack_bytes = want;
+  }
+  c->sc.es_stack = c->es_stack;
+  return MCCFR_OK;
+}
+
+// Waits for the context's work and reports (and clears) an external-sampling
+// launch whose records overflowed the sort input (its tables were left as
+// they were).
+int check_es(mccfr_ctx* c) {
+  uint32_t flag = 0;
+  HIPX(c, hipMemcpyAsync(&flag, &c->sc.cnt->es_overflow, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+  HIPX(c, hipStreamSynchronize(c->stream));
+  if (!flag) return MCCFR_OK;
+  const uint32_t zero = 0;
+  HIPX(c, hipMemcpy(&c->sc.cnt->es_overflow, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return fail(c, MCCFR_E_STATE,
+              "an external-sampling launch produced more records than its scratch holds (20 per lane of "
+              "max_batch): it updated no table; use fewer traversals per launch or a larger max_batch");
 }
 
 // Slot-major rows <-> action-major planes.
@@ -253,6 +290,9 @@ int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, const DevScratch& sc, P
     if (seen[hs[i].tenant]++) return fail(c, MCCFR_E_ARG, "a tenant appears in two segments of one launch");
     if (i == 0) p.row = t->row;
     if (t->row != p.row) return fail(c, MCCFR_E_ARG, "segments of one launch need the same row width");
+    if (i == 0) p.lc.es = t->cfg.sampling == MCCFR_SAMPLE_EXTERNAL;
+    if ((t->cfg.sampling == MCCFR_SAMPLE_EXTERNAL) != (p.lc.es != 0))
+      return fail(c, MCCFR_E_ARG, "segments of one launch need the same sampling");
     if (hs[i].n == 0) return fail(c, MCCFR_E_ARG, "segment with no traversals");
     if (hs[i].update_player < 0 || hs[i].update_player >= t->cfg.num_players)
       return fail(c, MCCFR_E_ARG, "bad update player");
@@ -373,6 +413,9 @@ bool make_dev_game(const mccfr_game_cfg& cfg, const mccfr_root_state* root, DevG
   g.table_log2 = cfg.table_log2;
   g.use_cfr_plus = cfg.use_cfr_plus;
   g.use_linear_cfr = cfg.use_linear_cfr;
+  if (cfg.sampling != MCCFR_SAMPLE_OUTCOME && cfg.sampling != MCCFR_SAMPLE_EXTERNAL)
+    return bad("sampling must be MCCFR_SAMPLE_OUTCOME or MCCFR_SAMPLE_EXTERNAL");
+  g.sampling = cfg.sampling;
   g.seed_lo = (uint32_t)cfg.seed;
   g.seed_hi = (uint32_t)(cfg.seed >> 32);
   g.tag = 0;
@@ -513,11 +556,15 @@ int tenant_destroy(mccfr_ctx* c, uint32_t id) {
 
 int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* host_trace, hipEvent_t done,
                  const DevScratch* scratch) {
-  const DevScratch& sc = scratch ? *scratch : c->sc;
   if (scratch && host_trace) return fail(c, MCCFR_E_ARG, "no trace for a launch on outside scratch");
   Prepared p;
-  int rc = prepare(c, segs, n, sc, p);
+  int rc = prepare(c, segs, n, scratch ? *scratch : c->sc, p);
   if (rc) return rc;
+  if (p.lc.es) {
+    if (scratch) return fail(c, MCCFR_E_ARG, "external sampling runs on the context's scratch only");
+    if ((rc = ensure_es_stack(c, p.row, p.lc.lanes))) return rc;
+  }
+  const DevScratch& sc = scratch ? *scratch : c->sc;
   for (uint32_t i = 0; i < n; ++i)
     if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;
   if (host_trace && !c->dtrace) HIPX(c, hipMalloc(&c->dtrace, (size_t)c->sc.cap_lanes * sizeof(mccfr_trace)));
@@ -555,6 +602,7 @@ int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* hos
     }
     HIPX(c, hipStreamSynchronize(c->stream));
   }
+  if (p.lc.es && !done) return check_es(c);  // (the server checks its own launches)
   return MCCFR_OK;
 }
 
@@ -869,6 +917,7 @@ int mccfr_train_tenants(mccfr_ctx* c, const uint32_t* tenants, const uint32_t* c
   {
     Prepared p;
     if ((rc = prepare(c, hs.data(), n, c->sc, p))) return rc;
+    if (p.lc.es && (rc = ensure_es_stack(c, p.row, p.lc.lanes))) return rc;
     for (uint32_t i = 0; i < n; ++i)
       if ((rc = rm_sweep_if_dirty(c, c->tenants[tenants[i]]))) return rc;
     std::vector<uint64_t> key = {(uint64_t)p.row,          (uint64_t)p.lc.lanes, (uint64_t)p.lc.n_segs,
@@ -913,7 +962,7 @@ done:
     t.upd_idx = hs[i].upd_idx;
   }
   HIPX(c, hipStreamSynchronize(c->stream));
-  return MCCFR_OK;
+  return check_es(c);
 }
 
 int mccfr_train(mccfr_ctx* c, int64_t iters, uint32_t batch) {
@@ -936,6 +985,7 @@ int mccfr_set_max_batch(mccfr_ctx* c, uint32_t max_batch) {
   if (cap != c->sc.cap_lanes) {
     free_scratch(c->sc);
     const hipError_t e = alloc_scratch(c->sc, cap);
+    c->sc.es_stack = c->es_stack;
     if (e != hipSuccess) {
       free_scratch(c->sc);
       c->max_batch = 0;
@@ -1031,6 +1081,7 @@ int mccfr_autotune(mccfr_ctx* c, const mccfr_segment* segs, uint32_t n_segs, int
   Prepared p;
   int rc = prepare(c, hs.data(), n_segs, c->sc, p);
   if (rc) return rc;
+  if (p.lc.es) return fail(c, MCCFR_E_ARG, "the autotuner sweeps the outcome-sampling walk's shapes");
   for (uint32_t i = 0; i < n_segs; ++i)
     if ((rc = rm_sweep_if_dirty(c, c->tenants[segs[i].tenant]))) return rc;
   if ((rc = stage_segments(c, p.segs))) return rc;

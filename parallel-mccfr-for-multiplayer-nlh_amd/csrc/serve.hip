@@ -275,6 +275,8 @@ int mccfr_server_submit(mccfr_server* s, const mccfr_game_cfg* cfg, const mccfr_
   DevGame g;
   const char* why = nullptr;
   if (!make_dev_game(*cfg, root, g, &why)) return fail(s->ctx, MCCFR_E_ARG, why);
+  if (cfg->sampling != MCCFR_SAMPLE_OUTCOME)
+    return fail(s->ctx, MCCFR_E_ARG, "the server runs outcome-sampling requests (MCCFR_SAMPLE_OUTCOME)");
   if (iterations < 1) return fail(s->ctx, MCCFR_E_ARG, "iterations must be >= 1");
   if (batch < 1 || batch > s->ctx->max_batch) return fail(s->ctx, MCCFR_E_ARG, "batch must be 1..max_batch");
   Request r;

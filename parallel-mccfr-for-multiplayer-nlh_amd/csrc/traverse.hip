@@ -53,6 +53,14 @@ constexpr uint32_t kTurnUnknown = 0xFFu;
 // sd_state bit of a traversal that ended in a showdown (the low bits hold the folded seats).
 constexpr uint32_t kShowdownFlag = 0x80u;
 
+// Record flags above the legal mask (bits 8..23 of a record's action word):
+// an external-sampling record adds its value to its action's regret only
+// (kEsRecord), and the first record of each node adds the node's
+// strategy-sum share t * sigma to every legal action (kEsHead).
+constexpr uint32_t kEsRecord = 1u << 24;
+constexpr uint32_t kEsHead = 1u << 25;
+constexpr uint32_t kLegalBits = 0xFFFFu;
+
 // Waiting lanes (with a next traversal) that trigger a refill round of the walk.
 constexpr int kRefillLanes = 48;
 
@@ -612,71 +620,11 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   }
 }
 
-// Exclusive prefix of v over a 256-thread workgroup (every thread calls it);
-// the workgroup total in *total when total is not null.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += o;
-  }
-  if (lane == 63) s_w[w] = incl;
-  __syncthreads();
-  uint32_t before = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t x = s_w[k];
-    if (k < w) before += x;
-    tot += x;
-  }
-  __syncthreads();
-  if (total) *total = tot;
-  return before + incl - v;
-}
-
-// ---------------------------------------------------------------------------
-// Showdown payoffs (cfr_gpu.cu:694-752): side pots in ascending levels over
-// every seat's investment, each level shared by the best live ranks that
-// reach it.  One lane per traversal that ended in a showdown, after the walk;
-// the walk leaves the investments (sd_inv) and the folded seats (sd_state).
-// ---------------------------------------------------------------------------
-template <bool TRACE>
-__global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                  const DevGame* __restrict__ tenants, DevScratch sc,
-                                                  mccfr_trace* __restrict__ trace) {
-  static_assert(kEmitLanes == 256, "a showdown workgroup is one emission block");
-  __shared__ uint32_t s_w[4];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = t < lc.lanes;
-  // every load is issued before the block scan (its barriers), so their
-  // latencies overlap it; lanes return only after the scan
-  const uint32_t cnt = in ? (uint32_t)sc.node_count[t] : 0u;
-  const uint32_t st = in ? (uint32_t)sc.sd_state[t] : 0u;
-  const bool showdown = (st & kShowdownFlag) != 0;
-  int N = 0, up = 0;
-  if (uniform(t & ~63u) < lc.lanes) {  // wave-uniform: the wave's segment
-    const DevSeg& Sg = segs[find_seg(segs, lc.n_segs, uniform(t & ~63u))];
-    N = tenants[uniform(Sg.tenant)].N;
-    up = Sg.update_player;
-  }
-  const size_t L = sc.cap_lanes;
-  int inv[kMaxPlayers];
-  uint32_t rkv[kMaxPlayers];
-#pragma unroll
-  for (int p = 0; p < kMaxPlayers; ++p) {
-    inv[p] = showdown && p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
-    rkv[p] = showdown && p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
-  }
-  {
-    // the emission block's record count (the walk has set every lane's count)
-    uint32_t total = 0;
-    (void)block_excl_scan_256(cnt, s_w, &total);
-    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
-  }
-  if (!showdown) return;
-  const uint32_t folded = st & ~kShowdownFlag;
+// Side-pot payoff of the update player at a showdown (cfr_gpu.cu:694-752):
+// layers over the distinct investments in ascending order, each shared by the
+// best live ranks that reach it.  inv / rkv: every seat's investment and
+// 7-card rank (0 beyond the table's seats).
+__device__ __forceinline__ float showdown_payoff(int up, uint32_t folded, const int* inv, const uint32_t* rkv) {
   int inv_up = inv[0];
   uint32_t rk_up = rkv[0];
 #pragma unroll
@@ -735,6 +683,484 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
       prev = lvl;
     }
   }
+  return pay;
+}
+
+// ---------------------------------------------------------------------------
+// k_walk_es: external sampling (mccfr.h MCCFR_SAMPLE_EXTERNAL).  One lane per
+// traversal walks its tree depth first: other seats' nodes sample one action
+// from the BF16 snapshot as in k_walk, and an update-player node explores
+// every legal action in action order.  Such a node pushes a frame (the state
+// before its action, its slot and legal mask, the actions left and the value
+// of each explored child) onto the lane's stack in HBM, laid out
+// [frame][word][lane] so that a wave's frame accesses are coalesced; a leaf's
+// value returns into the top frame, which restores its state for the next
+// action or, with none left, forms v(I) = sum_a sigma(a) v(a) (sigma from the
+// FP32 regrets, as the reference's update node forms it) and emits a record
+// (a | mask << 8 | kEsRecord, v(a) - v(I)) per legal action, the first
+// flagged kEsHead (it carries the node's strategy-sum share).  The update
+// player acts at most on every other level, so the depth guard of 20 bounds
+// the stack at kEsFrames frames.  Records are appended to the sort input
+// with one atomic per wave and iteration (k_es_begin zeroes the counter);
+// beyond the input's capacity nothing more is written and k_emit_scan marks
+// the launch as overflowed (no table is touched).  Showdown payoffs are
+// settled where the walk reaches them, since the value is needed at once.
+// ---------------------------------------------------------------------------
+constexpr int kEsFrames = 10;
+template <int ROW>
+constexpr int es_frame_words() {
+  return 14 + ROW;  // 0..5 seats, 6 seat masks + queue, 7 history, 8 pot, 9 bet, 10 lfr, 11 street | raises |
+                    // depth, 12 slot, 13 legal | left << 16, 14.. child values
+}
+
+__global__ void k_es_begin(DevScratch sc) {
+  if (threadIdx.x == 0) sc.cnt->n_es = 0;
+}
+
+template <int ROW, bool TRACE>
+__global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                 const DevGame* __restrict__ tenants, DevScratch sc,
+                                                 uint32_t* __restrict__ stack, mccfr_trace* __restrict__ trace) {
+  constexpr int WG = 256;
+  constexpr int FW = es_frame_words<ROW>();
+  __shared__ uint32_t s_sb[kMaxPlayers * WG];
+  __shared__ uint32_t s_bk[kMaxPlayers * WG];
+  __shared__ uint32_t s_menu[WG / 64][MCCFR_MAX_ACTIONS];
+  __shared__ uint32_t s_rcp[WG / 64][MCCFR_MAX_ACTIONS];
+  const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+  const uint32_t t = blockIdx.x * WG + (uint32_t)tx;
+  const uint32_t base = uniform(t & ~63u);
+  if (base >= lc.lanes) return;
+  const uint32_t si = find_seg(segs, lc.n_segs, base);
+  const DevSeg& Sg = segs[si];
+  const DevGame& G = tenants[uniform(Sg.tenant)];
+  uint32_t* menu = s_menu[wv];
+  uint32_t* rcp = s_rcp[wv];
+  if (lane < MCCFR_MAX_ACTIONS) {
+    menu[lane] = menu_word(G, lane);
+    rcp[lane] = G.kind[lane] == MCCFR_ACT_POT_FRAC ? den_reciprocal(G.den[lane]) : 0u;
+  }
+  __builtin_amdgcn_wave_barrier();
+  uint32_t* sb = s_sb + tx;
+  uint32_t* bk = s_bk + tx;
+  Rules rules;
+  rules.check_mask = G.check_mask;
+  rules.call_mask = G.call_mask;
+  rules.fold_mask = G.fold_mask;
+  rules.allin_mask = G.allin_mask;
+  rules.raise_mask = G.raise_mask;
+  rules.max_raises = G.max_raises;
+  rules.bb = G.bb;
+  {
+    uint32_t rm = rules.raise_mask;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      rules.fast[i] = 0u;
+      if (rm) {
+        const int a = __builtin_ctz(rm);
+        rm &= rm - 1;
+        const bool min_raise = G.kind[a] == MCCFR_ACT_RAISE_MIN;
+        rules.fast[i] = (min_raise ? 0u : (uint32_t)G.num[a]) | ((min_raise ? 1u : (uint32_t)G.den[a]) << 8) |
+                        ((1u << a) << 16);
+      }
+    }
+    rules.raise_rest = rm;
+  }
+  const int N = G.N;
+  const int A = G.num_actions;
+  const int hist_bits = G.hist_bits;
+  const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
+  const uint32_t seg_key = uniform((uint32_t)((uint64_t)si << lc.slot_bits));
+  const int up = Sg.update_player;
+  const uint32_t allmask = (1u << N) - 1u;
+  const size_t L = sc.cap_lanes;   // root records
+  const size_t SL = lc.lanes;      // stack frames
+  const uint32_t cap_rec = sc.cap_lanes * (uint32_t)kMaxDepth;
+  const bool real = t - Sg.lane0 < Sg.n;  // padding lanes of the segment's last wave walk nothing
+
+  // root state (as k_walk)
+  uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
+  int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0;
+  const int dealer = real ? (int)sc.root_dealer[t] : 0;
+  if (real) {
+    const int root_street = G.root_street;
+#pragma unroll
+    for (int p = 0; p < kMaxPlayers; ++p) {
+      if (p < N) {
+        sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
+        bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
+      }
+    }
+    if (root_street == 0) {
+      const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
+      const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
+      const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
+      const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
+      sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
+      if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
+      const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
+      const int bb_amt = rules.bb < s_bb ? rules.bb : s_bb;
+      sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 15);
+      if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
+      pot = sb_amt + bb_amt;
+      current_bet = bb_amt;
+      qmask = allmask & ~all_in;
+      qpos = wrap(bb_pos + 1, N);
+    } else {
+      folded = G.root_folded;
+      all_in = G.root_allin;
+      pot = G.root_pot;
+      street = root_street;
+      ab = G.root_ab;
+      qmask = allmask & ~(folded | all_in);
+      qpos = wrap(dealer + 1, N);
+    }
+  }
+  DrawStream rng;
+  rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + (t - Sg.lane0), G.tag);
+  rng.k = G.walk_k0;
+
+  int sp = 0;            // frames on the stack
+  bool done = !real;     // the traversal's root value is known
+  bool ret_mode = false; // a value is returning up the tree
+  float ret = 0.0f;
+  int decisions = 0, updates = 0, upd_legal = 0, leaves = 0, terminal = 0;
+  auto fr = [&](int f, int w) -> uint32_t& { return stack[((size_t)f * FW + w) * SL + t]; };
+
+  // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585) for `player`
+  auto apply = [&](int player, int act) {
+    const uint32_t sbv = sb[player * WG];
+    const int stk = (int)(sbv & kStackMask);
+    const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+    const int to_call = current_bet - bet;
+    qmask &= ~(1u << player);
+    qpos = wrap(player + 1, N);
+    const int chips = chips_lds(G, menu, rcp, act, pot, stk, to_call, current_bet, lfr);
+    ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
+    if (rules.fold_mask >> act & 1u) {
+      folded |= 1u << player;
+    } else {
+      const int ns = stk - chips, nb = bet + chips;
+      sb[player * WG] = (uint32_t)ns | ((uint32_t)nb << 15) | ((uint32_t)street << 30);
+      pot += chips;
+      if (ns == 0) all_in |= 1u << player;
+      if (nb > current_bet) {
+        const int raise = nb - current_bet;
+        if (raise >= lfr || current_bet == 0) lfr = raise;
+        current_bet = nb;
+        ++raises;
+        qmask = allmask & ~(folded | all_in) & ~(1u << player);
+      }
+    }
+    ++depth;
+  };
+
+  while (true) {
+    if (__ballot(!done) == 0) break;
+    uint32_t emit_k = 0, e_slot = 0, e_vm = 0;
+    float rd[ROW];
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) rd[a] = 0.0f;
+    if (!done && !ret_mode) {
+      // one recursion level of traverse_gpu (cfr_gpu.cu:644-860)
+      bool leaf = false;
+      if (depth >= kMaxDepth) {
+        ret = 0.0f;  // depth guard (cfr_gpu.cu:661)
+        terminal = 3;
+        leaf = true;
+      } else if (__popc(allmask & ~folded) == 1) {
+        const int inv_up = G.stack[up] - (int)(sb[up * WG] & kStackMask);
+        ret = ((int)__builtin_ctz(allmask & ~folded) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
+        terminal = 1;
+        leaf = true;
+      } else if (qmask == 0) {
+        // street change (cfr_gpu.cu:668-692); with nobody left to act, or
+        // after the river, the showdown
+        bool showdown = street == 3;
+        if (!showdown) {
+          ++street;
+          current_bet = 0;
+          lfr = rules.bb;
+          raises = 0;
+          qmask = allmask & ~(folded | all_in);
+          qpos = wrap(dealer + 1, N);
+          showdown = qmask == 0;
+          if (!showdown) ++depth;
+        }
+        if (showdown) {
+          int inv[kMaxPlayers];
+          uint32_t rkv[kMaxPlayers];
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p) {
+            inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
+            rkv[p] = p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+          }
+          ret = showdown_payoff(up, folded, inv, rkv);
+          terminal = 2;
+          leaf = true;
+        }
+      } else {
+        const uint32_t rot = ((qmask >> qpos) | (qmask << (N - qpos))) & allmask;
+        const int player = wrap(qpos + (int)__builtin_ctz(rot), N);
+        const uint32_t sbv = sb[player * WG];
+        const int stk = (int)(sbv & kStackMask);
+        const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+        const int to_call = current_bet - bet;
+        const uint32_t vm = rules_mask(rules, menu, pot, stk, to_call, current_bet, lfr, raises);
+        uint32_t Bp = bk[player * WG];
+        if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
+          const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
+          Bp = (Bp & 0xFF00FFFFu) | (tb << 16);
+          bk[player * WG] = Bp;
+        }
+        const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
+        const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
+        ++decisions;
+        int act = -1;
+        if (player == up && sp >= kEsFrames) {
+          // cannot happen (the update player acts on at most every other
+          // level); flagged for the host rather than written out of bounds
+          sc.cnt->es_overflow = 2u;
+          ret = 0.0f;
+          terminal = 3;
+          leaf = true;
+        } else if (player == up) {
+          // push the node: its state before the action, slot, legal actions
+          ++updates;
+          upd_legal += __popc(vm);
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p)
+            if (p < N) fr(sp, p) = sb[p * WG];
+          fr(sp, 6) = folded | (all_in << 8) | (qmask << 16) | ((uint32_t)qpos << 24);
+          fr(sp, 7) = ab;
+          fr(sp, 8) = (uint32_t)pot;
+          fr(sp, 9) = (uint32_t)current_bet;
+          fr(sp, 10) = (uint32_t)lfr;
+          fr(sp, 11) = (uint32_t)street | ((uint32_t)raises << 8) | ((uint32_t)depth << 16);
+          fr(sp, 12) = slot;
+          act = __builtin_ctz(vm);
+          fr(sp, 13) = vm | ((vm & (vm - 1u)) << 16);
+          ++sp;
+        } else {
+          // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot, as k_walk
+          float sg[ROW];
+          load_bf16_row<ROW>(G.sig + (size_t)slot * ROW, sg);
+          float cum[ROW];
+          float total = 0.0f;
+#pragma unroll
+          for (int a = 0; a < ROW; ++a) {
+            total += (vm >> a & 1u) ? sg[a] : 0.0f;
+            cum[a] = total;
+          }
+          if (total >= 1e-7f) {
+            const float r = u01(rng.next()) * total;
+            int below = 0;
+#pragma unroll
+            for (int a = 0; a < ROW; ++a) below += cum[a] < r ? 1 : 0;
+            act = below;
+          }
+          if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508)
+            for (int i = 0; i < G.n_passive; ++i) {
+              const int a = (int)(G.passive_order[i] & 15u);
+              if (vm >> a & 1u) {
+                act = a;
+                break;
+              }
+            }
+            if (act < 0) act = 0;
+          }
+        }
+        if (act >= 0) {
+          if (TRACE) phash = fnv_path(phash, slot, (uint32_t)act);
+          apply(player, act);
+        }
+      }
+      if (leaf) {
+        ++leaves;
+        ret_mode = true;
+      }
+    } else if (!done) {
+      // the value `ret` returns into the top frame
+      if (sp == 0) {
+        done = true;
+        if (TRACE) {
+          mccfr_trace r;
+          r.utility = ret;
+          r.path_hash = phash;
+          r.decisions = (uint16_t)decisions;
+          r.updates = (uint8_t)updates;
+          r.terminal = (uint8_t)terminal;
+          r.update_legal = (uint16_t)upd_legal;
+          r.reserved = (uint16_t)leaves;
+          trace[t] = r;
+        }
+      } else {
+        const int f = sp - 1;
+        const uint32_t w13 = fr(f, 13);
+        const uint32_t vm = w13 & 0xFFFFu, left = w13 >> 16;
+        const int cur = 31 - (int)__builtin_clz(vm & ~left);  // explored in action order
+        fr(f, 14 + cur) = __float_as_uint(ret);
+        if (left) {
+          // the node's next action, from its saved state
+          const int a = __builtin_ctz(left);
+          fr(f, 13) = vm | ((left & (left - 1u)) << 16);
+#pragma unroll
+          for (int p = 0; p < kMaxPlayers; ++p)
+            if (p < N) sb[p * WG] = fr(f, p);
+          const uint32_t w6 = fr(f, 6);
+          folded = w6 & 0xFFu;
+          all_in = (w6 >> 8) & 0xFFu;
+          qmask = (w6 >> 16) & 0xFFu;
+          qpos = (int)(w6 >> 24);
+          ab = fr(f, 7);
+          pot = (int)fr(f, 8);
+          current_bet = (int)fr(f, 9);
+          lfr = (int)fr(f, 10);
+          const uint32_t w11 = fr(f, 11);
+          street = (int)(w11 & 0xFFu);
+          raises = (int)((w11 >> 8) & 0xFFu);
+          depth = (int)(w11 >> 16);
+          if (TRACE) phash = fnv_path(phash, fr(f, 12), (uint32_t)a);
+          apply(up, a);
+          ret_mode = false;
+        } else {
+          // every action explored: v(I) and the node's records
+          const uint32_t slot = fr(f, 12);
+          float rr[ROW], v[ROW];
+          load_f32_row<ROW>(G.R + (size_t)slot * 2 * ROW, rr);
+          float sum_r = 0.0f;
+#pragma unroll
+          for (int a = 0; a < ROW; ++a) {
+            const bool legal = a < A && (vm >> a & 1u);
+            rr[a] = legal ? fmaxf(0.0f, rr[a]) : 0.0f;
+            sum_r += rr[a];
+            v[a] = legal ? __uint_as_float(fr(f, 14 + a)) : 0.0f;
+          }
+          const float uni = 1.0f / (float)__popc(vm);
+          float vi = 0.0f;
+#pragma unroll
+          for (int a = 0; a < ROW; ++a) {
+            const bool legal = a < A && (vm >> a & 1u);
+            const float sig = sum_r > 1e-7f ? rr[a] / sum_r : (legal ? uni : 0.0f);
+            if (legal) vi += sig * v[a];
+          }
+#pragma unroll
+          for (int a = 0; a < ROW; ++a) rd[a] = v[a] - vi;
+          emit_k = (uint32_t)__popc(vm);
+          e_slot = slot;
+          e_vm = vm;
+          ret = vi;
+          --sp;
+        }
+      }
+    }
+    // records of the nodes completed in this iteration: a wave prefix over
+    // the lanes' counts, one atomic for the wave
+    uint32_t incl = emit_k;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total) {
+      // once past the capacity the counter stops growing (it cannot wrap)
+      // and the launch is flagged; nothing more is written
+      uint32_t wbase = 0;
+      if (lane == 0) {
+        if (__hip_atomic_load(&sc.cnt->n_es, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > cap_rec) {
+          wbase = 0xFFFFFFFFu;
+          sc.cnt->es_overflow = 1u;
+        } else {
+          wbase = atomicAdd(&sc.cnt->n_es, total);
+        }
+      }
+      wbase = __shfl(wbase, 0, 64);
+      uint32_t off = wbase + incl - emit_k;
+      if (emit_k && wbase != 0xFFFFFFFFu && off + emit_k <= cap_rec) {
+        uint32_t left = e_vm;
+        uint32_t head = kEsHead;
+#pragma unroll
+        for (int a = 0; a < ROW; ++a) {
+          if (left >> a & 1u) {
+            sc.key[0][off] = seg_key | e_slot;
+            sc.val[0][off] = make_uint2((uint32_t)a | (e_vm << 8) | kEsRecord | head, __float_as_uint(rd[a]));
+            head = 0;
+            ++off;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Exclusive prefix of v over a 256-thread workgroup (every thread calls it);
+// the workgroup total in *total when total is not null.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t x = s_w[k];
+    if (k < w) before += x;
+    tot += x;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return before + incl - v;
+}
+
+
+// ---------------------------------------------------------------------------
+// Showdown payoffs (cfr_gpu.cu:694-752): side pots in ascending levels over
+// every seat's investment, each level shared by the best live ranks that
+// reach it.  One lane per traversal that ended in a showdown, after the walk;
+// the walk leaves the investments (sd_inv) and the folded seats (sd_state).
+// ---------------------------------------------------------------------------
+template <bool TRACE>
+__global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                  const DevGame* __restrict__ tenants, DevScratch sc,
+                                                  mccfr_trace* __restrict__ trace) {
+  static_assert(kEmitLanes == 256, "a showdown workgroup is one emission block");
+  __shared__ uint32_t s_w[4];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < lc.lanes;
+  // every load is issued before the block scan (its barriers), so their
+  // latencies overlap it; lanes return only after the scan
+  const uint32_t cnt = in ? (uint32_t)sc.node_count[t] : 0u;
+  const uint32_t st = in ? (uint32_t)sc.sd_state[t] : 0u;
+  const bool showdown = (st & kShowdownFlag) != 0;
+  int N = 0, up = 0;
+  if (uniform(t & ~63u) < lc.lanes) {  // wave-uniform: the wave's segment
+    const DevSeg& Sg = segs[find_seg(segs, lc.n_segs, uniform(t & ~63u))];
+    N = tenants[uniform(Sg.tenant)].N;
+    up = Sg.update_player;
+  }
+  const size_t L = sc.cap_lanes;
+  int inv[kMaxPlayers];
+  uint32_t rkv[kMaxPlayers];
+#pragma unroll
+  for (int p = 0; p < kMaxPlayers; ++p) {
+    inv[p] = showdown && p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
+    rkv[p] = showdown && p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+  }
+  {
+    // the emission block's record count (the walk has set every lane's count)
+    uint32_t total = 0;
+    (void)block_excl_scan_256(cnt, s_w, &total);
+    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
+  }
+  if (!showdown) return;
+  const uint32_t folded = st & ~kShowdownFlag;
+  const float pay = showdown_payoff(up, folded, inv, rkv);
   sc.node_u[t] = pay;
   if (TRACE) trace[t].utility = pay;
 }
@@ -750,6 +1176,34 @@ __global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __
 __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                     const DevGame* __restrict__ tenants, DevScratch sc) {
   __shared__ uint32_t s_w[16];
+  if (lc.es) {
+    // external sampling: k_walk_es appended the records itself; a launch whose
+    // records exceeded the sort input updates no table and is flagged
+    for (uint32_t i = threadIdx.x; i < lc.n_segs; i += blockDim.x) {
+      const DevSeg& S = segs[i];
+      const DevGame& G = tenants[S.tenant];
+      SegCtx c;
+      c.R = G.R;
+      c.S = G.S;
+      c.sig = G.sig;
+      c.tw = (float)S.t_weight;
+      c.num_actions = (int16_t)G.num_actions;
+      c.cfr_plus = (int16_t)(G.use_cfr_plus != 0);
+      sc.seg_ctx[i] = c;
+    }
+    for (uint32_t i = threadIdx.x; i < head1_bytes(sc.cap_lanes) / 4; i += blockDim.x)
+      reinterpret_cast<uint32_t*>(sc.head1)[i] = 0x40404040u;
+    if (threadIdx.x == 0) {
+      const uint32_t n = sc.cnt->n_es;
+      if (n > sc.cap_lanes * (uint32_t)kMaxDepth) sc.cnt->es_overflow = 1u;
+      sc.cnt->n_rec = sc.cnt->es_overflow ? 0u : n;  // sticky until the host has seen it
+      sc.cnt->n_long = 0;
+      sc.cnt->n_pieces = 0;
+      sc.cnt->n_cand = 0;
+      sc.cnt->n_runs = 0;
+    }
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < lc.n_segs; i += blockDim.x) {
     const DevSeg& S = segs[i];
     const DevGame& G = tenants[S.tenant];
@@ -1382,7 +1836,8 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
         bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
         for (uint32_t k = 0; k < run.y; ++k) {
           const uint2 v = s_v[wv][run.x - base + k];
-          const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+          const uint32_t act = v.x & 0xFFu, vm = (v.x >> 8) & kLegalBits;
+          const bool es = (v.x & kEsRecord) != 0;
           const float u = __uint_as_float(v.y);
           if (vm != cur_vm) {
             if (cnt) {
@@ -1401,12 +1856,14 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 #pragma unroll
             for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
           }
-          ++cnt;
+          cnt += (!es || (v.x & kEsHead)) ? 1u : 0u;  // nodes of the stretch
           float q = qv[0];
 #pragma unroll
           for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-          const float rd_sampled = u / q - u;
-          const float rd_other = 0.0f - u;
+          // outcome sampling: u / q - u for the sampled action, -u for the
+          // other legal ones; external sampling: the record's own action only
+          const float rd_sampled = es ? u : u / q - u;
+          const float rd_other = es ? 0.0f : 0.0f - u;
 #pragma unroll
           for (int b = 0; b < ROW; ++b)
             if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
@@ -1459,7 +1916,8 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
         bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
         for (uint32_t k = 0; k < run.y; ++k) {
           const uint2 v = s_v[wv][run.x - base + k];
-          const uint32_t act = v.x & 0xFFu, vm = v.x >> 8;
+          const uint32_t act = v.x & 0xFFu, vm = (v.x >> 8) & kLegalBits;
+          const bool es = (v.x & kEsRecord) != 0;
           const float u = __uint_as_float(v.y);
           if (vm != cur_vm) {
             float rr[ROW];
@@ -1481,12 +1939,14 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 #pragma unroll
             for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
           }
-          ++cnt;
+          cnt += (!es || (v.x & kEsHead)) ? 1u : 0u;  // nodes of the stretch
           float q = qv[0];
 #pragma unroll
           for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-          const float rd_sampled = u / q - u;
-          const float rd_other = 0.0f - u;
+          // outcome sampling: u / q - u for the sampled action, -u for the
+          // other legal ones; external sampling: the record's own action only
+          const float rd_sampled = es ? u : u / q - u;
+          const float rd_other = es ? 0.0f : 0.0f - u;
 #pragma unroll
           for (int b = 0; b < ROW; ++b)
             if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
@@ -1590,7 +2050,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
     float m_sum = 0.0f, m_uni = 0.0f;
 #pragma unroll
     for (int b = 0; b < (int)(kPiece / 64); ++b) {
-      const uint32_t act = v[b].x & 0xFFu, vm = v[b].x >> 8;
+      const uint32_t act = v[b].x & 0xFFu, vm = (v[b].x >> 8) & kLegalBits;
+      const bool es = (v[b].x & kEsRecord) != 0;
       const float u = __uint_as_float(v[b].y);
       if (vm != m_vm) {
         m_vm = vm;
@@ -1606,8 +2067,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
       const float qa = fmaxf(m_sum > 1e-7f ? fmaxf(0.0f, r_act) / m_sum : m_uni, 0.05f);
       const uint32_t k = (uint32_t)(lane + 64 * b);
       s_av[wv][k] = v[b].x;
-      s_rs[wv][k] = u / qa - u;
-      s_u[wv][k] = u;
+      s_rs[wv][k] = es ? u : u / qa - u;
+      s_u[wv][k] = es ? 0.0f : u;
     }
     wave_lds_sync();
     // phase 2: lane a of each group sums action a
@@ -1620,7 +2081,7 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
     bool legal = false;
     for (uint32_t k = (uint32_t)g; k < len; k += kGroups) {
       const uint32_t av = s_av[wv][k];
-      const uint32_t act = av & 0xFFu, vm = av >> 8;
+      const uint32_t act = av & 0xFFu, vm = (av >> 8) & kLegalBits;
       if (vm != cached_vm) {
         cached_vm = vm;
         float t = 0.0f;
@@ -1633,7 +2094,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
       }
       if (legal) {
         dr += (double)((uint32_t)a == act ? s_rs[wv][k] : 0.0f - s_u[wv][k]);
-        ds += (double)(c.tw * sig_a);
+        // an external-sampling node adds its strategy-sum share once (its head record)
+        if (!(av & kEsRecord) || (av & kEsHead)) ds += (double)(c.tw * sig_a);
       }
     }
     double tr = 0.0, ts = 0.0;
@@ -1733,6 +2195,23 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
   if (after_deal) (void)hipEventRecord(after_deal, s);
   DevScratch ws = sc;
   ws.turn6 = in.bt.turn6;
+  if (lc.es) {
+    // external sampling: one lane per traversal, depth first, showdowns settled in the walk
+    hipLaunchKernelGGL(k_es_begin, dim3(1), dim3(64), 0, s, sc);
+    const dim3 grid(lc.lanes / 256);
+#define MCCFR_ES(R, T) \
+  hipLaunchKernelGGL((k_walk_es<R, T>), grid, dim3(256), 0, s, lc, segs, tenants, ws, sc.es_stack, trace)
+    if (row == 8) {
+      if (trace) MCCFR_ES(8, true);
+      else MCCFR_ES(8, false);
+    } else {
+      if (trace) MCCFR_ES(16, true);
+      else MCCFR_ES(16, false);
+    }
+#undef MCCFR_ES
+    if (after_walk) (void)hipEventRecord(after_walk, s);
+    return;
+  }
   if (row == 8)
     walk_dispatch<8>(s, lc, segs, tenants, ws, shape, trace);
   else
@@ -1745,13 +2224,19 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
     hipLaunchKernelGGL(k_showdown<false>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
 }
 
+size_t es_stack_bytes(int row, uint32_t lanes) {
+  const int words = row == 8 ? es_frame_words<8>() : es_frame_words<16>();
+  return (size_t)kEsFrames * (size_t)words * lanes * sizeof(uint32_t);
+}
+
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits) {
   const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
   hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
-  hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
-  // at most lanes * kMaxDepth records; the sort kernels loop over the tiles
-  const uint64_t max_rec = (uint64_t)lc.lanes * kMaxDepth;
+  if (!lc.es) hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
+  // at most lanes * kMaxDepth records (external sampling: up to the sort
+  // input's capacity); the sort kernels loop over the tiles
+  const uint64_t max_rec = (uint64_t)(lc.es ? sc.cap_lanes : lc.lanes) * kMaxDepth;
   const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
   // a multiple of the XCD count (the tile order assumes it)
   const uint32_t sgrid = std::min<uint32_t>((std::max<uint32_t>(max_rec_tiles, 1u) + 7u) / 8u * 8u, 2048u);

@@ -24,6 +24,7 @@ MAX_ACTIONS = 16
 HAND_TABLE_WORDS = 32487834
 
 FOLD, CHECK, CALL, CHECK_CALL, RAISE_MIN, POT_FRAC, ALL_IN = range(7)
+SAMPLE_OUTCOME, SAMPLE_EXTERNAL = 0, 1  # mccfr_game_cfg.sampling
 
 
 # scripted archetypes of the strategy harness (MCCFR_OPP_*, eval.h:5-22)
@@ -63,6 +64,8 @@ class GameCfg(C.Structure):
         ("table_log2", C.c_int32),
         ("seed", C.c_uint64),
         ("menu", Action * MAX_ACTIONS),
+        ("sampling", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
