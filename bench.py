@@ -28,7 +28,9 @@ The same JSON line carries:
   config1       the reference's CPU-runnable case (3 players, 20bb, 2^22
                 slots, launches of 2^10) on the CPU and through the GPU graph;
   large_table   12 launches of 2^22 traversals against 2^28-slot 16-wide
-                tables (the large-table stress at size, fewer traversals).
+                tables (the large-table stress at size, fewer traversals);
+  external_sampling  the headline game with every legal action explored at
+                update-player nodes (launches of 2^18 traversals).
 """
 import argparse
 import ctypes as C
@@ -329,6 +331,58 @@ def serving_bench(mccfr, workloads, hr, n_req, sat_req, budget, max_running):
     return out
 
 
+def es_bench(mccfr, batch_log2=18, table_log2=26, warmup=6, steps=12, table_seed=2026):
+    """External sampling (every legal action at update-player nodes) on the
+    headline game: 6 players, 100bb, the 6-action menu, <= 3 raises per
+    street, 2^26-slot tables, launches of 2^batch_log2 traversals from a
+    seeded random regret state.  The walk's algorithmic bytes sum, over each
+    explored tree, the root record (6N + 1 B), a BF16 snapshot row per
+    sampled (other seats') node, an FP32 regret row per update-player node
+    and a 12-byte record per (update node, legal action)."""
+    cfg = mccfr.cfg_6p_100bb(table_log2=table_log2, seed=0)
+    cfg.sampling = mccfr.SAMPLE_EXTERNAL
+    B = 1 << batch_log2
+    N = cfg.num_players
+    # records: ~20 per traversal here, the sort input holds 20 per lane of max_batch
+    with mccfr.Context(cfg, 4 * B) as ctx:
+        ctx.use_generated_hand_table()
+        ctx.randomize_tables(table_seed)
+        row = ctx.row
+        L = 0
+        for _ in range(warmup):
+            ctx.run_launch(L % N, L // N + 1, L * B, B)
+            L += 1
+        ctx.set_timing(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.run_launch(L % N, L // N + 1, L * B, B)
+            L += 1
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        deal_ms, walk_ms, upd_ms = ctx.kernel_times(steps)
+        ctx.set_timing(False)
+        tr = ctx.run_launch(L % N, L // N + 1, L * B, B, trace=True)
+        recs, long_runs, pieces = ctx.last_launch_stats()
+    d = tr["decisions"].astype(np.float64).sum()
+    u = tr["updates"].astype(np.float64).sum()
+    v = tr["update_legal"].astype(np.float64).sum()
+    walk_bytes = (6.0 * N + 1.0) * B + 2.0 * row * (d - u) + 4.0 * row * u + 12.0 * v
+    tree_bytes = 72.0 * N * B + 2.0 * row * d + u * 4.0 * row + 16.0 * v
+    wms = float(walk_ms.mean())
+    return {"workload": "6max_100bb, external sampling at update-player nodes", "traversals_per_launch": B,
+            "table_slots": 1 << table_log2, "steps": steps, "traversals_per_s": steps * B / dt,
+            "ms_per_launch": dt * 1e3 / steps,
+            "kernel_ms": {"deal": float(deal_ms.mean()), "walk": wms, "table_update": float(upd_ms.mean())},
+            "per_traversal": {"decisions": d / B, "update_nodes": u / B, "records": v / B,
+                              "leaves": float(tr["reserved"].astype(np.float64).mean())},
+            "records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
+            "walk_roofline": {"bound": "hbm", "algorithmic_bytes_per_launch": walk_bytes,
+                              "achieved": walk_bytes / (wms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": walk_bytes / (wms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "tree_bytes_per_launch": tree_bytes, "tree_gbs": tree_bytes / (dt / steps) / 1e9}
+
+
 def large_table_run(mccfr, table_log2, batch_log2, traversals, table_seed):
     """6 players, 200bb (4,000 chips), 10-action menu, <= 4 raises per street,
     2^28-slot tables (2^32 regret + strategy-sum entries, 32 GiB FP32, plus the
@@ -412,11 +466,16 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the reduced large-table leg")
+    ap.add_argument("--no-es", action="store_true", help="skip the external-sampling leg")
     ap.add_argument("--large-leg-launches", type=int, default=12,
                     help="launches of 2^22 traversals in the default run's large-table leg")
     ap.add_argument("--profile", action="store_true", help="headline launches only (for rocprofv3 runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="with --profile: also run a plain device copy of this size (byte-counter calibration)")
+    ap.add_argument("--gather-probe-mib", type=int, default=0,
+                    help="with --profile: also run 2^24 random 16-byte reads over a table of this size")
+    ap.add_argument("--profile-hands", type=int, default=0,
+                    help="with --profile: also run the hand-evaluation microbenchmark on 2^N hands")
     ap.add_argument("--table-seed", type=int, default=2026,
                     help="seed of the synthetic regret state the timed launches start from")
     args = ap.parse_args()
@@ -503,6 +562,13 @@ def main():
         if args.copy_probe_mib:
             result["copy_probe"] = {"bytes": args.copy_probe_mib << 20,
                                     "ms": ctx.copy_probe(args.copy_probe_mib << 20, 2)}
+        if args.gather_probe_mib:
+            reads = 1 << 24
+            result["gather_probe"] = {"table_bytes": args.gather_probe_mib << 20, "reads": reads,
+                                      "bytes_read": 16 * reads, "bytes_written": 4 * reads,
+                                      "ms": ctx.gather_probe(args.gather_probe_mib << 20, reads, 2)}
+        if args.profile_hands:
+            result["hand_eval"] = hand_bench(ctx, args.profile_hands)
         print(json.dumps(result), flush=True)
         return
 
@@ -577,6 +643,12 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: config 1 (3 players, 20bb) on the CPU and the GPU")
         result["config1"] = config1_bench(mccfr, hr, threads)
+
+    if not args.no_es:
+        log("bench: external sampling")
+        result["external_sampling"] = es_bench(mccfr)
+        log(f"bench: external sampling {result['external_sampling']['traversals_per_s'] / 1e6:.1f} M traversals/s "
+            f"({result['external_sampling']['per_traversal']['records']:.1f} records per traversal)")
 
     if not args.no_large:
         log("bench: large table (reduced run)")

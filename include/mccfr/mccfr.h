@@ -298,6 +298,12 @@ int mccfr_last_launch_stats(mccfr_ctx* ctx, uint32_t* records, uint32_t* long_ru
  * (calibration of the HBM byte counters; returns ms per copy). */
 int mccfr_copy_probe(mccfr_ctx* ctx, size_t bytes, int reps, float* ms_per_copy);
 
+/* `reads` random 16-byte row reads from a fresh table of table_bytes bytes (a
+ * power of two), each row's word sum stored as 4 bytes; `reps` passes
+ * (calibration of the HBM byte counters for narrow gathers such as the
+ * walk's strategy rows; returns ms per pass). */
+int mccfr_gather_probe(mccfr_ctx* ctx, size_t table_bytes, size_t reads, int reps, float* ms_per_pass);
+
 /* Writes n hands of 7 distinct uniform cards (rows of 8 bytes, 8th byte 0) to
  * device memory: hand i is the first 7 positions of a Fisher-Yates shuffle
  * driven by draw stream (seed, id = i, tag). */

@@ -147,20 +147,37 @@ __device__ __forceinline__ bool distinct_cards(const uint32_t c[7]) {
   return ok && __popcll(m) == 7;
 }
 
+// kWalkHands hands per lane, walked in lockstep: their dependent gathers are
+// independent of each other, so a lane keeps kWalkHands loads in flight
+// (one hand per lane leaves the walk latency-bound: 7 dependent gathers).
+constexpr int kWalkHands = 4;
 __global__ __launch_bounds__(256) void k_eval7_walk(const int32_t* __restrict__ hr,
                                                     const uint8_t* __restrict__ cards, size_t stride,
                                                     uint16_t* __restrict__ out, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    uint32_t c[7];
-    load_hand(cards, stride, i, c);
-    if (!distinct_cards(c)) {  // codes above 51 or repeated cards have no rank
-      out[i] = 0;
-      continue;
-    }
-    int p = hr[53 + card_rm(c[0])];
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += T * kWalkHands) {
+    uint32_t c[kWalkHands][7];
+    bool ok[kWalkHands];
+    int p[kWalkHands];
 #pragma unroll
-    for (int k = 1; k < 6; ++k) p = hr[p + card_rm(c[k])];
-    out[i] = (uint16_t)decode_packed((uint32_t)hr[p + card_rm(c[6])] & 0xFFFFu);
+    for (int j = 0; j < kWalkHands; ++j) {
+      const size_t i = i0 + (size_t)j * T;
+      ok[j] = false;
+      if (i < n) {
+        load_hand(cards, stride, i, c[j]);
+        ok[j] = distinct_cards(c[j]);  // codes above 51 or repeated cards have no rank
+      }
+      p[j] = ok[j] ? hr[53 + card_rm(c[j][0])] : 0;
+    }
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+#pragma unroll
+      for (int j = 0; j < kWalkHands; ++j) p[j] = ok[j] ? hr[p[j] + card_rm(c[j][k])] : 0;
+#pragma unroll
+    for (int j = 0; j < kWalkHands; ++j) {
+      const size_t i = i0 + (size_t)j * T;
+      if (i < n) out[i] = ok[j] ? (uint16_t)decode_packed((uint32_t)hr[p[j] + card_rm(c[j][6])] & 0xFFFFu) : 0;
+    }
   }
 }
 
@@ -271,6 +288,21 @@ __global__ __launch_bounds__(256) void k_build_buckets(const int32_t* __restrict
   }
 }
 
+// Random 16-byte row reads (byte-counter calibration for narrow gathers):
+// read i takes row splitmix(i) mod rows (rows a power of two) and its sum of
+// words is stored as 4 bytes, so every read is used.
+__global__ __launch_bounds__(256) void k_gather(const uint4* __restrict__ table, uint64_t row_mask,
+                                                uint32_t* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t z = (uint64_t)i + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const uint4 v = table[z & row_mask];
+    out[i] = v.x + v.y + v.z + v.w;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -321,7 +353,8 @@ void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64
 }
 void launch_eval7_walk(hipStream_t s, const int32_t* hr, const uint8_t* cards, size_t stride, uint16_t* out,
                        size_t n) {
-  hipLaunchKernelGGL(k_eval7_walk, dim3(grid_for(n, 256, 1u << 20)), dim3(256), 0, s, hr, cards, stride, out, n);
+  hipLaunchKernelGGL(k_eval7_walk, dim3(grid_for((n + kWalkHands - 1) / kWalkHands, 256, 1u << 20)), dim3(256), 0, s,
+                     hr, cards, stride, out, n);
 }
 void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size_t stride, uint16_t* out,
                         size_t n) {
@@ -356,6 +389,9 @@ void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b, bool
 }
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 256, 1u << 16)), dim3(256), 0, s, src, dst, n16);
+}
+void launch_gather(hipStream_t s, const uint4* table, uint64_t rows, uint32_t* out, size_t n) {
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, table, rows - 1, out, n);
 }
 
 }  // namespace mccfr

@@ -129,5 +129,6 @@ void launch_eval7_arith(hipStream_t s, ArithTables t, const uint8_t* cards, size
 void launch_random_hands(hipStream_t s, uint64_t seed, uint32_t tag, uint8_t* out, size_t n);
 void launch_randomize(hipStream_t s, int row, int num_actions, float* R, float* S, uint64_t slots, uint64_t seed);
 void launch_copy(hipStream_t s, const uint4* src, uint4* dst, size_t n16);
+void launch_gather(hipStream_t s, const uint4* table, uint64_t rows, uint32_t* out, size_t n);
 
 }  // namespace mccfr

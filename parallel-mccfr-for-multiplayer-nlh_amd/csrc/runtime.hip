@@ -1454,6 +1454,36 @@ int mccfr_copy_probe(mccfr_ctx* c, size_t bytes, int reps, float* ms_per_copy) {
   return MCCFR_OK;
 }
 
+int mccfr_gather_probe(mccfr_ctx* c, size_t table_bytes, size_t reads, int reps, float* ms_per_pass) {
+  if (!c || reads == 0 || reps < 1 || table_bytes < 16 || (table_bytes & (table_bytes - 1)))
+    return fail(c, MCCFR_E_ARG, "bad gather probe arguments (table_bytes a power of two >= 16)");
+  void *a = nullptr, *b = nullptr;
+  HIPX(c, hipMalloc(&a, table_bytes));
+  hipError_t e = hipMalloc(&b, reads * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(c, MCCFR_E_NOMEM, "device allocation failed");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  launch_zero(c->stream, a, table_bytes);
+  launch_gather(c->stream, (const uint4*)a, table_bytes / 16, (uint32_t*)b, reads);  // warm-up
+  (void)hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps; ++r) launch_gather(c->stream, (const uint4*)a, table_bytes / 16, (uint32_t*)b, reads);
+  (void)hipEventRecord(e1, c->stream);
+  e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(c, MCCFR_E_HIP, hipGetErrorString(e));
+  if (ms_per_pass) *ms_per_pass = ms / reps;
+  return MCCFR_OK;
+}
+
 // ---- device-memory plumbing for hosts without their own HIP bindings (tests, bench)
 int mccfr_device_alloc(mccfr_ctx* c, size_t bytes, void** out) {
   if (!c || !out) return fail(c, MCCFR_E_ARG, "null argument");

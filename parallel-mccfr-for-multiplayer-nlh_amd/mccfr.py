@@ -164,6 +164,7 @@ _SIGS = {
     "mccfr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mccfr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "mccfr_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
+    "mccfr_gather_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "mccfr_random_hands_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_size_t]),
     "mccfr_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "mccfr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -494,6 +495,11 @@ class Context:
         return float(ms.value)
 
     # device memory -----------------------------------------------------------
+    def gather_probe(self, table_bytes: int, reads: int, reps: int) -> float:
+        ms = C.c_float()
+        self.check(lib().mccfr_gather_probe(self.h, table_bytes, reads, reps, C.byref(ms)))
+        return float(ms.value)
+
     def dalloc(self, nbytes: int) -> int:
         p = C.c_void_p()
         self.check(lib().mccfr_device_alloc(self.h, nbytes, C.byref(p)))
