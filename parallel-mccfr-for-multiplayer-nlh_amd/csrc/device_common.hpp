@@ -94,6 +94,7 @@ struct LaunchCfg {
   uint32_t slot_bits;    // record key = segment << slot_bits | slot
   uint32_t cap_lanes;    // lane capacity of the scratch lists
   uint32_t es;           // 1: the launch's tenants use external sampling (k_walk_es)
+  uint32_t null_key;     // external sampling: key of a null record, above every real key
 };
 
 // Counters the kernels of one launch write (reset by the launch itself).
@@ -103,9 +104,11 @@ struct LaunchCounters {
   uint32_t n_long;       // runs handed to the long-run reduction
   uint32_t n_pieces;     // pieces of the long runs (with n_long: one 64-bit word, bumped together)
   uint32_t n_runs;       // short runs listed for the row-parallel reduction
-  uint32_t n_es;         // external-sampling records appended by the walk (may exceed the capacity)
+  uint32_t n_es;         // external sampling: records reserved by the walk, in whole chunks
   uint32_t es_overflow;  // sticky: 1 records beyond capacity, 2 stack beyond its frames (host clears)
-  uint32_t pad;
+  uint32_t n_sort;       // records the radix passes sort (external sampling: the reservations)
+  uint32_t n_es_valid;   // external sampling: real records among them (the rest are null records)
+  uint32_t pad[2];
 };
 static_assert(offsetof(LaunchCounters, n_long) % 8 == 0 &&
                   offsetof(LaunchCounters, n_pieces) == offsetof(LaunchCounters, n_long) + 4,

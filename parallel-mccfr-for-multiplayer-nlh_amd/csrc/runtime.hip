@@ -314,13 +314,16 @@ int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, const DevScratch& sc, P
     slot_bits = std::max(slot_bits, (uint32_t)t->cfg.table_log2);
   }
   const uint32_t seg_bits = n > 1 ? 32u - (uint32_t)__builtin_clz(n - 1) : 0u;
-  if (slot_bits + seg_bits > 32)
+  // external sampling sorts one bit more: its null records take the top key
+  const uint32_t es_bit = p.lc.es ? 1u : 0u;
+  if (slot_bits + seg_bits + es_bit > 32)
     return fail(c, MCCFR_E_ARG, "too many segments for the tables' slot bits (slot bits + segment bits > 32)");
   p.lc.n_segs = n;
   p.lc.lanes = lanes;
   p.lc.slot_bits = slot_bits;
   p.lc.cap_lanes = sc.cap_lanes;
-  p.key_bits = slot_bits + seg_bits;
+  p.key_bits = slot_bits + seg_bits + es_bit;
+  p.lc.null_key = p.key_bits == 32 ? 0xFFFFFFFFu : (1u << p.key_bits) - 1u;
   return MCCFR_OK;
 }
 

@@ -714,8 +714,15 @@ constexpr int es_frame_words() {
 }
 
 __global__ void k_es_begin(DevScratch sc) {
-  if (threadIdx.x == 0) sc.cnt->n_es = 0;
+  if (threadIdx.x == 0) {
+    sc.cnt->n_es = 0;
+    sc.cnt->n_es_valid = 0;
+  }
 }
+
+// Records of one reservation of k_walk_es: at least the most one iteration of
+// a wave can emit (64 lanes x 16 actions).
+constexpr uint32_t kEsChunk = 1024;
 
 template <int ROW, bool TRACE>
 __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __restrict__ segs,
@@ -855,6 +862,18 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
     ++depth;
   };
 
+  // the wave's current reservation [c_cur, c_end) of the sort input (wave
+  // uniform); kEsDead once the input is full
+  constexpr uint32_t kEsDead = 0xFFFFFFFFu;
+  uint32_t c_cur = 0, c_end = 0, valid = 0;
+  auto close_chunk = [&]() {  // null records in the reservation's unused tail
+    if (c_cur == kEsDead) return;
+    for (uint32_t k = c_cur + (uint32_t)lane; k < c_end; k += 64) {
+      sc.key[0][k] = lc.null_key;
+      sc.val[0][k] = make_uint2(0u, 0u);
+    }
+    c_cur = c_end;
+  };
   while (true) {
     if (__ballot(!done) == 0) break;
     uint32_t emit_k = 0, e_slot = 0, e_vm = 0;
@@ -1055,7 +1074,8 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
       }
     }
     // records of the nodes completed in this iteration: a wave prefix over
-    // the lanes' counts, one atomic for the wave
+    // the lanes' counts, written into the wave's reservation (a new one of
+    // kEsChunk records, one atomic, when this iteration's do not fit)
     uint32_t incl = emit_k;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -1064,20 +1084,28 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
     }
     const uint32_t total = __shfl(incl, 63, 64);
     if (total) {
-      // once past the capacity the counter stops growing (it cannot wrap)
-      // and the launch is flagged; nothing more is written
-      uint32_t wbase = 0;
-      if (lane == 0) {
-        if (__hip_atomic_load(&sc.cnt->n_es, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > cap_rec) {
-          wbase = 0xFFFFFFFFu;
-          sc.cnt->es_overflow = 1u;
+      if (c_cur != kEsDead && c_end - c_cur < total) {
+        close_chunk();
+        // once the input is full the counter stops growing (it cannot wrap)
+        // and the launch is flagged; nothing more is written
+        uint32_t base = 0;
+        if (lane == 0) {
+          if (__hip_atomic_load(&sc.cnt->n_es, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > cap_rec)
+            base = kEsDead;
+          else
+            base = atomicAdd(&sc.cnt->n_es, kEsChunk);
+        }
+        base = __shfl(base, 0, 64);
+        if (base == kEsDead || base + kEsChunk > cap_rec) {
+          if (lane == 0) sc.cnt->es_overflow = 1u;
+          c_cur = c_end = kEsDead;
         } else {
-          wbase = atomicAdd(&sc.cnt->n_es, total);
+          c_cur = base;
+          c_end = base + kEsChunk;
         }
       }
-      wbase = __shfl(wbase, 0, 64);
-      uint32_t off = wbase + incl - emit_k;
-      if (emit_k && wbase != 0xFFFFFFFFu && off + emit_k <= cap_rec) {
+      if (c_cur != kEsDead) {
+        uint32_t off = c_cur + incl - emit_k;
         uint32_t left = e_vm;
         uint32_t head = kEsHead;
 #pragma unroll
@@ -1089,9 +1117,13 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
             ++off;
           }
         }
+        c_cur += total;
       }
+      valid += total;
     }
   }
+  close_chunk();
+  if (lane == 0 && valid) atomicAdd(&sc.cnt->n_es_valid, valid);
 }
 
 // Exclusive prefix of v over a 256-thread workgroup (every thread calls it);
@@ -1194,9 +1226,12 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
     for (uint32_t i = threadIdx.x; i < head1_bytes(sc.cap_lanes) / 4; i += blockDim.x)
       reinterpret_cast<uint32_t*>(sc.head1)[i] = 0x40404040u;
     if (threadIdx.x == 0) {
-      const uint32_t n = sc.cnt->n_es;
-      if (n > sc.cap_lanes * (uint32_t)kMaxDepth) sc.cnt->es_overflow = 1u;
-      sc.cnt->n_rec = sc.cnt->es_overflow ? 0u : n;  // sticky until the host has seen it
+      // the walk reserved n_es records in whole chunks (the tails of which it
+      // filled with null records, keyed above every real key, so they sort
+      // last) and emitted n_es_valid real ones
+      const bool bad = sc.cnt->es_overflow != 0u;  // sticky until the host has seen it
+      sc.cnt->n_sort = bad ? 0u : sc.cnt->n_es;
+      sc.cnt->n_rec = bad ? 0u : sc.cnt->n_es_valid;
       sc.cnt->n_long = 0;
       sc.cnt->n_pieces = 0;
       sc.cnt->n_cand = 0;
@@ -1268,6 +1303,7 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
     reinterpret_cast<uint32_t*>(sc.head1)[i] = 0x40404040u;
   if (threadIdx.x == 0) {
     sc.cnt->n_rec = tot;
+    sc.cnt->n_sort = tot;
     sc.cnt->n_long = 0;
     sc.cnt->n_pieces = 0;
     sc.cnt->n_cand = 0;
@@ -1363,7 +1399,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ hist, uint32_t max_tiles) {
   constexpr uint32_t Bins = 1u << BITS;
   __shared__ uint32_t s_h[Bins];
-  const uint32_t n = cnt->n_rec;
+  const uint32_t n = cnt->n_sort;
   const uint32_t nt = n_tiles_of(n);
   const int tx = threadIdx.x, lane = tx & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -1398,7 +1434,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_hist(const uint32_t* __restrict_
 __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ bin_total,
                                                  const LaunchCounters* __restrict__ cnt, uint32_t max_tiles) {
   __shared__ uint32_t s_w[4];
-  const uint32_t nt = n_tiles_of(cnt->n_rec);
+  const uint32_t nt = n_tiles_of(cnt->n_sort);
   uint32_t* row = hist + (size_t)blockIdx.x * max_tiles;
   uint32_t carry = 0;
   for (uint32_t c0 = 0; c0 < nt; c0 += 1024) {
@@ -1438,7 +1474,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
   __shared__ uint32_t s_k[kSortTile];      // the tile in digit order
   __shared__ uint2 s_v[kSortTile];
   __shared__ uint32_t s_w[4];
-  const uint32_t n = cnt->n_rec;
+  const uint32_t n = cnt->n_sort;
   const uint32_t nt = n_tiles_of(n);
   const int tx = threadIdx.x, lane = tx & 63, w = tx >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
