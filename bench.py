@@ -30,7 +30,7 @@ The same JSON line carries:
   large_table   12 launches of 2^22 traversals against 2^28-slot 16-wide
                 tables (the large-table stress at size, fewer traversals);
   external_sampling  the headline game with every legal action explored at
-                update-player nodes (launches of 2^18 traversals).
+                update-player nodes (launches of 2^20 traversals).
 """
 import argparse
 import ctypes as C
@@ -331,7 +331,7 @@ def serving_bench(mccfr, workloads, hr, n_req, sat_req, budget, max_running):
     return out
 
 
-def es_bench(mccfr, batch_log2=18, table_log2=26, warmup=6, steps=12, table_seed=2026):
+def es_bench(mccfr, batch_log2=20, table_log2=26, warmup=6, steps=12, table_seed=2026):
     """External sampling (every legal action at update-player nodes) on the
     headline game: 6 players, 100bb, the 6-action menu, <= 3 raises per
     street, 2^26-slot tables, launches of 2^batch_log2 traversals from a

@@ -707,6 +707,12 @@ __device__ __forceinline__ float showdown_payoff(int up, uint32_t folded, const 
 // settled where the walk reaches them, since the value is needed at once.
 // ---------------------------------------------------------------------------
 constexpr int kEsFrames = 10;
+// Traversals per lane of the external-sampling walk (a wave's lanes share its
+// 64 * kEsTpl traversals; 64 * kEsTpl <= 256 keeps them in one segment).
+constexpr uint32_t kEsTpl = 4;
+static_assert(64 * kEsTpl <= kSegAlign && kSegAlign % (64 * kEsTpl) == 0, "a wave's traversals lie in one segment");
+// Workgroups of the external-sampling walk for a launch of `lanes` lanes.
+inline uint32_t es_grid(uint32_t lanes) { return (lanes + 256u * kEsTpl - 1u) / (256u * kEsTpl); }
 template <int ROW>
 constexpr int es_frame_words() {
   return 14 + ROW;  // 0..5 seats, 6 seat masks + queue, 7 history, 8 pot, 9 bet, 10 lfr, 11 street | raises |
@@ -735,9 +741,13 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
   __shared__ uint32_t s_menu[WG / 64][MCCFR_MAX_ACTIONS];
   __shared__ uint32_t s_rcp[WG / 64][MCCFR_MAX_ACTIONS];
   const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
-  const uint32_t t = blockIdx.x * WG + (uint32_t)tx;
-  const uint32_t base = uniform(t & ~63u);
+  // the wave's kEsTpl * 64 traversals, one 256-lane block of one segment; its
+  // lanes take them one after another, a lane taking the next as soon as its
+  // tree is done (P: the lane's slot among all the launch's lanes, for its stack)
+  const uint32_t P = blockIdx.x * WG + (uint32_t)tx;
+  const uint32_t base = uniform((P >> 6) * 64u * kEsTpl);
   if (base >= lc.lanes) return;
+  const uint32_t range_end = min(base + 64u * kEsTpl, lc.lanes);
   const uint32_t si = find_seg(segs, lc.n_segs, base);
   const DevSeg& Sg = segs[si];
   const DevGame& G = tenants[uniform(Sg.tenant)];
@@ -781,15 +791,28 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
   const int up = Sg.update_player;
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;   // root records
-  const size_t SL = lc.lanes;      // stack frames
+  const size_t SL = (size_t)gridDim.x * WG;  // stack frames: one stack per lane of the grid
   const uint32_t cap_rec = sc.cap_lanes * (uint32_t)kMaxDepth;
-  const bool real = t - Sg.lane0 < Sg.n;  // padding lanes of the segment's last wave walk nothing
 
-  // root state (as k_walk)
+  uint32_t t = 0;          // the lane's traversal (its lane in the launch's lane space)
   uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
-  int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0;
-  const int dealer = real ? (int)sc.root_dealer[t] : 0;
-  if (real) {
+  int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
+  DrawStream rng;
+  // root state of traversal t (as k_walk)
+  auto start = [&]() {
+    folded = 0;
+    all_in = 0;
+    ab = 0;
+    qmask = 0;
+    phash = 2166136261u;
+    qpos = 0;
+    pot = 0;
+    current_bet = 0;
+    lfr = rules.bb;
+    street = 0;
+    raises = 0;
+    depth = 0;
+    dealer = (int)sc.root_dealer[t];
     const int root_street = G.root_street;
 #pragma unroll
     for (int p = 0; p < kMaxPlayers; ++p) {
@@ -822,17 +845,17 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
       qmask = allmask & ~(folded | all_in);
       qpos = wrap(dealer + 1, N);
     }
-  }
-  DrawStream rng;
-  rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + (t - Sg.lane0), G.tag);
-  rng.k = G.walk_k0;
+    rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + (t - Sg.lane0), G.tag);
+    rng.k = G.walk_k0;
+  };
 
   int sp = 0;            // frames on the stack
-  bool done = !real;     // the traversal's root value is known
+  bool done = true;      // no traversal in hand (its root value is known, or none taken yet)
+  uint32_t next = base;  // the wave's next traversal to hand out (wave uniform)
   bool ret_mode = false; // a value is returning up the tree
   float ret = 0.0f;
   int decisions = 0, updates = 0, upd_legal = 0, leaves = 0, terminal = 0;
-  auto fr = [&](int f, int w) -> uint32_t& { return stack[((size_t)f * FW + w) * SL + t]; };
+  auto fr = [&](int f, int w) -> uint32_t& { return stack[((size_t)f * FW + w) * SL + P]; };
 
   // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585) for `player`
   auto apply = [&](int player, int act) {
@@ -875,7 +898,25 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
     c_cur = c_end;
   };
   while (true) {
-    if (__ballot(!done) == 0) break;
+    // lanes without a traversal take the wave's next ones (padding lanes of
+    // the segment are skipped: they walk nothing)
+    const uint64_t idle = __ballot(done);
+    if (idle && next < range_end) {
+      const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+      if (done && next + rank < range_end) {
+        t = next + rank;
+        if (t - Sg.lane0 < Sg.n) {
+          start();
+          done = false;
+          sp = 0;
+          ret_mode = false;
+          ret = 0.0f;
+          decisions = updates = upd_legal = leaves = terminal = 0;
+        }
+      }
+      next = min(next + (uint32_t)__popcll(idle), range_end);
+    }
+    if (__ballot(!done) == 0 && next >= range_end) break;
     uint32_t emit_k = 0, e_slot = 0, e_vm = 0;
     float rd[ROW];
 #pragma unroll
@@ -2234,7 +2275,7 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
   if (lc.es) {
     // external sampling: one lane per traversal, depth first, showdowns settled in the walk
     hipLaunchKernelGGL(k_es_begin, dim3(1), dim3(64), 0, s, sc);
-    const dim3 grid(lc.lanes / 256);
+    const dim3 grid(es_grid(lc.lanes));
 #define MCCFR_ES(R, T) \
   hipLaunchKernelGGL((k_walk_es<R, T>), grid, dim3(256), 0, s, lc, segs, tenants, ws, sc.es_stack, trace)
     if (row == 8) {
@@ -2262,7 +2303,7 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
 
 size_t es_stack_bytes(int row, uint32_t lanes) {
   const int words = row == 8 ? es_frame_words<8>() : es_frame_words<16>();
-  return (size_t)kEsFrames * (size_t)words * lanes * sizeof(uint32_t);
+  return (size_t)kEsFrames * (size_t)words * es_grid(lanes) * 256u * sizeof(uint32_t);
 }
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
