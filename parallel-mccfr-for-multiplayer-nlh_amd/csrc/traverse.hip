@@ -693,7 +693,8 @@ __device__ __forceinline__ float showdown_payoff(int up, uint32_t folded, const 
 // every legal action in action order.  Such a node pushes a frame (the state
 // before its action, its slot and legal mask, the actions left and the value
 // of each explored child) onto the lane's stack in HBM, laid out
-// [frame][word][lane] so that a wave's frame accesses are coalesced; a leaf's
+// [lane][frame][word] (16-byte accesses; a wave's lanes sit at different
+// depths, so lane-interleaved words would not coalesce either); a leaf's
 // value returns into the top frame, which restores its state for the next
 // action or, with none left, forms v(I) = sum_a sigma(a) v(a) (sigma from the
 // FP32 regrets, as the reference's update node forms it) and emits a record
@@ -715,8 +716,8 @@ static_assert(64 * kEsTpl <= kSegAlign && kSegAlign % (64 * kEsTpl) == 0, "a wav
 inline uint32_t es_grid(uint32_t lanes) { return (lanes + 256u * kEsTpl - 1u) / (256u * kEsTpl); }
 template <int ROW>
 constexpr int es_frame_words() {
-  return 14 + ROW;  // 0..5 seats, 6 seat masks + queue, 7 history, 8 pot, 9 bet, 10 lfr, 11 street | raises |
-                    // depth, 12 slot, 13 legal | left << 16, 14.. child values
+  return 16 + ROW;  // 0..5 seats, 6 seat masks + queue, 7 history, 8 pot, 9 bet, 10 lfr, 11 street | raises |
+                    // depth, 12 slot, 13 legal | left << 16, 14..15 unused, 16.. child values
 }
 
 __global__ void k_es_begin(DevScratch sc) {
@@ -791,7 +792,6 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
   const int up = Sg.update_player;
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;   // root records
-  const size_t SL = (size_t)gridDim.x * WG;  // stack frames: one stack per lane of the grid
   const uint32_t cap_rec = sc.cap_lanes * (uint32_t)kMaxDepth;
 
   uint32_t t = 0;          // the lane's traversal (its lane in the launch's lane space)
@@ -855,7 +855,11 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
   bool ret_mode = false; // a value is returning up the tree
   float ret = 0.0f;
   int decisions = 0, updates = 0, upd_legal = 0, leaves = 0, terminal = 0;
-  auto fr = [&](int f, int w) -> uint32_t& { return stack[((size_t)f * FW + w) * SL + P]; };
+  // a lane's frames lie together, [lane][frame][word] (the lanes of a wave sit
+  // at different depths, so per-lane runs of 16-byte accesses beat
+  // lane-interleaved words)
+  auto fr = [&](int f, int w) -> uint32_t& { return stack[((size_t)P * kEsFrames + f) * FW + w]; };
+  auto fr4 = [&](int f, int q) -> uint4& { return reinterpret_cast<uint4*>(&fr(f, 0))[q]; };
 
   // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585) for `player`
   auto apply = [&](int player, int act) {
@@ -918,6 +922,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
     }
     if (__ballot(!done) == 0 && next >= range_end) break;
     uint32_t emit_k = 0, e_slot = 0, e_vm = 0;
+    int ap_player = -1, ap_act = 0;  // the action both paths apply, once, below
     float rd[ROW];
 #pragma unroll
     for (int a = 0; a < ROW; ++a) rd[a] = 0.0f;
@@ -988,18 +993,15 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           // push the node: its state before the action, slot, legal actions
           ++updates;
           upd_legal += __popc(vm);
+          uint32_t w[6];
 #pragma unroll
-          for (int p = 0; p < kMaxPlayers; ++p)
-            if (p < N) fr(sp, p) = sb[p * WG];
-          fr(sp, 6) = folded | (all_in << 8) | (qmask << 16) | ((uint32_t)qpos << 24);
-          fr(sp, 7) = ab;
-          fr(sp, 8) = (uint32_t)pot;
-          fr(sp, 9) = (uint32_t)current_bet;
-          fr(sp, 10) = (uint32_t)lfr;
-          fr(sp, 11) = (uint32_t)street | ((uint32_t)raises << 8) | ((uint32_t)depth << 16);
-          fr(sp, 12) = slot;
+          for (int p = 0; p < kMaxPlayers; ++p) w[p] = p < N ? sb[p * WG] : 0u;
+          fr4(sp, 0) = make_uint4(w[0], w[1], w[2], w[3]);
+          fr4(sp, 1) = make_uint4(w[4], w[5], folded | (all_in << 8) | (qmask << 16) | ((uint32_t)qpos << 24), ab);
+          fr4(sp, 2) = make_uint4((uint32_t)pot, (uint32_t)current_bet, (uint32_t)lfr,
+                                  (uint32_t)street | ((uint32_t)raises << 8) | ((uint32_t)depth << 16));
           act = __builtin_ctz(vm);
-          fr(sp, 13) = vm | ((vm & (vm - 1u)) << 16);
+          fr4(sp, 3) = make_uint4(slot, vm | ((vm & (vm - 1u)) << 16), 0u, 0u);
           ++sp;
         } else {
           // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot, as k_walk
@@ -1032,7 +1034,8 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         }
         if (act >= 0) {
           if (TRACE) phash = fnv_path(phash, slot, (uint32_t)act);
-          apply(player, act);
+          ap_player = player;
+          ap_act = act;
         }
       }
       if (leaf) {
@@ -1056,36 +1059,40 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         }
       } else {
         const int f = sp - 1;
-        const uint32_t w13 = fr(f, 13);
+        const uint4 q3 = fr4(f, 3);
+        const uint32_t w13 = q3.y;
         const uint32_t vm = w13 & 0xFFFFu, left = w13 >> 16;
         const int cur = 31 - (int)__builtin_clz(vm & ~left);  // explored in action order
-        fr(f, 14 + cur) = __float_as_uint(ret);
+        fr(f, 16 + cur) = __float_as_uint(ret);
         if (left) {
           // the node's next action, from its saved state
           const int a = __builtin_ctz(left);
           fr(f, 13) = vm | ((left & (left - 1u)) << 16);
+          const uint4 q0 = fr4(f, 0), q1 = fr4(f, 1), q2 = fr4(f, 2);
+          const uint32_t w[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
 #pragma unroll
           for (int p = 0; p < kMaxPlayers; ++p)
-            if (p < N) sb[p * WG] = fr(f, p);
-          const uint32_t w6 = fr(f, 6);
+            if (p < N) sb[p * WG] = w[p];
+          const uint32_t w6 = q1.z;
           folded = w6 & 0xFFu;
           all_in = (w6 >> 8) & 0xFFu;
           qmask = (w6 >> 16) & 0xFFu;
           qpos = (int)(w6 >> 24);
-          ab = fr(f, 7);
-          pot = (int)fr(f, 8);
-          current_bet = (int)fr(f, 9);
-          lfr = (int)fr(f, 10);
-          const uint32_t w11 = fr(f, 11);
+          ab = q1.w;
+          pot = (int)q2.x;
+          current_bet = (int)q2.y;
+          lfr = (int)q2.z;
+          const uint32_t w11 = q2.w;
           street = (int)(w11 & 0xFFu);
           raises = (int)((w11 >> 8) & 0xFFu);
           depth = (int)(w11 >> 16);
-          if (TRACE) phash = fnv_path(phash, fr(f, 12), (uint32_t)a);
-          apply(up, a);
+          if (TRACE) phash = fnv_path(phash, q3.x, (uint32_t)a);
+          ap_player = up;
+          ap_act = a;
           ret_mode = false;
         } else {
           // every action explored: v(I) and the node's records
-          const uint32_t slot = fr(f, 12);
+          const uint32_t slot = q3.x;
           float rr[ROW], v[ROW];
           load_f32_row<ROW>(G.R + (size_t)slot * 2 * ROW, rr);
           float sum_r = 0.0f;
@@ -1094,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
             const bool legal = a < A && (vm >> a & 1u);
             rr[a] = legal ? fmaxf(0.0f, rr[a]) : 0.0f;
             sum_r += rr[a];
-            v[a] = legal ? __uint_as_float(fr(f, 14 + a)) : 0.0f;
+            v[a] = legal ? __uint_as_float(fr(f, 16 + a)) : 0.0f;
           }
           const float uni = 1.0f / (float)__popc(vm);
           float vi = 0.0f;
@@ -1114,6 +1121,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         }
       }
     }
+    if (ap_player >= 0) apply(ap_player, ap_act);
     // records of the nodes completed in this iteration: a wave prefix over
     // the lanes' counts, written into the wave's reservation (a new one of
     // kEsChunk records, one atomic, when this iteration's do not fit)
