@@ -476,10 +476,15 @@ def main():
                     help="with --profile: also run 2^24 random 16-byte reads over a table of this size")
     ap.add_argument("--profile-hands", type=int, default=0,
                     help="with --profile: also run the hand-evaluation microbenchmark on 2^N hands")
+    ap.add_argument("--walk-shape", default="",
+                    help="with --profile: the walk's launch shape as WG:TPL (workgroup lanes : traversals "
+                         "per lane, 0 = persistent waves); default the library's 256:1")
     ap.add_argument("--table-seed", type=int, default=2026,
                     help="seed of the synthetic regret state the timed launches start from")
     args = ap.parse_args()
 
+    if args.walk_shape and not args.profile:
+        sys.exit("bench.py: --walk-shape is for --profile runs (the timed line uses the default shape)")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU delta exchange is design-only)", file=sys.stderr)
@@ -500,6 +505,9 @@ def main():
         f"steps {args.steps}")
     ctx = mccfr.Context(cfg, B)
     ctx.use_generated_hand_table()
+    if args.walk_shape:
+        wg, tpl = (int(x) for x in args.walk_shape.split(":"))
+        ctx.set_launch_shape(wg, tpl)
     hr = mccfr.generate_hand_table()
     state = {"launch": 0, "tid": 0}
 

@@ -20,8 +20,9 @@
 //                 (key = segment << slot_bits | slot), the key bits spread
 //                 evenly over ceil(key_bits / 10) passes.
 //   k_runs        every run of equal keys is one table slot of one tenant;
-//                 runs are listed (k_reduce_cand finds the ends of the long
-//                 ones).  k_reduce_rows: one lane owns a short run and sums
+//                 runs are listed (reduce_cand, in k_reduce_rows' leading
+//                 workgroups, finds the ends of the long ones).
+//                 k_reduce_rows: one lane owns a short run and sums
 //                 every action's contributions (cfr_gpu.cu:784-837) in FP64
 //                 in record order, then applies them: regrets += sum (CFR+
 //                 clamp), strategy sums += sum, BF16 snapshot re-matched.
@@ -363,36 +364,45 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
 
-  // Each lane walks the traversals t_j = base + j * jstride + lane, j < tpl, one
-  // after another.  A lane whose traversal has ended waits until kRefillLanes
-  // lanes that have a next traversal are waiting, or until no lane walks; then
-  // every waiting lane finishes its traversal (showdown, stores) and starts
-  // its next one.  The finishing code runs for many lanes at once, and lanes
-  // are not held idle until the longest traversal of the wave has ended.
+  // The wave's traversals form one pool, t(k) = base + (k / 64) * jstride +
+  // k % 64 for k < 64 * tpl, taken in order of k by whichever lanes are free.
+  // A lane whose traversal has ended waits until kRefillLanes lanes (or the
+  // pool's remaining traversals) are waiting, or until no lane walks; then
+  // every waiting lane finishes its traversal (showdown, stores) and takes
+  // the pool's next one.  The finishing code runs for many lanes at once, and
+  // a lane is idle only once the pool is empty.  A traversal's result depends
+  // on t alone (its draws, deal and output slots), not on the lane.
   DrawStream rng;
   uint32_t folded = 0, all_in = 0, ab = 0, qmask = 0, phash = 2166136261u;
   int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
   int npend = 0, decisions = 0, upd_legal = 0;
   int status = 5;  // 0 walking, 1 fold, 2 showdown, 3 depth limit, 5 waiting to start, 6 no traversal left
   float u = 0.0f;
-  uint32_t t = 0, jn = 0;
-  // this lane's slots that are a segment's padding lanes hold no records
+  uint32_t t = 0;
+  // the pool's traversals in the segment (t(k) grows with k, so they are the
+  // first pool_n); the slots that are a segment's padding lanes hold no records
+  uint32_t pool_n = 0;
   for (uint32_t j = 0; j < tpl; ++j) {
     const uint32_t tt = base + j * jstride + (uint32_t)lane;
-    if (tt < lc.lanes && tt - lane0 >= seg_n) {
+    const bool in_seg = tt - lane0 < seg_n;
+    pool_n += (uint32_t)__popcll(__ballot(in_seg));
+    if (tt < lc.lanes && !in_seg) {
       sc.node_count[tt] = 0;
       sc.sd_state[tt] = 0;
     }
   }
+  pool_n = uniform(pool_n);
+  uint32_t pool_next = 0;  // wave-uniform
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
 
   while (true) {
-    const uint32_t t_next = base + jn * jstride + (uint32_t)lane;
-    const bool has_next = jn < tpl && t_next - lane0 < seg_n;
     const uint64_t walking = __ballot(status == 0);
     const uint64_t waiting = __ballot(status != 0 && status != 6);
-    const uint64_t ready = __ballot(status != 0 && status != 6 && has_next);
     if (waiting == 0 && walking == 0) break;
-    if (waiting != 0 && (walking == 0 || __popcll(ready) >= kRefillLanes)) {
+    const uint32_t left = pool_n - pool_next;
+    const uint32_t n_wait = (uint32_t)__popcll(waiting);
+    if (waiting != 0 &&
+        (walking == 0 || (left != 0 && min(n_wait, left) >= min((uint32_t)kRefillLanes, left)))) {
       // The tenant's root and stack fields are read here, where they are used,
       // rather than kept in scalar registers through the walk (the compiler
       // barrier stops their loads from being hoisted out of the loop).
@@ -426,11 +436,12 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
           sc.node_av[(size_t)k * L + t] = pa[k * WG];
         }
       }
+      const uint32_t k = pool_next + (uint32_t)__popcll(waiting & lanes_below);
+      pool_next = uniform(min(pool_n, pool_next + n_wait));
       if (status != 0 && status != 6) {
         status = 6;
-        if (has_next) {
-          t = t_next;
-          ++jn;
+        if (k < pool_n) {
+          t = base + (k >> 6) * jstride + (k & 63u);
           folded = 0;
           all_in = 0;
           ab = 0;
@@ -1645,7 +1656,7 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // run that ends inside its wave or among the next kShortRun records, with at
 // most kShortRun records, is listed in the wave's region for k_reduce_rows;
 // the longer ones (at most 4 per wave) in the wave's region for
-// k_reduce_cand.  Plain stores, no atomics: every region has one writer.
+// reduce_cand.  Plain stores, no atomics: every region has one writer.
 __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const int lane = threadIdx.x & 63;
@@ -1728,12 +1739,15 @@ __device__ __forceinline__ uint32_t first_head64(const uint8_t* __restrict__ bas
 // with one 64-bit atomic (the long-run and piece counters share a word); a
 // lane writes the pieces of its runs of up to 4 pieces, the wave those of
 // longer runs.
-__global__ __launch_bounds__(256) void k_reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys) {
+// Runs as the leading `nblk` workgroups of k_reduce_rows (the two depend only
+// on k_runs), `blk` being the workgroup's index among them.
+__device__ __forceinline__ void reduce_cand(DevScratch sc, const uint32_t* __restrict__ keys, uint32_t blk,
+                                            uint32_t nblk) {
   const uint32_t n = sc.cnt->n_rec;
   const uint32_t n_waves = (n + 63) / 64;
   const int lane = threadIdx.x & 63;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w0 < n_waves; w0 += stride) {
+  const uint32_t stride = nblk * blockDim.x;
+  for (uint32_t w0 = (blk * blockDim.x + threadIdx.x) & ~63u; w0 < n_waves; w0 += stride) {
     const uint32_t w = w0 + (uint32_t)lane;
     const uint32_t nc = w < n_waves ? sc.n_cand[w] : 0u;
     // the region's 4 candidate slots, loaded beside their count (slots past
@@ -1872,9 +1886,17 @@ __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, ui
 // are latency-bound, not bandwidth-bound.  8-wide rows are held in registers
 // through the record loop (96 registers, 5 waves per SIMD; variants with 6
 // waves measured the same); 16-wide rows are re-read (below).
+// The first `cand_blocks` workgroups find the long runs' ends and pieces
+// instead (reduce_cand): that latency-bound search overlaps the short runs'
+// updates rather than taking a launch of its own.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
-                                                     const uint2* __restrict__ vals) {
+                                                     const uint2* __restrict__ vals, uint32_t cand_blocks) {
+  if (blockIdx.x < cand_blocks) {
+    reduce_cand(sc, keys, blockIdx.x, cand_blocks);
+    return;
+  }
+  const uint32_t blk = blockIdx.x - cand_blocks;
   constexpr int kWaves = 256 / 64;
   // runs of a pair of regions start among its 128 records and hold at most kShortRun
   constexpr uint32_t kSpan = 2 * 64 + kShortRun;
@@ -1888,8 +1910,8 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
   SegCtx c0{};
   if (one_seg) c0 = sc.seg_ctx[0];
   const uint32_t slot_mask = (uint32_t)((1ull << lc.slot_bits) - 1ull);
-  const uint32_t wstride = gridDim.x * kWaves;
-  for (uint32_t pw = blockIdx.x * kWaves + wv; pw < n_pairs; pw += wstride) {
+  const uint32_t wstride = (gridDim.x - cand_blocks) * kWaves;
+  for (uint32_t pw = blk * kWaves + wv; pw < n_pairs; pw += wstride) {
     const uint32_t r0 = 2 * pw, base = r0 * 64u;
     for (uint32_t k = lane; k < kSpan; k += 64) s_v[wv][k] = base + k < n ? vals[base + k] : make_uint2(0u, 0u);
     const uint32_t nr0 = sc.n_runs[r0];
@@ -2351,10 +2373,12 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint2* vals = sc.val[passes & 1];
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
   hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
-  hipLaunchKernelGGL(k_reduce_cand, dim3(rgrid / 64 + 1), dim3(256), 0, s, sc, keys);
-#define MCCFR_REDUCE(R)                                                                                \
-  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);              \
-  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);             \
+  // the long-run search's workgroups lead the short runs' grid
+  const uint32_t cand_blocks = rgrid / 64 + 1;
+#define MCCFR_REDUCE(R)                                                                                 \
+  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(cand_blocks + 8192), dim3(256), 0, s, lc, sc, keys, vals,   \
+                     cand_blocks);                                                                      \
+  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);          \
   hipLaunchKernelGGL(k_reduce_runs<R>, dim3(2048), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
     MCCFR_REDUCE(8);
