@@ -2373,13 +2373,22 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint2* vals = sc.val[passes & 1];
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
   hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
-  // the long-run search's workgroups lead the short runs' grid
+  // the long-run search's workgroups lead the short runs' grid.  The reduce
+  // grids (4 waves per workgroup, all grid-stride) are sized from the record
+  // bound, so a small launch does not dispatch thousands of empty workgroups
   const uint32_t cand_blocks = rgrid / 64 + 1;
-#define MCCFR_REDUCE(R)                                                                                 \
-  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(cand_blocks + 8192), dim3(256), 0, s, lc, sc, keys, vals,   \
-                     cand_blocks);                                                                      \
-  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(8192), dim3(256), 0, s, lc, sc, keys, vals);          \
-  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(2048), dim3(256), 0, s, lc, sc, keys)
+  const auto wg4 = [](uint64_t items, uint32_t cap) {
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>((items + 3) / 4, 1), cap);
+  };
+  const uint64_t max_long = max_rec / (kShortRun + 1) + 1;
+  const uint32_t rows_grid = wg4((max_rec + 127) / 128, 8192);  // a wave per pair of 64-record regions
+  const uint32_t piece_grid = wg4(max_rec / kPiece + max_long, 8192);
+  const uint32_t runs_grid = wg4(max_long, 2048);
+#define MCCFR_REDUCE(R)                                                                                    \
+  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(cand_blocks + rows_grid), dim3(256), 0, s, lc, sc, keys, vals, \
+                     cand_blocks);                                                                         \
+  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(piece_grid), dim3(256), 0, s, lc, sc, keys, vals);           \
+  hipLaunchKernelGGL(k_reduce_runs<R>, dim3(runs_grid), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
     MCCFR_REDUCE(8);
   } else {

@@ -2,6 +2,9 @@
 reference's size and royal-flush checks (hand_eval.cpp:59-86,
 cfr_gpu.cu:270-299) plus a range check on every word, all before any device
 work.  No GPU: these entry points are host code."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -88,3 +91,41 @@ def test_file_checks(tmp_path, table):
     assert mccfr.validate_hand_table_file(str(corrupt)) == mccfr.E_TABLE
     assert mccfr.validate_hand_table_file(str(tmp_path / "missing.dat")) == mccfr.E_IO
     assert mccfr.lib().mccfr_validate_hand_table_file(None) == mccfr.E_ARG
+
+
+GEN_TABLE = os.path.join(os.path.dirname(mccfr.LIB_PATH), "gen_table")
+
+
+def run_gen_table(path):
+    if not os.path.exists(GEN_TABLE):
+        pytest.fail(f"{GEN_TABLE} is missing: build with `make -C {os.path.dirname(GEN_TABLE)}`")
+    return subprocess.run([GEN_TABLE, str(path)], capture_output=True, text=True, timeout=300)
+
+
+def test_gen_table_writes_the_generated_table(tmp_path, table):
+    """The build-time generator (`gen_table [output_path]`, the reference's
+    tools/gen_lookup_table.cpp:63-66 invocation) writes the same words as
+    the in-memory generator; a valid file is kept; a file that fails the
+    load checks is written again."""
+    out = tmp_path / "data" / "handranks.dat"
+    out.parent.mkdir()
+    r = run_gen_table(out)
+    assert r.returncode == 0, r.stderr
+    assert "wrote" in r.stdout
+    words = np.fromfile(out, dtype="<i4")
+    assert words.size == WORDS and np.array_equal(words, table)
+    assert mccfr.validate_hand_table_file(str(out)) == 0
+    r = run_gen_table(out)
+    assert r.returncode == 0 and "present and valid" in r.stdout
+    words[royal_word_index(words) - 5] = WORDS + 7  # a state word outside the table
+    words.tofile(out)
+    assert mccfr.validate_hand_table_file(str(out)) == mccfr.E_TABLE
+    r = run_gen_table(out)
+    assert r.returncode == 0 and "wrote" in r.stdout, r.stderr
+    assert np.array_equal(np.fromfile(out, dtype="<i4"), table)
+    assert not os.path.exists(str(out) + ".tmp")
+
+
+def test_gen_table_reports_an_unwritable_path(tmp_path):
+    r = run_gen_table(tmp_path / "no_such_dir" / "handranks.dat")
+    assert r.returncode == 1 and "cannot create" in r.stderr
