@@ -476,6 +476,8 @@ def main():
                     help="with --profile: also run 2^24 random 16-byte reads over a table of this size")
     ap.add_argument("--profile-hands", type=int, default=0,
                     help="with --profile: also run the hand-evaluation microbenchmark on 2^N hands")
+    ap.add_argument("--profile-es", type=int, default=0,
+                    help="with --profile: also run this many external-sampling launches of 2^20 traversals")
     ap.add_argument("--walk-shape", default="",
                     help="with --profile: the walk's launch shape as WG:TPL (workgroup lanes : traversals "
                          "per lane, 0 = persistent waves); default the library's 256:1")
@@ -577,6 +579,9 @@ def main():
                                       "ms": ctx.gather_probe(args.gather_probe_mib << 20, reads, 2)}
         if args.profile_hands:
             result["hand_eval"] = hand_bench(ctx, args.profile_hands)
+        if args.profile_es:
+            result["external_sampling"] = es_bench(mccfr, warmup=2, steps=args.profile_es,
+                                                   table_seed=args.table_seed)
         print(json.dumps(result), flush=True)
         return
 

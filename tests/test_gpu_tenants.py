@@ -226,6 +226,28 @@ def test_results_do_not_depend_on_launch_shape(hand_table):
         assert np.array_equal(S, out[0][1])
 
 
+def test_persistent_pool_equals_one_traversal_per_lane(hand_table):
+    """The persistent shape at 2^20 traversals: one workgroup per resident slot,
+    so each wave's pool holds 2-3 traversals per lane (the last round is only
+    partly filled) and finished lanes take the pool's next traversal.  Every
+    lane's record and both tables equal the default shape's."""
+    cfg = mccfr.cfg_6p_100bb(table_log2=16, seed=4)
+    n = 1 << 20
+    got = []
+    for wg, tpl in ((256, 1), (256, 0), (64, 0)):
+        with mccfr.Context(cfg, n) as ctx:
+            ctx.set_hand_table(hand_table)
+            ctx.set_launch_shape(wg, tpl)
+            ctx.randomize_tables(6)
+            ctx.run_launch(0, 1, 0, n)
+            tr = ctx.run_launch(1, 1, n, n, trace=True)
+            got.append((tr, ctx.download()[:2]))
+    tr0, (R0, S0) = got[0]
+    for tr, (R, S) in got[1:]:
+        assert np.array_equal(tr, tr0)
+        assert np.array_equal(R, R0) and np.array_equal(S, S0)
+
+
 def test_autotune_picks_a_shape(hand_table):
     games, counts = workloads.uneven_mix(seed=2, n_games=8, table_log2=14, traversals=1 << 15)
     with mccfr.Context(small_ref(), 1 << 16) as ctx:
