@@ -1053,7 +1053,15 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         ++leaves;
         ret_mode = true;
       }
-    } else if (!done) {
+    }
+    // A leaf's value returns in the iteration that reached it (the forward
+    // and return paths run side by side in a wave anyway; a leaf applies no
+    // action).  A completed node's value goes on into its parent in the same
+    // iteration when that emits nothing (the parent has an action left, or the
+    // tree is done): each iteration applies at most one action and emits at
+    // most one node's records.
+#pragma unroll 1
+    for (int pass = 0; pass < 2 && !done && ret_mode; ++pass) {
       // the value `ret` returns into the top frame
       if (sp == 0) {
         done = true;
@@ -1073,6 +1081,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         const uint4 q3 = fr4(f, 3);
         const uint32_t w13 = q3.y;
         const uint32_t vm = w13 & 0xFFFFu, left = w13 >> 16;
+        if (pass == 1 && !left) break;  // a second completion waits for the next iteration
         const int cur = 31 - (int)__builtin_clz(vm & ~left);  // explored in action order
         fr(f, 16 + cur) = __float_as_uint(ret);
         if (left) {
