@@ -177,3 +177,27 @@ def test_eval_mode_on_strategy_and_checkpoint(tmp_path, hand_table):
              "--table-log2", 15, "--handranks", "generated")
     assert r3.returncode == 0, r3.stderr[-2000:]
     assert "=== Eval Results (9-player) ===" in r3.stdout
+
+
+@pytest.mark.gpu
+def test_train_from_the_build_time_table_file(tmp_path):
+    """The file gen_table writes (the build's data/handranks.dat) trains
+    exactly as the in-memory table does: same checkpoint and strategy bytes;
+    the default --handranks path is relative to the working directory, as in
+    the reference (main.cpp)."""
+    gen = os.path.join(os.path.dirname(mccfr.LIB_PATH), "gen_table")
+    data = tmp_path / "data"
+    data.mkdir()
+    r = subprocess.run([gen, str(data / "handranks.dat")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    common = ["--players", 3, "--batch", 4096, "--table-log2", 15, "--iters", 2]
+    print("poker_mccfr train from the table file and from the generated table", flush=True)
+    a = tmp_path / "file.bin"
+    r = run(*common, "--save", a, cwd=tmp_path)  # default: data/handranks.dat
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "NOTE:" not in r.stderr  # the file was found, no fallback
+    b = tmp_path / "gen.bin"
+    r = run(*common, "--save", b, "--handranks", "generated", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert open(a, "rb").read() == open(b, "rb").read()
+    assert open(str(a) + ".ckpt", "rb").read() == open(str(b) + ".ckpt", "rb").read()
