@@ -36,6 +36,13 @@
 //       brute-force evaluator; the first-launch closed form (all-zero tables:
 //       passive fallback everywhere, no draws); chip conservation; the
 //       sum_a sigma(a)*rdelta(a) = 0 invariant.
+//   orc_replay_launch with sampling = external (every legal action at the
+//   update player's nodes, values returned up the tree)
+//       brute force of the heads-up 60-chip game on the 8-action menu: the
+//       whole tree enumerated, exact counterfactual values and expected
+//       contributions; equal per traversal against one-hot opponents, equal
+//       in the mean over many traversals against mixed ones; the
+//       sum_a sigma(a)*rdelta(a) = 0 invariant per update node.
 //   orc_cpu_baseline      same contributions as orc_replay_launch (test).
 // Parity unpinned: cuRAND streams and __float2bfloat16 (replaced by the
 // project's own Philox and RNE definitions), and eval5/eval6 against the
