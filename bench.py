@@ -343,7 +343,8 @@ def es_bench(mccfr, batch_log2=20, table_log2=26, warmup=6, steps=12, table_seed
     cfg.sampling = mccfr.SAMPLE_EXTERNAL
     B = 1 << batch_log2
     N = cfg.num_players
-    # records: ~20 per traversal here, the sort input holds 20 per lane of max_batch
+    # one record per update-player node, ~22 per traversal here (more as the
+    # strategy sharpens); the sort input holds 20 per lane of max_batch
     with mccfr.Context(cfg, 4 * B) as ctx:
         ctx.use_generated_hand_table()
         ctx.randomize_tables(table_seed)

@@ -56,6 +56,7 @@ struct DevScratch {
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
   LaunchCounters* cnt;
   uint32_t* es_stack;    // external-sampling frames [kEsFrames][words][launch lanes] (not in the arena)
+  float* es_rd;          // external sampling: [sort input][row] regret deltas of each node record (not in the arena)
   uint32_t cap_lanes;
   uint32_t max_tiles;
 };

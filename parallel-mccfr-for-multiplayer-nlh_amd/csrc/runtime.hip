@@ -202,7 +202,7 @@ void free_all(mccfr_ctx* c) {
     if (t.used && t.owned) (void)hipFree(t.owned);
   void* ptrs[] = {c->d_tenants, c->d_segs, c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets,
                   c->dtrace, c->buckets.flop5, c->buckets.turn6, c->eval_bk.flop5, c->eval_bk.turn6,
-                  c->es_stack};
+                  c->es_stack, c->es_rd};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -222,25 +222,32 @@ int rm_sweep_if_dirty(mccfr_ctx* c, Tenant& t) {
   return MCCFR_OK;
 }
 
-// Frames of the external-sampling walk for a launch of `lanes` lanes (grown,
-// never shrunk; a captured graph holding the old frames is dropped).
+// Frames of the external-sampling walk for a launch of `lanes` lanes, and a
+// regret-delta row for every record the sort input holds (both grown, never
+// shrunk; a captured graph holding the old buffers is dropped).
 int ensure_es_stack(mccfr_ctx* c, int row, uint32_t lanes) {
-  const size_t want = es_stack_bytes(row, lanes);
-  if (want > c->es_stack_bytes) {
+  const auto grow = [&](auto*& buf, size_t& have, size_t want, const char* what) -> int {
+    if (want <= have) return MCCFR_OK;
     HIPX(c, hipStreamSynchronize(c->stream));
     drop_graph(c);
-    if (c->es_stack) (void)hipFree(c->es_stack);
-    c->es_stack = nullptr;
-    c->es_stack_bytes = 0;
-    const hipError_t e = hipMalloc(&c->es_stack, want);
-    if (e != hipSuccess) {
-      c->es_stack = nullptr;
-      return fail(c, MCCFR_E_NOMEM, std::string("external-sampling frames: ") + hipGetErrorString(e));
-    }
-    c->es_stack_bytes = want;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    have = 0;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return fail(c, MCCFR_E_NOMEM, std::string(what) + hipGetErrorString(e));
+    buf = static_cast<std::remove_reference_t<decltype(buf)>>(p);
+    have = want;
+    return MCCFR_OK;
+  };
+  int rc = grow(c->es_stack, c->es_stack_bytes, es_stack_bytes(row, lanes), "external-sampling frames: ");
+  if (!rc) {
+    const size_t rows = (size_t)c->sc.cap_lanes * kMaxDepth;  // the sort input's records
+    rc = grow(c->es_rd, c->es_rd_bytes, rows * (size_t)row * sizeof(float), "external-sampling delta rows: ");
   }
   c->sc.es_stack = c->es_stack;
-  return MCCFR_OK;
+  c->sc.es_rd = c->es_rd;
+  return rc;
 }
 
 // Waits for the context's work and reports (and clears) an external-sampling
@@ -989,6 +996,7 @@ int mccfr_set_max_batch(mccfr_ctx* c, uint32_t max_batch) {
     free_scratch(c->sc);
     const hipError_t e = alloc_scratch(c->sc, cap);
     c->sc.es_stack = c->es_stack;
+    c->sc.es_rd = c->es_rd;  // regrown for the new capacity by the next external-sampling launch
     if (e != hipSuccess) {
       free_scratch(c->sc);
       c->max_batch = 0;

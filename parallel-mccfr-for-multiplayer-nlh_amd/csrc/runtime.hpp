@@ -69,6 +69,8 @@ struct mccfr_ctx {
   mccfr_trace* dtrace = nullptr;
   uint32_t* es_stack = nullptr;  // external-sampling walk frames, grown to the largest such launch
   size_t es_stack_bytes = 0;
+  float* es_rd = nullptr;  // external-sampling regret-delta rows, one per sort-input record
+  size_t es_rd_bytes = 0;
   mccfr::DevSeg* h_stage = nullptr;  // pinned [kStageRing][kMaxSegs]
   hipEvent_t stage_ev[mccfr::kStageRing] = {};
   uint32_t stage_next = 0;

@@ -54,12 +54,12 @@ constexpr uint32_t kTurnUnknown = 0xFFu;
 // sd_state bit of a traversal that ended in a showdown (the low bits hold the folded seats).
 constexpr uint32_t kShowdownFlag = 0x80u;
 
-// Record flags above the legal mask (bits 8..23 of a record's action word):
-// an external-sampling record adds its value to its action's regret only
-// (kEsRecord), and the first record of each node adds the node's
-// strategy-sum share t * sigma to every legal action (kEsHead).
+// Record flag above the legal mask (bits 8..23 of a record's action word):
+// an external-sampling record stands for one update-player node; its second
+// word indexes the node's row of regret deltas v(a) - v(I) (DevScratch::es_rd,
+// zero for illegal actions), added to the legal actions' regrets, and the node
+// adds its strategy-sum share t * sigma to every legal action.
 constexpr uint32_t kEsRecord = 1u << 24;
-constexpr uint32_t kEsHead = 1u << 25;
 constexpr uint32_t kLegalBits = 0xFFFFu;
 
 // Waiting lanes (with a next traversal) that trigger a refill round of the walk.
@@ -708,13 +708,13 @@ __device__ __forceinline__ float showdown_payoff(int up, uint32_t folded, const 
 // depths, so lane-interleaved words would not coalesce either); a leaf's
 // value returns into the top frame, which restores its state for the next
 // action or, with none left, forms v(I) = sum_a sigma(a) v(a) (sigma from the
-// FP32 regrets, as the reference's update node forms it) and emits a record
-// (a | mask << 8 | kEsRecord, v(a) - v(I)) per legal action, the first
-// flagged kEsHead (it carries the node's strategy-sum share).  The update
-// player acts at most on every other level, so the depth guard of 20 bounds
-// the stack at kEsFrames frames.  Records are appended to the sort input
-// with one atomic per wave and iteration (k_es_begin zeroes the counter);
-// beyond the input's capacity nothing more is written and k_emit_scan marks
+// FP32 regrets, as the reference's update node forms it) and emits one
+// record (mask << 8 | kEsRecord, its index) with its row of v(a) - v(I) in
+// DevScratch::es_rd at that index.  The update player acts at most on every
+// other level, so the depth guard of 20 bounds the stack at kEsFrames frames.
+// Records are appended to the sort input in per-wave reservations of
+// kEsChunk (one atomic each; k_es_begin zeroes the counter, unused tails hold
+// null records); beyond the input's capacity nothing more is written and k_emit_scan marks
 // the launch as overflowed (no table is touched).  Showdown payoffs are
 // settled where the walk reaches them, since the value is needed at once.
 // ---------------------------------------------------------------------------
@@ -739,8 +739,8 @@ __global__ void k_es_begin(DevScratch sc) {
 }
 
 // Records of one reservation of k_walk_es: at least the most one iteration of
-// a wave can emit (64 lanes x 16 actions).
-constexpr uint32_t kEsChunk = 1024;
+// a wave can emit (a node per lane).
+constexpr uint32_t kEsChunk = 256;
 
 template <int ROW, bool TRACE>
 __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __restrict__ segs,
@@ -1133,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           }
 #pragma unroll
           for (int a = 0; a < ROW; ++a) rd[a] = v[a] - vi;
-          emit_k = (uint32_t)__popc(vm);
+          emit_k = 1u;
           e_slot = slot;
           e_vm = vm;
           ret = vi;
@@ -1174,16 +1174,20 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
         }
       }
       if (c_cur != kEsDead) {
-        uint32_t off = c_cur + incl - emit_k;
-        uint32_t left = e_vm;
-        uint32_t head = kEsHead;
+        if (emit_k) {
+          // the node's record and its row of regret deltas (16-byte stores)
+          const uint32_t off = c_cur + incl - emit_k;
+          sc.key[0][off] = seg_key | e_slot;
+          sc.val[0][off] = make_uint2((e_vm << 8) | kEsRecord, off);
+          f32x4* row = reinterpret_cast<f32x4*>(sc.es_rd + (size_t)off * ROW);
 #pragma unroll
-        for (int a = 0; a < ROW; ++a) {
-          if (left >> a & 1u) {
-            sc.key[0][off] = seg_key | e_slot;
-            sc.val[0][off] = make_uint2((uint32_t)a | (e_vm << 8) | kEsRecord | head, __float_as_uint(rd[a]));
-            head = 0;
-            ++off;
+          for (int q4 = 0; q4 < ROW / 4; ++q4) {
+            f32x4 o;
+            o.x = (e_vm >> (4 * q4) & 1u) ? rd[4 * q4] : 0.0f;
+            o.y = (e_vm >> (4 * q4 + 1) & 1u) ? rd[4 * q4 + 1] : 0.0f;
+            o.z = (e_vm >> (4 * q4 + 2) & 1u) ? rd[4 * q4 + 2] : 0.0f;
+            o.w = (e_vm >> (4 * q4 + 3) & 1u) ? rd[4 * q4 + 3] : 0.0f;
+            row[q4] = o;
           }
         }
         c_cur += total;
@@ -1972,17 +1976,26 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 #pragma unroll
             for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
           }
-          cnt += (!es || (v.x & kEsHead)) ? 1u : 0u;  // nodes of the stretch
-          float q = qv[0];
+          ++cnt;  // every record is one node
+          if (es) {
+            // external sampling: the node's row of regret deltas
+            float rdv[ROW];
+            load_f32_row<ROW>(sc.es_rd + (size_t)v.y * ROW, rdv);
 #pragma unroll
-          for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-          // outcome sampling: u / q - u for the sampled action, -u for the
-          // other legal ones; external sampling: the record's own action only
-          const float rd_sampled = es ? u : u / q - u;
-          const float rd_other = es ? 0.0f : 0.0f - u;
+            for (int b = 0; b < ROW; ++b)
+              if (b < A && (vm >> b & 1u)) dr[b] += (double)rdv[b];
+          } else {
+            float q = qv[0];
 #pragma unroll
-          for (int b = 0; b < ROW; ++b)
-            if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+            for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+            // outcome sampling: u / q - u for the sampled action, -u for the
+            // other legal ones
+            const float rd_sampled = u / q - u;
+            const float rd_other = 0.0f - u;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b)
+              if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+          }
         }
         float ss[ROW];
         load_f32_row<ROW>(c.S + rs_off, ss);
@@ -2055,17 +2068,26 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 #pragma unroll
             for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
           }
-          cnt += (!es || (v.x & kEsHead)) ? 1u : 0u;  // nodes of the stretch
-          float q = qv[0];
+          ++cnt;  // every record is one node
+          if (es) {
+            // external sampling: the node's row of regret deltas
+            float rdv[ROW];
+            load_f32_row<ROW>(sc.es_rd + (size_t)v.y * ROW, rdv);
 #pragma unroll
-          for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
-          // outcome sampling: u / q - u for the sampled action, -u for the
-          // other legal ones; external sampling: the record's own action only
-          const float rd_sampled = es ? u : u / q - u;
-          const float rd_other = es ? 0.0f : 0.0f - u;
+            for (int b = 0; b < ROW; ++b)
+              if (b < A && (vm >> b & 1u)) dr[b] += (double)rdv[b];
+          } else {
+            float q = qv[0];
 #pragma unroll
-          for (int b = 0; b < ROW; ++b)
-            if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+            for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+            // outcome sampling: u / q - u for the sampled action, -u for the
+            // other legal ones
+            const float rd_sampled = u / q - u;
+            const float rd_other = 0.0f - u;
+#pragma unroll
+            for (int b = 0; b < ROW; ++b)
+              if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+          }
         }
         asm volatile("" ::: "memory");  // the rows below are loaded here, not kept from the loop
         // R <- R + dR (CFR+ clamp), S <- S + dS four actions at a time, loads of
@@ -2183,7 +2205,8 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
       const float qa = fmaxf(m_sum > 1e-7f ? fmaxf(0.0f, r_act) / m_sum : m_uni, 0.05f);
       const uint32_t k = (uint32_t)(lane + 64 * b);
       s_av[wv][k] = v[b].x;
-      s_rs[wv][k] = es ? u : u / qa - u;
+      // external sampling: the node's delta-row index in place of the regret
+      s_rs[wv][k] = es ? __uint_as_float(v[b].y) : u / qa - u;
       s_u[wv][k] = es ? 0.0f : u;
     }
     wave_lds_sync();
@@ -2209,9 +2232,11 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
         sig_a = pos ? (legal ? fmaxf(0.0f, ra) : 0.0f) / t : (legal ? uni : 0.0f);
       }
       if (legal) {
-        dr += (double)((uint32_t)a == act ? s_rs[wv][k] : 0.0f - s_u[wv][k]);
-        // an external-sampling node adds its strategy-sum share once (its head record)
-        if (!(av & kEsRecord) || (av & kEsHead)) ds += (double)(c.tw * sig_a);
+        if (av & kEsRecord)
+          dr += (double)sc.es_rd[(size_t)__float_as_uint(s_rs[wv][k]) * ROW + a];
+        else
+          dr += (double)((uint32_t)a == act ? s_rs[wv][k] : 0.0f - s_u[wv][k]);
+        ds += (double)(c.tw * sig_a);  // every record is one node
       }
     }
     double tr = 0.0, ts = 0.0;
