@@ -338,7 +338,8 @@ def es_bench(mccfr, batch_log2=20, table_log2=26, warmup=6, steps=12, table_seed
     seeded random regret state.  The walk's algorithmic bytes sum, over each
     explored tree, the root record (6N + 1 B), a BF16 snapshot row per
     sampled (other seats') node, an FP32 regret row per update-player node
-    and a 12-byte record per (update node, legal action)."""
+    and, per update-player node, its 12-byte sort record and its FP32 row of
+    regret deltas."""
     cfg = mccfr.cfg_6p_100bb(table_log2=table_log2, seed=0)
     cfg.sampling = mccfr.SAMPLE_EXTERNAL
     B = 1 << batch_log2
@@ -368,14 +369,14 @@ def es_bench(mccfr, batch_log2=20, table_log2=26, warmup=6, steps=12, table_seed
     d = tr["decisions"].astype(np.float64).sum()
     u = tr["updates"].astype(np.float64).sum()
     v = tr["update_legal"].astype(np.float64).sum()
-    walk_bytes = (6.0 * N + 1.0) * B + 2.0 * row * (d - u) + 4.0 * row * u + 12.0 * v
+    walk_bytes = (6.0 * N + 1.0) * B + 2.0 * row * (d - u) + 4.0 * row * u + (12.0 + 4.0 * row) * u
     tree_bytes = 72.0 * N * B + 2.0 * row * d + u * 4.0 * row + 16.0 * v
     wms = float(walk_ms.mean())
     return {"workload": "6max_100bb, external sampling at update-player nodes", "traversals_per_launch": B,
             "table_slots": 1 << table_log2, "steps": steps, "traversals_per_s": steps * B / dt,
             "ms_per_launch": dt * 1e3 / steps,
             "kernel_ms": {"deal": float(deal_ms.mean()), "walk": wms, "table_update": float(upd_ms.mean())},
-            "per_traversal": {"decisions": d / B, "update_nodes": u / B, "records": v / B,
+            "per_traversal": {"decisions": d / B, "update_nodes": u / B, "update_node_actions": v / B,
                               "leaves": float(tr["reserved"].astype(np.float64).mean())},
             "records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
             "walk_roofline": {"bound": "hbm", "algorithmic_bytes_per_launch": walk_bytes,
@@ -662,7 +663,7 @@ def main():
         log("bench: external sampling")
         result["external_sampling"] = es_bench(mccfr)
         log(f"bench: external sampling {result['external_sampling']['traversals_per_s'] / 1e6:.1f} M traversals/s "
-            f"({result['external_sampling']['per_traversal']['records']:.1f} records per traversal)")
+            f"({result['external_sampling']['per_traversal']['update_nodes']:.1f} update nodes per traversal)")
 
     if not args.no_large:
         log("bench: large table (reduced run)")
