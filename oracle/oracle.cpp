@@ -217,9 +217,14 @@ inline void philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t 
 
 // Draw k of traversal `tid`: word k&3 of Philox(counter = {k>>2, tid_lo, tid_hi, tag},
 // key = {seed_lo, seed_hi}).
+// `path` (external sampling) names the task of the traversal the stream
+// belongs to: 0 for the root (and every outcome-sampling walk), the task's
+// path code for the subtree under an update-player action; it occupies the
+// counter's first word above the block index (k < 256 on every walk).
 struct Stream {
   uint32_t key[2];
   uint32_t tid_lo, tid_hi, tag;
+  uint32_t path = 0;
   uint32_t k = 0;
   uint32_t blk[4];
   uint32_t blk_id = 0xFFFFFFFFu;
@@ -229,13 +234,29 @@ struct Stream {
   uint32_t next() {
     const uint32_t b = k >> 2;
     if (b != blk_id) {
-      const uint32_t ctr[4] = {b, tid_lo, tid_hi, tag};
+      const uint32_t ctr[4] = {b | (path << 6), tid_lo, tid_hi, tag};
       philox(ctr, key, blk);
       blk_id = b;
     }
     return blk[k++ & 3];
   }
 };
+
+// Path code of the task under action a of an update-player node whose own
+// task has code p (external sampling): FNV-1a step, 26 bits, never 0.
+inline uint32_t es_child_path(uint32_t p, int a) {
+  const uint32_t h = ((p ^ (uint32_t)(a + 1)) * 16777619u) & 0x03FFFFFFu;
+  return h ? h : 1u;
+}
+
+// FNV-1a of one (slot, action) pair from the offset basis: an external-
+// sampling trace sums these (mod 2^32) over every entered (node, action), an
+// order-free check of the explored tree.
+inline uint32_t fnv_pair(uint32_t slot, int a) {
+  uint32_t h = 2166136261u;
+  for (int i = 0; i < 4; ++i) h = (h ^ ((slot >> (8 * i)) & 0xFFu)) * 16777619u;
+  return (h ^ (uint32_t)(a & 0xFF)) * 16777619u;
+}
 
 // Uniform in (0, 1]: ((x >> 8) + 1) * 2^-24, exact in FP32.
 inline float u01(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
@@ -472,7 +493,7 @@ struct Walker {
   const int32_t* hr;
   const Tables& T;
   Sink& sink;
-  Stream& rng;
+  Stream* rng;  // the current task's stream (external sampling swaps it per task)
   int up;
   float tw;  // strategy-sum weight (float)iteration, cfr_gpu.cu:828
   uint32_t path_hash = 2166136261u;
@@ -499,9 +520,27 @@ struct Walker {
     log->push_back(d);
   }
 
+  // Outcome sampling chains FNV-1a over the path's (slot, action) bytes in
+  // walk order; external sampling adds each pair's FNV-1a (order free: the
+  // device explores the tree's tasks in no fixed order).
   void hash_step(uint32_t slot, int a) {
+    if (cfg.sampling == MCCFR_SAMPLE_EXTERNAL) {
+      path_hash += fnv_pair(slot, a);
+      return;
+    }
     for (int i = 0; i < 4; ++i) path_hash = (path_hash ^ ((slot >> (8 * i)) & 0xFFu)) * 16777619u;
     path_hash = (path_hash ^ (uint32_t)(a & 0xFF)) * 16777619u;
+  }
+
+  // A leaf of kind 1 (fold), 2 (showdown) or 3 (depth guard): the trace keeps
+  // the kind (outcome sampling: the path's one leaf) or the set of kinds
+  // reached, bit k for kind k (external sampling).
+  void leaf(int kind) {
+    ++leaves;
+    if (cfg.sampling == MCCFR_SAMPLE_EXTERNAL)
+      terminal |= 1 << kind;
+    else
+      terminal = kind;
   }
 
   // cfr_gpu.cu:510-537
@@ -511,7 +550,7 @@ struct Walker {
     for (int a = 0; a < A; ++a)
       if (vm >> a & 1u) total += T.s(slot, a);
     if (total < 1e-7f) return passive_fallback(cfg, vm);
-    const float r = u01(rng.next()) * total;
+    const float r = u01(rng->next()) * total;
     float cum = 0.0f;
     int result = -1;
     for (int a = 0; a < A; ++a) {
@@ -598,13 +637,12 @@ struct Walker {
   // recursion is a single root-to-leaf path; the state is not restored after
   // the child returns because nothing reads it afterwards.
   float traverse(Game& g, int depth) {
-    if (depth >= 20) { terminal = 3; ++leaves; return 0.0f; }
+    if (depth >= 20) { leaf(3); return 0.0f; }
     int active = 0, winner = -1;
     for (int i = 0; i < g.N; ++i)
       if (!(g.folded >> i & 1u)) { ++active; winner = i; }
     if (active == 1) {
-      terminal = 1;
-      ++leaves;
+      leaf(1);
       return winner == up ? (float)(g.pot - g.invested[up]) : (float)(-(int)g.invested[up]);
     }
     if (g.n_to_act == 0) {
@@ -621,8 +659,7 @@ struct Walker {
         }
         if (g.n_to_act > 0) return traverse(g, depth + 1);
       }
-      terminal = 2;
-      ++leaves;
+      leaf(2);
       return showdown(g);
     }
     const int player = g.to_act[0];
@@ -647,8 +684,12 @@ struct Walker {
         strat[a] = (sum_r > 1e-7f) ? strat[a] / sum_r : ((vm >> a & 1u) ? 1.0f / (float)nvalid : 0.0f);
       if (cfg.sampling == MCCFR_SAMPLE_EXTERNAL) {
         // external sampling: every legal action's subtree, in action order,
-        // from a copy of this node's state; no draw at this node
+        // from a copy of this node's state; no draw at this node.  Each
+        // subtree is a task with its own stream: the traversal's counters
+        // with the task's path code (es_child_path of this task's code and
+        // the action), from draw 0.
         float v[MCCFR_MAX_ACTIONS] = {};
+        Stream* const own = rng;
         for (int a = 0; a < A; ++a) {
           if (!(vm >> a & 1u)) continue;
           Game child = g;
@@ -659,7 +700,13 @@ struct Walker {
           for (int i = 0; i < child.n_to_act; ++i) child.to_act[i] = child.to_act[i + 1];
           if (apply(child, player, a)) rebuild_queue(child, player);
           log_decision(child, player, a, vm, slot, hist, pot_before);
+          Stream task = *own;
+          task.path = es_child_path(own->path, a);
+          task.k = 0;
+          task.blk_id = 0xFFFFFFFFu;
+          rng = &task;
           v[a] = traverse(child, depth + 1);
+          rng = own;
         }
         float vi = 0.0f;  // v(I) = sum_a sigma(a) v(a), in action order
         for (int a = 0; a < A; ++a)
@@ -808,7 +855,7 @@ void run_traversal(const mccfr_game_cfg& cfg, const mccfr_root_state* root, cons
   Stream rng(cfg.seed, tid, tag);
   Game g;
   setup_game(cfg, root, hr, lut, rng, g);
-  Walker<Tables, Sink> w{cfg, hr, T, sink, rng, up, tw};
+  Walker<Tables, Sink> w{cfg, hr, T, sink, &rng, up, tw};
   const float u = w.traverse(g, 0);
   if (tr) {
     tr->utility = u;
@@ -1515,7 +1562,7 @@ int orc_walk_cards(const mccfr_game_cfg* cfg, const mccfr_root_state* root, cons
   std::vector<Contribution> contrib;
   SinkList sink{&contrib};
   std::vector<orc_decision> dec;
-  Walker<TablesAoS, SinkList> w{*cfg, hr, T, sink, rng, up, (float)t_weight};
+  Walker<TablesAoS, SinkList> w{*cfg, hr, T, sink, &rng, up, (float)t_weight};
   w.log = &dec;
   const float u = w.traverse(g, 0);
   if (trace) {

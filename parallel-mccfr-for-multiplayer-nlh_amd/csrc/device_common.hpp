@@ -151,8 +151,14 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   out[3] = c3;
 }
 
+// External sampling splits a traversal's tree into tasks (the root's, and
+// one per action of every update-player node), each with its own stream:
+// `path` is the task's code (0 for the root and every outcome-sampling walk),
+// held in the counter's first word above the block index (k < 256 on every
+// walk).
 struct DrawStream {
   uint32_t k0, k1, tid_lo, tid_hi, tag;
+  uint32_t path;   // task path code (external sampling), 0 otherwise
   uint32_t k;      // next draw index
   uint32_t blk;    // block id held in w[] (0xFFFFFFFF = none)
   uint32_t w[4];
@@ -162,13 +168,21 @@ struct DrawStream {
     tid_lo = (uint32_t)tid;
     tid_hi = (uint32_t)(tid >> 32);
     tag = tg;
+    path = 0;
+    k = 0;
+    blk = 0xFFFFFFFFu;
+  }
+  // the stream of the task under action a of an update-player node of this task
+  __host__ __device__ void enter_child(uint32_t node_path, int a) {
+    const uint32_t h = ((node_path ^ (uint32_t)(a + 1)) * 16777619u) & 0x03FFFFFFu;
+    path = h ? h : 1u;
     k = 0;
     blk = 0xFFFFFFFFu;
   }
   __host__ __device__ uint32_t next() {
     const uint32_t b = k >> 2;
     if (b != blk) {
-      philox4x32_10(b, tid_lo, tid_hi, tag, k0, k1, w);
+      philox4x32_10(b | (path << 6), tid_lo, tid_hi, tag, k0, k1, w);
       blk = b;
     }
     const uint32_t i = k & 3u;

@@ -712,6 +712,11 @@ __device__ __forceinline__ float showdown_payoff(int up, uint32_t folded, const 
 // record (mask << 8 | kEsRecord, its index) with its row of v(a) - v(I) in
 // DevScratch::es_rd at that index.  The update player acts at most on every
 // other level, so the depth guard of 20 bounds the stack at kEsFrames frames.
+// Every update-player action's subtree is a task with a stream of its own
+// (DrawStream::enter_child: the traversal's counters with the task's path
+// code, from draw 0), so a subtree's draws do not depend on the order in which
+// subtrees are walked; the trace's path hash sums each entered (node, action)
+// pair's FNV-1a and its terminal field holds the set of leaf kinds reached.
 // Records are appended to the sort input in per-wave reservations of
 // kEsChunk (one atomic each; k_es_begin zeroes the counter, unused tails hold
 // null records); beyond the input's capacity nothing more is written and k_emit_scan marks
@@ -728,7 +733,7 @@ inline uint32_t es_grid(uint32_t lanes) { return (lanes + 256u * kEsTpl - 1u) / 
 template <int ROW>
 constexpr int es_frame_words() {
   return 16 + ROW;  // 0..5 seats, 6 seat masks + queue, 7 history, 8 pot, 9 bet, 10 lfr, 11 street | raises |
-                    // depth, 12 slot, 13 legal | left << 16, 14..15 unused, 16.. child values
+                    // depth, 12 slot, 13 legal | left << 16, 14 the node's task path, 15 unused, 16.. child values
 }
 
 __global__ void k_es_begin(DevScratch sc) {
@@ -942,12 +947,12 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
       bool leaf = false;
       if (depth >= kMaxDepth) {
         ret = 0.0f;  // depth guard (cfr_gpu.cu:661)
-        terminal = 3;
+        terminal |= 1 << 3;
         leaf = true;
       } else if (__popc(allmask & ~folded) == 1) {
         const int inv_up = G.stack[up] - (int)(sb[up * WG] & kStackMask);
         ret = ((int)__builtin_ctz(allmask & ~folded) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
-        terminal = 1;
+        terminal |= 1 << 1;
         leaf = true;
       } else if (qmask == 0) {
         // street change (cfr_gpu.cu:668-692); with nobody left to act, or
@@ -972,7 +977,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
             rkv[p] = p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
           }
           ret = showdown_payoff(up, folded, inv, rkv);
-          terminal = 2;
+          terminal |= 1 << 2;
           leaf = true;
         }
       } else {
@@ -998,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           // level); flagged for the host rather than written out of bounds
           sc.cnt->es_overflow = 2u;
           ret = 0.0f;
-          terminal = 3;
+          terminal |= 1 << 3;
           leaf = true;
         } else if (player == up) {
           // push the node: its state before the action, slot, legal actions
@@ -1012,8 +1017,9 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           fr4(sp, 2) = make_uint4((uint32_t)pot, (uint32_t)current_bet, (uint32_t)lfr,
                                   (uint32_t)street | ((uint32_t)raises << 8) | ((uint32_t)depth << 16));
           act = __builtin_ctz(vm);
-          fr4(sp, 3) = make_uint4(slot, vm | ((vm & (vm - 1u)) << 16), 0u, 0u);
+          fr4(sp, 3) = make_uint4(slot, vm | ((vm & (vm - 1u)) << 16), rng.path, 0u);
           ++sp;
+          rng.enter_child(rng.path, act);  // the first action's task
         } else {
           // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot, as k_walk
           float sg[ROW];
@@ -1044,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           }
         }
         if (act >= 0) {
-          if (TRACE) phash = fnv_path(phash, slot, (uint32_t)act);
+          if (TRACE) phash += fnv_path(2166136261u, slot, (uint32_t)act);
           ap_player = player;
           ap_act = act;
         }
@@ -1106,7 +1112,8 @@ __global__ __launch_bounds__(256) void k_walk_es(LaunchCfg lc, const DevSeg* __r
           street = (int)(w11 & 0xFFu);
           raises = (int)((w11 >> 8) & 0xFFu);
           depth = (int)(w11 >> 16);
-          if (TRACE) phash = fnv_path(phash, q3.x, (uint32_t)a);
+          if (TRACE) phash += fnv_path(2166136261u, q3.x, (uint32_t)a);
+          rng.enter_child(q3.z, a);  // the action's task
           ap_player = up;
           ap_act = a;
           ret_mode = false;
@@ -1902,7 +1909,7 @@ __device__ __forceinline__ SegCtx ctx_of_key(const SegCtx* __restrict__ sctx, ui
 // The first `cand_blocks` workgroups find the long runs' ends and pieces
 // instead (reduce_cand): that latency-bound search overlaps the short runs'
 // updates rather than taking a launch of its own.
-template <int ROW>
+template <int ROW, bool ES>
 __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                      const uint2* __restrict__ vals, uint32_t cand_blocks) {
   if (blockIdx.x < cand_blocks) {
@@ -1942,22 +1949,23 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
         double dr[ROW];
 #pragma unroll
         for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
-        // the current mask's sum of positive regrets, uniform share and q per action
+        // the current mask's sigma per action (a legal action's share of the
+        // positive regrets, or the uniform share), formed once per mask: q is
+        // max(sigma, 0.05) and a stretch's dS uses sigma itself
         uint32_t cur_vm = 0, cnt = 0;
-        float sum_r = 0.0f, uni = 0.0f, qv[ROW];
+        float sv[ROW];
 #pragma unroll
-        for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
+        for (int b = 0; b < ROW; ++b) sv[b] = 0.0f;
         // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
         auto stretch_ds = [&](int b) -> double {
           const bool legal = b < A && (cur_vm >> b & 1u);
-          const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, rr[b]) : 0.0f) / sum_r : (legal ? uni : 0.0f);
-          return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
+          return legal ? (double)cnt * (double)(c.tw * sv[b]) : 0.0;
         };
         bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
         for (uint32_t k = 0; k < run.y; ++k) {
           const uint2 v = s_v[wv][run.x - base + k];
           const uint32_t act = v.x & 0xFFu, vm = (v.x >> 8) & kLegalBits;
-          const bool es = (v.x & kEsRecord) != 0;
+          const bool es = ES && (v.x & kEsRecord) != 0;
           const float u = __uint_as_float(v.y);
           if (vm != cur_vm) {
             if (cnt) {
@@ -1970,11 +1978,10 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
             float t = 0.0f;
 #pragma unroll
             for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
-            sum_r = t;
-            uni = 1.0f / (float)__popc(vm);
+            const float uni = 1.0f / (float)__popc(vm);
             const bool pos = t > 1e-7f;
 #pragma unroll
-            for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
+            for (int b = 0; b < ROW; ++b) sv[b] = pos ? fmaxf(0.0f, rr[b]) / t : uni;
           }
           ++cnt;  // every record is one node
           if (es) {
@@ -1985,16 +1992,17 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
             for (int b = 0; b < ROW; ++b)
               if (b < A && (vm >> b & 1u)) dr[b] += (double)rdv[b];
           } else {
-            float q = qv[0];
+            float sa = sv[0];
 #pragma unroll
-            for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+            for (int b = 1; b < ROW; ++b) sa = ((uint32_t)b == act) ? sv[b] : sa;
+            const float q = fmaxf(sa, 0.05f);
             // outcome sampling: u / q - u for the sampled action, -u for the
-            // other legal ones
-            const float rd_sampled = u / q - u;
-            const float rd_other = 0.0f - u;
+            // other legal ones (each rounded to FP32, then widened once)
+            const double rd_sampled = (double)(u / q - u);
+            const double rd_other = (double)(0.0f - u);
 #pragma unroll
             for (int b = 0; b < ROW; ++b)
-              if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+              if (b < A && (vm >> b & 1u)) dr[b] += (uint32_t)b == act ? rd_sampled : rd_other;
           }
         }
         float ss[ROW];
@@ -2028,25 +2036,24 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
         double dr[ROW];
 #pragma unroll
         for (int b = 0; b < ROW; ++b) dr[b] = 0.0;
-        // the current mask's sum of positive regrets, uniform share and q per
-        // action.  The 16-wide regret row is read again where it is needed (at
-        // a mask change and for the apply) rather than held through the
-        // record loop: 176 -> 114 registers, 2 -> 4 waves per SIMD.
+        // the current mask's sigma per action, formed once per mask (q is
+        // max(sigma, 0.05)).  The 16-wide regret row is read again where it
+        // is needed (at a mask change and for the apply) rather than held
+        // through the record loop: 176 -> 114 registers, 2 -> 4 waves per SIMD.
         uint32_t cur_vm = 0, cnt = 0;
-        float sum_r = 0.0f, uni = 0.0f, qv[ROW];
+        float sv[ROW];
 #pragma unroll
-        for (int b = 0; b < ROW; ++b) qv[b] = 0.0f;
+        for (int b = 0; b < ROW; ++b) sv[b] = 0.0f;
         // dS of a mask stretch: count * (tw * sigma_a) for the legal actions
-        auto stretch_ds = [&](int b, float r_b) -> double {
+        auto stretch_ds = [&](int b) -> double {
           const bool legal = b < A && (cur_vm >> b & 1u);
-          const float sig = sum_r > 1e-7f ? (legal ? fmaxf(0.0f, r_b) : 0.0f) / sum_r : (legal ? uni : 0.0f);
-          return legal ? (double)cnt * (double)(c.tw * sig) : 0.0;
+          return legal ? (double)cnt * (double)(c.tw * sv[b]) : 0.0;
         };
         bool spilled = false;  // dS of earlier stretches (a mask change inside the run) is in s_ds
         for (uint32_t k = 0; k < run.y; ++k) {
           const uint2 v = s_v[wv][run.x - base + k];
           const uint32_t act = v.x & 0xFFu, vm = (v.x >> 8) & kLegalBits;
-          const bool es = (v.x & kEsRecord) != 0;
+          const bool es = ES && (v.x & kEsRecord) != 0;
           const float u = __uint_as_float(v.y);
           if (vm != cur_vm) {
             float rr[ROW];
@@ -2054,7 +2061,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
             if (cnt) {
 #pragma unroll
               for (int b = 0; b < ROW; ++b)
-                s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b, rr[b]);
+                s_ds[b][threadIdx.x] = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
               spilled = true;
             }
             cnt = 0;
@@ -2062,11 +2069,10 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
             float t = 0.0f;
 #pragma unroll
             for (int b = 0; b < ROW; ++b) t += (b < A && (vm >> b & 1u)) ? fmaxf(0.0f, rr[b]) : 0.0f;
-            sum_r = t;
-            uni = 1.0f / (float)__popc(vm);
+            const float uni = 1.0f / (float)__popc(vm);
             const bool pos = t > 1e-7f;
 #pragma unroll
-            for (int b = 0; b < ROW; ++b) qv[b] = fmaxf(pos ? fmaxf(0.0f, rr[b]) / t : uni, 0.05f);
+            for (int b = 0; b < ROW; ++b) sv[b] = pos ? fmaxf(0.0f, rr[b]) / t : uni;
           }
           ++cnt;  // every record is one node
           if (es) {
@@ -2077,16 +2083,17 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
             for (int b = 0; b < ROW; ++b)
               if (b < A && (vm >> b & 1u)) dr[b] += (double)rdv[b];
           } else {
-            float q = qv[0];
+            float sa = sv[0];
 #pragma unroll
-            for (int b = 1; b < ROW; ++b) q = ((uint32_t)b == act) ? qv[b] : q;
+            for (int b = 1; b < ROW; ++b) sa = ((uint32_t)b == act) ? sv[b] : sa;
+            const float q = fmaxf(sa, 0.05f);
             // outcome sampling: u / q - u for the sampled action, -u for the
-            // other legal ones
-            const float rd_sampled = u / q - u;
-            const float rd_other = 0.0f - u;
+            // other legal ones (each rounded to FP32, then widened once)
+            const double rd_sampled = (double)(u / q - u);
+            const double rd_other = (double)(0.0f - u);
 #pragma unroll
             for (int b = 0; b < ROW; ++b)
-              if (b < A && (vm >> b & 1u)) dr[b] += (double)((uint32_t)b == act ? rd_sampled : rd_other);
+              if (b < A && (vm >> b & 1u)) dr[b] += (uint32_t)b == act ? rd_sampled : rd_other;
           }
         }
         asm volatile("" ::: "memory");  // the rows below are loaded here, not kept from the loop
@@ -2114,7 +2121,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const int b = 4 * q4 + j;
-            const double ds = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b, rr[b]);
+            const double ds = (spilled ? s_ds[b][threadIdx.x] : 0.0) + stretch_ds(b);
             float x = (float)((double)rr[b] + dr[b]);
             if (c.cfr_plus) x = fmaxf(0.0f, x);
             nr[b] = x;
@@ -2158,7 +2165,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(LaunchCfg lc, DevScratch sc
 // piece's FP64 sums are stored.  A piece is defined from its run's first
 // record, so the sums depend only on the run's records, not on where the run
 // sits in the launch.
-template <int ROW>
+template <int ROW, bool ES>
 __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch sc, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ vals) {
   constexpr int kGroups = 64 / ROW;
@@ -2189,7 +2196,7 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
 #pragma unroll
     for (int b = 0; b < (int)(kPiece / 64); ++b) {
       const uint32_t act = v[b].x & 0xFFu, vm = (v[b].x >> 8) & kLegalBits;
-      const bool es = (v[b].x & kEsRecord) != 0;
+      const bool es = ES && (v[b].x & kEsRecord) != 0;
       const float u = __uint_as_float(v[b].y);
       if (vm != m_vm) {
         m_vm = vm;
@@ -2232,7 +2239,7 @@ __global__ __launch_bounds__(256) void k_reduce_piece(LaunchCfg lc, DevScratch s
         sig_a = pos ? (legal ? fmaxf(0.0f, ra) : 0.0f) / t : (legal ? uni : 0.0f);
       }
       if (legal) {
-        if (av & kEsRecord)
+        if (ES && (av & kEsRecord))
           dr += (double)sc.es_rd[(size_t)__float_as_uint(s_rs[wv][k]) * ROW + a];
         else
           dr += (double)((uint32_t)a == act ? s_rs[wv][k] : 0.0f - s_u[wv][k]);
@@ -2418,15 +2425,26 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint32_t rows_grid = wg4((max_rec + 127) / 128, 8192);  // a wave per pair of 64-record regions
   const uint32_t piece_grid = wg4(max_rec / kPiece + max_long, 8192);
   const uint32_t runs_grid = wg4(max_long, 2048);
-#define MCCFR_REDUCE(R)                                                                                    \
-  hipLaunchKernelGGL(k_reduce_rows<R>, dim3(cand_blocks + rows_grid), dim3(256), 0, s, lc, sc, keys, vals, \
-                     cand_blocks);                                                                         \
-  hipLaunchKernelGGL(k_reduce_piece<R>, dim3(piece_grid), dim3(256), 0, s, lc, sc, keys, vals);           \
+  // the record kinds are known per launch (a launch is all external
+  // sampling or none), so outcome-sampling launches run reduce kernels
+  // without the delta-row path
+#define MCCFR_REDUCE(R, E)                                                                                    \
+  hipLaunchKernelGGL((k_reduce_rows<R, E>), dim3(cand_blocks + rows_grid), dim3(256), 0, s, lc, sc, keys, vals, \
+                     cand_blocks);                                                                            \
+  hipLaunchKernelGGL((k_reduce_piece<R, E>), dim3(piece_grid), dim3(256), 0, s, lc, sc, keys, vals);           \
   hipLaunchKernelGGL(k_reduce_runs<R>, dim3(runs_grid), dim3(256), 0, s, lc, sc, keys)
   if (row == 8) {
-    MCCFR_REDUCE(8);
+    if (lc.es) {
+      MCCFR_REDUCE(8, true);
+    } else {
+      MCCFR_REDUCE(8, false);
+    }
   } else {
-    MCCFR_REDUCE(16);
+    if (lc.es) {
+      MCCFR_REDUCE(16, true);
+    } else {
+      MCCFR_REDUCE(16, false);
+    }
   }
 #undef MCCFR_REDUCE
 }
