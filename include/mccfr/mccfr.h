@@ -84,9 +84,14 @@ typedef struct mccfr_game_cfg {
  *   v(I) = sum_a sigma(a) v(a) with sigma from the FP32 regrets (as the
  *   reference's update node forms it), its regret contributions are
  *   v(a) - v(I) for every legal a, its strategy-sum contributions t * sigma(a)
- *   (the reference's rule, cfr_gpu.cu:828-836).  Draws follow the walk in
- *   depth-first order (children in action order); an update node draws
- *   nothing.  A traversal's utility is its root value. */
+ *   (the reference's rule, cfr_gpu.cu:828-836).  The tree is split into
+ *   tasks: the root's, and one per legal action of every update node (the
+ *   subtree below that action).  A task draws from its own stream: word k of
+ *   Philox(counter = {k >> 2 | path << 6, traversal id, tag}, key = seed),
+ *   path 0 for the root task (whose draws follow the deal's), and for the
+ *   task under action a of a node in a task with code p,
+ *   path = ((p ^ (a + 1)) * 16777619) mod 2^26, or 1 where that is 0.
+ *   An update node draws nothing.  A traversal's utility is its root value. */
 #define MCCFR_SAMPLE_OUTCOME 0
 #define MCCFR_SAMPLE_EXTERNAL 1
 
@@ -101,10 +106,12 @@ void mccfr_game_cfg_reference(mccfr_game_cfg* cfg, int num_players, int starting
 typedef struct mccfr_trace {
   float utility;       /* terminal payoff of the update player (EXTERNAL: the root value) */
   uint32_t path_hash;  /* FNV-1a over (slot, action) of every decision on the path
-                          (EXTERNAL: of every (node, action) entered, depth first) */
+                          (EXTERNAL: 2166136261 plus, mod 2^32, the FNV-1a of each
+                          (node, action) entered, in any order) */
   uint16_t decisions;  /* decision nodes on the path (EXTERNAL: in the explored tree) */
   uint8_t updates;     /* update-player decision nodes on the path (EXTERNAL: in the tree) */
-  uint8_t terminal;    /* 1 fold, 2 showdown, 3 depth limit (EXTERNAL: of the last leaf) */
+  uint8_t terminal;    /* 1 fold, 2 showdown, 3 depth limit (EXTERNAL: bit k set when a
+                          leaf of kind k was reached) */
   uint16_t update_legal; /* legal actions summed over the update-player nodes */
   uint16_t reserved;     /* EXTERNAL: leaves of the explored tree */
 } mccfr_trace;         /* EXTERNAL: counts are taken modulo their field widths */

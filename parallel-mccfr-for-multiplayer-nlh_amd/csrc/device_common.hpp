@@ -151,6 +151,13 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   out[3] = c3;
 }
 
+// Path code of the task under action a of an update-player node whose own
+// task has code p (external sampling): one FNV-1a step, 26 bits, never 0.
+__host__ __device__ inline uint32_t es_child_path(uint32_t p, int a) {
+  const uint32_t h = ((p ^ (uint32_t)(a + 1)) * 16777619u) & 0x03FFFFFFu;
+  return h ? h : 1u;
+}
+
 // External sampling splits a traversal's tree into tasks (the root's, and
 // one per action of every update-player node), each with its own stream:
 // `path` is the task's code (0 for the root and every outcome-sampling walk),
@@ -174,8 +181,7 @@ struct DrawStream {
   }
   // the stream of the task under action a of an update-player node of this task
   __host__ __device__ void enter_child(uint32_t node_path, int a) {
-    const uint32_t h = ((node_path ^ (uint32_t)(a + 1)) * 16777619u) & 0x03FFFFFFu;
-    path = h ? h : 1u;
+    path = es_child_path(node_path, a);
     k = 0;
     blk = 0xFFFFFFFFu;
   }

@@ -21,6 +21,26 @@ constexpr uint32_t kDigitBits = 10;     // widest radix digit (a launch spreads 
 constexpr uint32_t kMinDigitBits = 8;   // narrowest (a scatter thread owns whole digits)
 constexpr uint32_t kSortBins = 1u << kDigitBits;
 
+// External sampling by generations (one-segment launches): generation 0's
+// tasks are the traversals' roots; a task walks (sampling the other seats'
+// actions) to a leaf or to an update-player node, and such a node makes one
+// task of the next generation per legal action.  A task starts at least one
+// level below its node, so the depth guard (20 levels) bounds the
+// generations.
+constexpr uint32_t kEsGens = 21;
+// Buffers for a scratch of cap_lanes lanes: a node per record of the sort
+// input (cap_node = 20 per lane), and as many tasks per generation.
+struct EsGenBufs {
+  uint32_t* task[2];   // [cap_task] a task of generation >= 1: its node << 4 | action, by generation parity
+  uint32_t* task_cnt;  // [kEsGens + 1] tasks of each generation (0: the roots, not counted)
+  uint32_t* node_cnt;  // [kEsGens + 1] update-player nodes each generation made
+  uint4* node;         // [cap_node][4] a node's state before its action, its slot and legal mask
+  float* root_v;       // [cap_lanes] root value of each traversal (trace launches)
+  uint32_t* tr_acc;    // [cap_lanes][8] trace sums per traversal (trace launches)
+  uint32_t cap_task, cap_node;
+};
+constexpr uint32_t kEsNoParent = 0xFFFFFFFFu;
+
 // Scratch of one launch, sized for `cap_lanes` lanes.  Node lists are
 // lane-strided: entry k of lane l at [k * cap_lanes + l].
 struct DevScratch {
@@ -56,7 +76,9 @@ struct DevScratch {
   double* partial;       // [max_pieces][2 * 16] FP64 sums of each piece
   LaunchCounters* cnt;
   uint32_t* es_stack;    // external-sampling frames [kEsFrames][words][launch lanes] (not in the arena)
-  float* es_rd;          // external sampling: [sort input][row] regret deltas of each node record (not in the arena)
+  float* es_rd;          // external sampling: [sort input][row] regret deltas of each node record (not in the arena);
+                         // the generation walk first collects each node's child values there
+  EsGenBufs gen;         // external sampling by generations (not in the arena)
   uint32_t cap_lanes;
   uint32_t max_tiles;
 };
@@ -116,6 +138,10 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
 
 // Bytes of external-sampling walk frames for a launch of `lanes` lanes.
 size_t es_stack_bytes(int row, uint32_t lanes);
+// Bytes of the generation walk's buffers for a scratch of `cap_lanes` lanes,
+// and their carving.
+size_t es_gen_bytes(uint32_t cap_lanes);
+void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes);
 
 void launch_rm_sweep(hipStream_t s, int row, const float* R, uint16_t* sig, uint64_t slots);
 void launch_normalize(hipStream_t s, int row, const float* S, float* out, uint64_t slots);

@@ -202,7 +202,7 @@ void free_all(mccfr_ctx* c) {
     if (t.used && t.owned) (void)hipFree(t.owned);
   void* ptrs[] = {c->d_tenants, c->d_segs, c->hr, c->lut, c->arith.nonflush, c->arith.flush, c->arith.offsets,
                   c->dtrace, c->buckets.flop5, c->buckets.turn6, c->eval_bk.flop5, c->eval_bk.turn6,
-                  c->es_stack, c->es_rd};
+                  c->es_stack, c->es_rd, c->es_gen};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -222,9 +222,10 @@ int rm_sweep_if_dirty(mccfr_ctx* c, Tenant& t) {
   return MCCFR_OK;
 }
 
-// Frames of the external-sampling walk for a launch of `lanes` lanes, and a
-// regret-delta row for every record the sort input holds (both grown, never
-// shrunk; a captured graph holding the old buffers is dropped).
+// Frames of the external-sampling walk for a launch of `lanes` lanes, a
+// regret-delta row for every record the sort input holds, and the generation
+// walk's buffers (all grown, never shrunk; a captured graph holding old
+// buffers is dropped).
 int ensure_es_stack(mccfr_ctx* c, int row, uint32_t lanes) {
   const auto grow = [&](auto*& buf, size_t& have, size_t want, const char* what) -> int {
     if (want <= have) return MCCFR_OK;
@@ -245,8 +246,19 @@ int ensure_es_stack(mccfr_ctx* c, int row, uint32_t lanes) {
     const size_t rows = (size_t)c->sc.cap_lanes * kMaxDepth;  // the sort input's records
     rc = grow(c->es_rd, c->es_rd_bytes, rows * (size_t)row * sizeof(float), "external-sampling delta rows: ");
   }
+  if (!rc && c->es_gen_cap != c->sc.cap_lanes) {
+    // nodes and tasks per generation for the sort input's records
+    rc = grow(c->es_gen, c->es_gen_size, es_gen_bytes(c->sc.cap_lanes), "external-sampling generations: ");
+    if (!rc) {
+      HIPX(c, hipStreamSynchronize(c->stream));
+      drop_graph(c);  // a graph holds the old carving
+      carve_es_gen(c->es_gen_bufs, c->es_gen, c->sc.cap_lanes);
+      c->es_gen_cap = c->sc.cap_lanes;
+    }
+  }
   c->sc.es_stack = c->es_stack;
   c->sc.es_rd = c->es_rd;
+  if (c->es_gen_cap == c->sc.cap_lanes) c->sc.gen = c->es_gen_bufs;
   return rc;
 }
 

@@ -71,6 +71,10 @@ struct mccfr_ctx {
   size_t es_stack_bytes = 0;
   float* es_rd = nullptr;  // external-sampling regret-delta rows, one per sort-input record
   size_t es_rd_bytes = 0;
+  char* es_gen = nullptr;  // external sampling by generations: tasks, nodes, trace sums
+  size_t es_gen_size = 0;
+  uint32_t es_gen_cap = 0;  // the scratch capacity its carving in sc.gen was made for
+  mccfr::EsGenBufs es_gen_bufs{};
   mccfr::DevSeg* h_stage = nullptr;  // pinned [kStageRing][kMaxSegs]
   hipEvent_t stage_ev[mccfr::kStageRing] = {};
   uint32_t stage_next = 0;

@@ -2334,6 +2334,498 @@ void walk_dispatch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// External sampling by generations: the walk of one-segment launches
+// (k_walk_es walks launches of several segments depth first; both explore
+// the same tasks with the same streams, so their results are the same).
+// Generation 0's tasks are the traversals' roots.  One lane walks a task as
+// k_walk walks a traversal -- sampling the other seats' actions from the BF16
+// snapshot with the task's own stream -- until a leaf, whose value goes to its
+// parent node's child slot (or is the traversal's root value), or an
+// update-player node.  Such a node gets an index (one atomic per wave on the
+// generation's node count), stores its state as a 64-byte record and makes
+// one task of the next generation per legal action, a 4-byte reference
+// (node << 4 | action) in the other task buffer (one atomic per wave on the
+// next generation's task count); the task applies its action to the node's
+// state when it starts.  A node's child values collect in its row of
+// DevScratch::es_rd; k_es_back then forms v(I) and the row of v(a) - v(I),
+// generation by generation from the deepest, and passes v(I) to the parent.
+// Every node is one record of the sort input, at its own index.  Past a
+// buffer's capacity the launch is flagged (es_overflow) and the later
+// generations do nothing: the launch updates no table.
+// Node record: q0 = seat words 0..3 (chips behind | bet << 15 | street tag <<
+// 30); q1 = seat words 4, 5, folded | all-in << 8 | queue << 16 | queue
+// position << 24, history; q2 = pot, current bet | last full raise << 16,
+// street | raises << 8 | depth << 16 | dealer << 24, traversal lane;
+// q3 = parent node << 4 | action (kEsNoParent at a traversal's first node),
+// its task's path code, slot, legal mask.
+// ---------------------------------------------------------------------------
+
+// Zeroes the generation counters and the launch's record counts.
+__global__ void k_es_gen_begin(DevScratch sc) {
+  const uint32_t i = threadIdx.x;
+  if (i < kEsGens + 1) {
+    sc.gen.task_cnt[i] = 0;
+    sc.gen.node_cnt[i] = 0;
+  }
+  if (i == 0) {
+    sc.cnt->n_es = 0;
+    sc.cnt->n_es_valid = 0;
+  }
+}
+
+// First node index of generation g (the nodes of earlier generations precede).
+__device__ __forceinline__ uint32_t es_gen_first(const EsGenBufs& eb, uint32_t g) {
+  uint32_t nb = 0;
+  for (uint32_t k = 0; k < g; ++k) nb += eb.node_cnt[k];
+  return uniform(nb);
+}
+
+template <int ROW, bool TRACE>
+__global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                const DevGame* __restrict__ tenants, DevScratch sc, uint32_t g) {
+  constexpr int WG = 256;
+  __shared__ uint32_t s_sb[kMaxPlayers * WG];
+  __shared__ uint32_t s_bk[kMaxPlayers * WG];
+  __shared__ uint32_t s_menu[MCCFR_MAX_ACTIONS];
+  __shared__ uint32_t s_rcp[MCCFR_MAX_ACTIONS];
+  const EsGenBufs& eb = sc.gen;
+  const DevSeg& Sg = segs[0];
+  const DevGame& G = tenants[uniform(Sg.tenant)];
+  // (task counts are final: the previous generation wrote them)
+  const uint32_t n_tasks = uniform(g == 0 ? Sg.n : min(eb.task_cnt[g], eb.cap_task));
+  if (n_tasks == 0 || blockIdx.x * WG >= n_tasks) return;  // workgroup-uniform
+  const int tx = threadIdx.x, lane = tx & 63;
+  if (tx < MCCFR_MAX_ACTIONS) {
+    s_menu[tx] = menu_word(G, tx);
+    s_rcp[tx] = G.kind[tx] == MCCFR_ACT_POT_FRAC ? den_reciprocal(G.den[tx]) : 0u;
+  }
+  __syncthreads();  // the last barrier: the returns below need not be uniform
+  // an earlier generation overflowed a buffer: the launch is void
+  if (uniform(sc.cnt->es_overflow) != 0u) return;
+  Rules rules;
+  rules.check_mask = G.check_mask;
+  rules.call_mask = G.call_mask;
+  rules.fold_mask = G.fold_mask;
+  rules.allin_mask = G.allin_mask;
+  rules.raise_mask = G.raise_mask;
+  rules.max_raises = G.max_raises;
+  rules.bb = G.bb;
+  {
+    uint32_t rm = rules.raise_mask;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      rules.fast[i] = 0u;
+      if (rm) {
+        const int a = __builtin_ctz(rm);
+        rm &= rm - 1;
+        const bool min_raise = G.kind[a] == MCCFR_ACT_RAISE_MIN;
+        rules.fast[i] = (min_raise ? 0u : (uint32_t)G.num[a]) | ((min_raise ? 1u : (uint32_t)G.den[a]) << 8) |
+                        ((1u << a) << 16);
+      }
+    }
+    rules.raise_rest = rm;
+  }
+  const int N = G.N;
+  const int hist_bits = G.hist_bits;
+  const uint32_t slot_mask = (uint32_t)((1ull << G.table_log2) - 1ull);
+  const int up = Sg.update_player;
+  const uint32_t allmask = (1u << N) - 1u;
+  const size_t L = sc.cap_lanes;
+  const uint32_t nb = es_gen_first(eb, g);
+  const uint32_t* __restrict__ tin = eb.task[g & 1u];
+  uint32_t* __restrict__ tout = eb.task[(g + 1u) & 1u];
+  uint32_t* sb = s_sb + tx;
+  uint32_t* bk = s_bk + tx;
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
+  const uint32_t stride = gridDim.x * WG;
+  for (uint32_t c0 = uniform(blockIdx.x * WG + (uint32_t)(tx & ~63)); c0 < n_tasks; c0 += stride) {
+    const uint32_t i = c0 + (uint32_t)lane;
+    const bool live = i < n_tasks;
+    uint32_t t = 0, link = kEsNoParent, path = 0, folded = 0, all_in = 0, ab = 0, qmask = 0;
+    int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
+    uint32_t phash = 0;  // trace: sum of the entered (node, action) pairs' FNV-1a
+    DrawStream rng;
+    if (live) {
+      if (g == 0) {
+        // the traversal's root (cfr_gpu.cu:954-981, or the tenant's postflop root), as k_walk
+        t = Sg.lane0 + i;
+        dealer = (int)sc.root_dealer[t];
+        const int root_street = G.root_street;
+#pragma unroll
+        for (int p = 0; p < kMaxPlayers; ++p) {
+          if (p < N) {
+            sb[p * WG] = (uint32_t)G.root_stack[p] | ((uint32_t)root_street << 30);
+            bk[p * WG] = sc.root_bkt[(size_t)p * L + t];
+          }
+        }
+        if (root_street == 0) {
+          const int sb_pos = (N == 2) ? dealer : wrap(dealer + 1, N);
+          const int bb_pos = (N == 2) ? wrap(dealer + 1, N) : wrap(dealer + 2, N);
+          const int s_sb = (int)(sb[sb_pos * WG] & kStackMask);
+          const int sb_amt = G.sb < s_sb ? G.sb : s_sb;
+          sb[sb_pos * WG] = (uint32_t)(s_sb - sb_amt) | ((uint32_t)sb_amt << 15);
+          if (s_sb - sb_amt == 0) all_in |= 1u << sb_pos;
+          const int s_bb = (int)(sb[bb_pos * WG] & kStackMask);
+          const int bb_amt = rules.bb < s_bb ? rules.bb : s_bb;
+          sb[bb_pos * WG] = (uint32_t)(s_bb - bb_amt) | ((uint32_t)bb_amt << 15);
+          if (s_bb - bb_amt == 0) all_in |= 1u << bb_pos;
+          pot = sb_amt + bb_amt;
+          current_bet = bb_amt;
+          qmask = allmask & ~all_in;
+          qpos = wrap(bb_pos + 1, N);
+        } else {
+          folded = G.root_folded;
+          all_in = G.root_allin;
+          pot = G.root_pot;
+          street = root_street;
+          ab = G.root_ab;
+          qmask = allmask & ~(folded | all_in);
+          qpos = wrap(dealer + 1, N);
+        }
+        rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + i, G.tag);
+        rng.k = G.walk_k0;  // the deal (and the preflop dealer draw) came first
+      } else {
+        // the task's node state, then its action (apply_action_gpu +
+        // rebuild_queue, cfr_gpu.cu:543-585, for the update player)
+        link = tin[i];
+        const uint4* q = eb.node + (size_t)(link >> 4) * 4;
+        const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        const uint32_t w[kMaxPlayers] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+        t = q2.w;
+#pragma unroll
+        for (int p = 0; p < kMaxPlayers; ++p) {
+          if (p < N) {
+            sb[p * WG] = w[p];
+            bk[p * WG] = sc.root_bkt[(size_t)p * L + t];  // the turn byte is fetched again if needed
+          }
+        }
+        folded = q1.z & 0xFFu;
+        all_in = (q1.z >> 8) & 0xFFu;
+        qmask = (q1.z >> 16) & 0xFFu;
+        ab = q1.w;
+        pot = (int)q2.x;
+        current_bet = (int)(q2.y & 0xFFFFu);
+        lfr = (int)(q2.y >> 16);
+        street = (int)(q2.z & 0xFFu);
+        raises = (int)((q2.z >> 8) & 0xFFu);
+        depth = (int)((q2.z >> 16) & 0xFFu);
+        dealer = (int)(q2.z >> 24);
+        const int a = (int)(link & 15u);
+        path = es_child_path(q3.y, a);
+        if (TRACE) phash += fnv_path(2166136261u, q3.z, (uint32_t)a);
+        const uint32_t sbv = sb[up * WG];
+        const int stk = (int)(sbv & kStackMask);
+        const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+        const int to_call = current_bet - bet;
+        qmask &= ~(1u << up);
+        qpos = wrap(up + 1, N);
+        const int chips = chips_lds(G, s_menu, s_rcp, a, pot, stk, to_call, current_bet, lfr);
+        ab = ((ab << hist_bits) | (uint32_t)a) & kHistoryMask;
+        if (rules.fold_mask >> a & 1u) {
+          folded |= 1u << up;
+        } else {
+          const int ns = stk - chips, nbt = bet + chips;
+          sb[up * WG] = (uint32_t)ns | ((uint32_t)nbt << 15) | ((uint32_t)street << 30);
+          pot += chips;
+          if (ns == 0) all_in |= 1u << up;
+          if (nbt > current_bet) {
+            const int raise = nbt - current_bet;
+            if (raise >= lfr || current_bet == 0) lfr = raise;
+            current_bet = nbt;
+            ++raises;
+            qmask = allmask & ~(folded | all_in) & ~(1u << up);
+          }
+        }
+        ++depth;
+        rng.init(G.seed_lo, G.seed_hi, Sg.first_tid + (t - Sg.lane0), G.tag);
+        rng.path = path;  // the task's own stream, from draw 0
+      }
+    }
+    // walk to a leaf (status 1 fold, 2 showdown, 3 depth guard) or an update
+    // node (status 4); one recursion level of traverse_gpu per pass
+    int status = live ? 0 : 9;
+    float u = 0.0f;
+    uint32_t n_slot = 0, n_vm = 0, dec = 0;
+    while (__ballot(status == 0)) {
+      if (status != 0) continue;
+      if (depth >= kMaxDepth) {  // depth guard (cfr_gpu.cu:661)
+        status = 3;
+        continue;
+      }
+      const uint32_t alive = allmask & ~folded;
+      if (__popc(alive) == 1) {
+        const int inv_up = G.stack[up] - (int)(sb[up * WG] & kStackMask);
+        u = ((int)__builtin_ctz(alive) == up) ? (float)(pot - inv_up) : (float)(-inv_up);
+        status = 1;
+        continue;
+      }
+      if (qmask == 0) {
+        // street change (cfr_gpu.cu:668-692), then the next level's decision in
+        // this pass; with nobody left to act, or after the river, the showdown
+        if (street == 3) {
+          status = 2;
+          continue;
+        }
+        ++street;
+        current_bet = 0;
+        lfr = rules.bb;
+        raises = 0;
+        qmask = allmask & ~(folded | all_in);
+        qpos = wrap(dealer + 1, N);
+        if (qmask == 0) {
+          status = 2;
+          continue;
+        }
+        ++depth;
+        if (depth >= kMaxDepth) {
+          status = 3;
+          continue;
+        }
+      }
+      const uint32_t rot = ((qmask >> qpos) | (qmask << (N - qpos))) & allmask;
+      const int player = wrap(qpos + (int)__builtin_ctz(rot), N);
+      const uint32_t sbv = sb[player * WG];
+      const int stk = (int)(sbv & kStackMask);
+      const int bet = ((int)(sbv >> 30) == street) ? (int)((sbv >> 15) & kStackMask) : 0;
+      const int to_call = current_bet - bet;
+      const uint32_t vm = rules_mask(rules, s_menu, pot, stk, to_call, current_bet, lfr, raises);
+      uint32_t Bp = bk[player * WG];
+      if (street == 2 && ((Bp >> 16) & 0xFFu) == kTurnUnknown) {  // first turn decision of this seat
+        const uint32_t tb = sc.turn6[sc.root_tidx[(size_t)player * L + t]];
+        Bp = (Bp & 0xFF00FFFFu) | (tb << 16);
+        bk[player * WG] = Bp;
+      }
+      const uint32_t bbk = street == 0 ? 0u : (Bp >> (8 * street)) & 0xFFu;
+      const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
+      ++dec;
+      if (player == up) {
+        status = 4;
+        n_slot = slot;
+        n_vm = vm;
+        continue;
+      }
+      // sample_action (cfr_gpu.cu:510-537) on the BF16 snapshot, as k_walk
+      float sg[ROW];
+      load_bf16_row<ROW>(G.sig + (size_t)slot * ROW, sg);
+      float cum[ROW];
+      float total = 0.0f;
+#pragma unroll
+      for (int a = 0; a < ROW; ++a) {
+        total += (vm >> a & 1u) ? sg[a] : 0.0f;
+        cum[a] = total;
+      }
+      int act = -1;
+      if (total >= 1e-7f) {
+        const float r = u01(rng.next()) * total;
+        int below = 0;
+#pragma unroll
+        for (int a = 0; a < ROW; ++a) below += cum[a] < r ? 1 : 0;
+        act = below;
+      }
+      if (act < 0) {  // passive fallback (cfr_gpu.cu:498-508)
+        for (int k = 0; k < G.n_passive; ++k) {
+          const int a = (int)(G.passive_order[k] & 15u);
+          if (vm >> a & 1u) {
+            act = a;
+            break;
+          }
+        }
+        if (act < 0) act = 0;
+      }
+      if (TRACE) phash += fnv_path(2166136261u, slot, (uint32_t)act);
+      // apply_action_gpu + rebuild_queue (cfr_gpu.cu:543-585)
+      qmask &= ~(1u << player);
+      qpos = wrap(player + 1, N);
+      const int chips = chips_lds(G, s_menu, s_rcp, act, pot, stk, to_call, current_bet, lfr);
+      ab = ((ab << hist_bits) | (uint32_t)act) & kHistoryMask;
+      if (rules.fold_mask >> act & 1u) {
+        folded |= 1u << player;
+      } else {
+        const int ns = stk - chips, nbt = bet + chips;
+        sb[player * WG] = (uint32_t)ns | ((uint32_t)nbt << 15) | ((uint32_t)street << 30);
+        pot += chips;
+        if (ns == 0) all_in |= 1u << player;
+        if (nbt > current_bet) {
+          const int raise = nbt - current_bet;
+          if (raise >= lfr || current_bet == 0) lfr = raise;
+          current_bet = nbt;
+          ++raises;
+          qmask = allmask & ~(folded | all_in) & ~(1u << player);
+        }
+      }
+      ++depth;
+    }
+    const bool leaf = status >= 1 && status <= 3;
+    if (status == 2) {  // showdown payoff (cfr_gpu.cu:694-752)
+      int inv[kMaxPlayers];
+      uint32_t rkv[kMaxPlayers];
+#pragma unroll
+      for (int p = 0; p < kMaxPlayers; ++p) {
+        inv[p] = p < N ? G.stack[p] - (int)(sb[p * WG] & kStackMask) : 0;
+        rkv[p] = p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+      }
+      u = showdown_payoff(up, folded, inv, rkv);
+    }
+    if (leaf) {  // status 3: the depth guard's 0 (cfr_gpu.cu:661)
+      if (link != kEsNoParent)
+        sc.es_rd[(size_t)(link >> 4) * ROW + (link & 15u)] = u;
+      else if (TRACE)
+        eb.root_v[t] = u;
+    }
+    // update nodes: an index each, then one task per legal action
+    const bool isnode = status == 4;
+    const uint64_t nmask = __ballot(isnode);
+    uint32_t node = 0xFFFFFFFFu, nch = 0, cfirst = 0;
+    if (nmask) {
+      uint32_t nbase = 0;
+      if (lane == 0) nbase = atomicAdd(&eb.node_cnt[g], (uint32_t)__popcll(nmask));
+      nbase = __shfl(nbase, 0, 64);
+      if (isnode) {
+        node = nb + nbase + (uint32_t)__popcll(nmask & lanes_below);
+        nch = node < eb.cap_node ? (uint32_t)__popc(n_vm) : 0u;
+        if (node >= eb.cap_node) sc.cnt->es_overflow = 1u;
+      }
+      uint32_t incl = nch;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+      }
+      const uint32_t total = __shfl(incl, 63, 64);
+      uint32_t cbase = 0;
+      if (lane == 0 && total) cbase = atomicAdd(&eb.task_cnt[g + 1], total);
+      cbase = __shfl(cbase, 0, 64);
+      cfirst = cbase + incl - nch;
+      if (total && cbase + total > eb.cap_task) {
+        if (lane == 0) sc.cnt->es_overflow = 3u;
+        nch = 0;  // the next generation does not run
+      }
+    }
+    if (isnode && node < eb.cap_node) {
+      // the node's state before its action, then a task per legal action
+      uint32_t w[kMaxPlayers];
+#pragma unroll
+      for (int p = 0; p < kMaxPlayers; ++p) w[p] = p < N ? sb[p * WG] : 0u;
+      uint4* q = eb.node + (size_t)node * 4;
+      q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      q[1] = make_uint4(w[4], w[5], folded | (all_in << 8) | (qmask << 16) | ((uint32_t)qpos << 24), ab);
+      q[2] = make_uint4((uint32_t)pot, (uint32_t)current_bet | ((uint32_t)lfr << 16),
+                        (uint32_t)street | ((uint32_t)raises << 8) | ((uint32_t)depth << 16) | ((uint32_t)dealer << 24),
+                        t);
+      q[3] = make_uint4(link, path, n_slot, n_vm);
+      uint32_t ci = cfirst;
+      for (uint32_t left = nch ? n_vm : 0u; left; left &= left - 1u) tout[ci++] = node << 4 | (uint32_t)__builtin_ctz(left);
+    }
+    if (TRACE && live) {
+      // per-traversal sums for the trace: the root task writes them first
+      uint32_t* acc = eb.tr_acc + (size_t)t * 8;
+      const uint32_t upd = isnode ? 1u : 0u, ul = isnode ? (uint32_t)__popc(n_vm) : 0u, lv = leaf ? 1u : 0u;
+      const uint32_t tb = leaf ? 1u << status : 0u;
+      if (g == 0) {
+        acc[0] = dec;
+        acc[1] = upd;
+        acc[2] = ul;
+        acc[3] = lv;
+        acc[4] = tb;
+        acc[5] = phash;
+      } else {
+        atomicAdd(acc + 0, dec);
+        atomicAdd(acc + 1, upd);
+        atomicAdd(acc + 2, ul);
+        atomicAdd(acc + 3, lv);
+        atomicOr(acc + 4, tb);
+        atomicAdd(acc + 5, phash);
+      }
+    }
+  }
+}
+
+// Generation g's update nodes, one lane each: v(I) = sum_a sigma(a) v(a)
+// (sigma from the FP32 regrets, in action order, as k_walk_es), the record
+// (at the node's index) and its row of v(a) - v(I), and v(I) into the
+// parent's child slot (or the traversal's root value).
+template <int ROW, bool TRACE>
+__global__ __launch_bounds__(256) void k_es_back(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                                 const DevGame* __restrict__ tenants, DevScratch sc, uint32_t g) {
+  const EsGenBufs& eb = sc.gen;
+  if (sc.cnt->es_overflow) return;
+  const uint32_t n = uniform(eb.node_cnt[g]);
+  if (n == 0 || blockIdx.x * blockDim.x >= n) return;
+  const uint32_t nb = es_gen_first(eb, g);
+  const DevGame& G = tenants[uniform(segs[0].tenant)];
+  const int A = G.num_actions;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t node = nb + j;  // below cap_node: no overflow was flagged
+    const uint4 q3 = eb.node[(size_t)node * 4 + 3];
+    const uint32_t slot = q3.z, vm = q3.w;
+    float rr[ROW], v[ROW], rd[ROW];
+    load_f32_row<ROW>(G.R + (size_t)slot * 2 * ROW, rr);
+    load_f32_row<ROW>(sc.es_rd + (size_t)node * ROW, v);  // child values (legal actions)
+    float sum_r = 0.0f;
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) {
+      const bool legal = a < A && (vm >> a & 1u);
+      rr[a] = legal ? fmaxf(0.0f, rr[a]) : 0.0f;
+      sum_r += rr[a];
+      v[a] = legal ? v[a] : 0.0f;
+    }
+    const float uni = 1.0f / (float)__popc(vm);
+    float vi = 0.0f;
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) {
+      const bool legal = a < A && (vm >> a & 1u);
+      const float sig = sum_r > 1e-7f ? rr[a] / sum_r : (legal ? uni : 0.0f);
+      if (legal) vi += sig * v[a];
+    }
+#pragma unroll
+    for (int a = 0; a < ROW; ++a) rd[a] = (vm >> a & 1u) ? v[a] - vi : 0.0f;
+    store_f32_row<ROW>(sc.es_rd + (size_t)node * ROW, rd);
+    sc.key[0][node] = slot;  // one segment: key = slot
+    sc.val[0][node] = make_uint2((vm << 8) | kEsRecord, node);
+    const uint32_t link = q3.x;
+    if (link != kEsNoParent)
+      sc.es_rd[(size_t)(link >> 4) * ROW + (link & 15u)] = vi;
+    else if (TRACE)
+      eb.root_v[eb.node[(size_t)node * 4 + 2].w] = vi;
+  }
+}
+
+// The launch's record count (one per node) for the sort; a count beyond the
+// node capacity is flagged.
+__global__ void k_es_gen_end(DevScratch sc) {
+  if (threadIdx.x != 0) return;
+  uint32_t total = 0;
+  bool over = false;
+  for (uint32_t k = 0; k <= kEsGens; ++k) {
+    const uint32_t c = sc.gen.node_cnt[k];
+    over = over || c > sc.gen.cap_node - total;
+    total = over ? total : total + c;
+  }
+  if (over && !sc.cnt->es_overflow) sc.cnt->es_overflow = 1u;
+  sc.cnt->n_es = total;
+  sc.cnt->n_es_valid = total;
+}
+
+// Trace records of a one-segment generation launch.
+__global__ __launch_bounds__(256) void k_es_gen_trace(const DevSeg* __restrict__ segs, DevScratch sc,
+                                                      mccfr_trace* __restrict__ trace) {
+  const DevSeg& Sg = segs[0];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Sg.n) return;
+  const uint32_t t = Sg.lane0 + i;
+  const uint32_t* acc = sc.gen.tr_acc + (size_t)t * 8;
+  mccfr_trace r;
+  r.utility = sc.gen.root_v[t];
+  r.path_hash = 2166136261u + acc[5];
+  r.decisions = (uint16_t)acc[0];
+  r.updates = (uint8_t)acc[1];
+  r.terminal = (uint8_t)acc[4];
+  r.update_legal = (uint16_t)acc[2];
+  r.reserved = (uint16_t)acc[3];
+  trace[t] = r;
+}
+
 }  // namespace
 
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
@@ -2343,8 +2835,44 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
   if (after_deal) (void)hipEventRecord(after_deal, s);
   DevScratch ws = sc;
   ws.turn6 = in.bt.turn6;
+  if (lc.es && lc.n_segs == 1 && sc.gen.task[0]) {
+    // external sampling by generations: each generation's tasks walk to
+    // their first update node or leaf; then the nodes' values, deepest
+    // generation first
+    hipLaunchKernelGGL(k_es_gen_begin, dim3(1), dim3(64), 0, s, ws);
+    const uint32_t g0 = (lc.lanes + 255u) / 256u;
+    const uint32_t gn = std::max<uint32_t>(1u, std::min<uint32_t>(4096u, (lc.lanes * 2u + 255u) / 256u));
+    for (uint32_t g = 0; g < kEsGens; ++g) {
+      const dim3 grid(g == 0 ? g0 : gn);
+#define MCCFR_GEN(R, T) hipLaunchKernelGGL((k_es_gen<R, T>), grid, dim3(256), 0, s, lc, segs, tenants, ws, g)
+      if (row == 8) {
+        if (trace) MCCFR_GEN(8, true);
+        else MCCFR_GEN(8, false);
+      } else {
+        if (trace) MCCFR_GEN(16, true);
+        else MCCFR_GEN(16, false);
+      }
+#undef MCCFR_GEN
+    }
+    for (uint32_t g = kEsGens; g-- > 0;) {
+#define MCCFR_BACK(R, T) hipLaunchKernelGGL((k_es_back<R, T>), dim3(gn), dim3(256), 0, s, lc, segs, tenants, ws, g)
+      if (row == 8) {
+        if (trace) MCCFR_BACK(8, true);
+        else MCCFR_BACK(8, false);
+      } else {
+        if (trace) MCCFR_BACK(16, true);
+        else MCCFR_BACK(16, false);
+      }
+#undef MCCFR_BACK
+    }
+    hipLaunchKernelGGL(k_es_gen_end, dim3(1), dim3(64), 0, s, ws);
+    if (trace) hipLaunchKernelGGL(k_es_gen_trace, dim3(g0), dim3(256), 0, s, segs, ws, trace);
+    if (after_walk) (void)hipEventRecord(after_walk, s);
+    return;
+  }
   if (lc.es) {
-    // external sampling: one lane per traversal, depth first, showdowns settled in the walk
+    // external sampling over several segments: one lane per traversal, depth
+    // first, showdowns settled in the walk
     hipLaunchKernelGGL(k_es_begin, dim3(1), dim3(64), 0, s, sc);
     const dim3 grid(es_grid(lc.lanes));
 #define MCCFR_ES(R, T) \
@@ -2376,6 +2904,37 @@ size_t es_stack_bytes(int row, uint32_t lanes) {
   const int words = row == 8 ? es_frame_words<8>() : es_frame_words<16>();
   return (size_t)kEsFrames * (size_t)words * es_grid(lanes) * 256u * sizeof(uint32_t);
 }
+
+size_t es_gen_bytes(uint32_t cap_lanes) {
+  const auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t cap_node = (size_t)cap_lanes * kMaxDepth;
+  const size_t cap_task = cap_node;
+  return 2 * al(cap_task * sizeof(uint32_t)) + 2 * al((kEsGens + 1) * sizeof(uint32_t)) +
+         al(cap_node * sizeof(uint4) * 4) + al((size_t)cap_lanes * sizeof(float)) +
+         al((size_t)cap_lanes * 8 * sizeof(uint32_t));
+}
+
+void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes) {
+  const auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t cap_node = (size_t)cap_lanes * kMaxDepth;
+  const size_t cap_task = cap_node;
+  char* p = base;
+  const auto take = [&](size_t bytes) {
+    char* q = p;
+    p += al(bytes);
+    return q;
+  };
+  b.task[0] = reinterpret_cast<uint32_t*>(take(cap_task * sizeof(uint32_t)));
+  b.task[1] = reinterpret_cast<uint32_t*>(take(cap_task * sizeof(uint32_t)));
+  b.task_cnt = reinterpret_cast<uint32_t*>(take((kEsGens + 1) * sizeof(uint32_t)));
+  b.node_cnt = reinterpret_cast<uint32_t*>(take((kEsGens + 1) * sizeof(uint32_t)));
+  b.node = reinterpret_cast<uint4*>(take(cap_node * sizeof(uint4) * 4));
+  b.root_v = reinterpret_cast<float*>(take((size_t)cap_lanes * sizeof(float)));
+  b.tr_acc = reinterpret_cast<uint32_t*>(take((size_t)cap_lanes * 8 * sizeof(uint32_t)));
+  b.cap_task = (uint32_t)cap_task;
+  b.cap_node = (uint32_t)cap_node;
+}
+
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t key_bits) {
