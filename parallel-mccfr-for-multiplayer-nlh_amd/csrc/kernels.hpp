@@ -28,16 +28,21 @@ constexpr uint32_t kSortBins = 1u << kDigitBits;
 // level below its node, so the depth guard (20 levels) bounds the
 // generations.
 constexpr uint32_t kEsGens = 21;
+// Nodes and tasks are allocated in kEsRegions regions, each with its own
+// counters on a line of its own (a wave uses region wave % kEsRegions), since
+// atomics on one address serialize across the device.
+constexpr uint32_t kEsRegions = 64;
+constexpr uint32_t kEsCtrStride = 32;  // dwords between two counters (128 bytes)
 // Buffers for a scratch of cap_lanes lanes: a node per record of the sort
 // input (cap_node = 20 per lane), and as many tasks per generation.
 struct EsGenBufs {
-  uint32_t* task[2];   // [cap_task] a task of generation >= 1: its node << 4 | action, by generation parity
-  uint32_t* task_cnt;  // [kEsGens + 1] tasks of each generation (0: the roots, not counted)
-  uint32_t* node_cnt;  // [kEsGens + 1] update-player nodes each generation made
-  uint4* node;         // [cap_node][4] a node's state before its action, its slot and legal mask
+  uint32_t* task[2];   // [kEsRegions][cap_task_r] tasks of generation >= 1: node << 4 | action, by parity
+  uint32_t* task_cnt;  // [kEsGens + 1][kEsRegions][kEsCtrStride] tasks of each generation and region
+  uint32_t* node_cnt;  // [kEsGens + 1][kEsRegions][kEsCtrStride] nodes each generation made in each region
+  uint4* node;         // [kEsRegions][cap_node_r][4] a node's state before its action, its slot and legal mask
   float* root_v;       // [cap_lanes] root value of each traversal (trace launches)
   uint32_t* tr_acc;    // [cap_lanes][8] trace sums per traversal (trace launches)
-  uint32_t cap_task, cap_node;
+  uint32_t cap_task_r, cap_node_r;  // per region
 };
 constexpr uint32_t kEsNoParent = 0xFFFFFFFFu;
 

@@ -2362,23 +2362,56 @@ void walk_dispatch(hipStream_t s, const LaunchCfg& lc, const DevSeg* segs, const
 // ---------------------------------------------------------------------------
 
 // Zeroes the generation counters and the launch's record counts.
-__global__ void k_es_gen_begin(DevScratch sc) {
-  const uint32_t i = threadIdx.x;
-  if (i < kEsGens + 1) {
-    sc.gen.task_cnt[i] = 0;
-    sc.gen.node_cnt[i] = 0;
+__global__ __launch_bounds__(256) void k_es_gen_begin(DevScratch sc) {
+  for (uint32_t i = threadIdx.x; i < (kEsGens + 1) * kEsRegions; i += blockDim.x) {
+    sc.gen.task_cnt[i * kEsCtrStride] = 0;
+    sc.gen.node_cnt[i * kEsCtrStride] = 0;
   }
-  if (i == 0) {
+  if (threadIdx.x == 0) {
     sc.cnt->n_es = 0;
     sc.cnt->n_es_valid = 0;
   }
 }
 
-// First node index of generation g (the nodes of earlier generations precede).
-__device__ __forceinline__ uint32_t es_gen_first(const EsGenBufs& eb, uint32_t g) {
-  uint32_t nb = 0;
-  for (uint32_t k = 0; k < g; ++k) nb += eb.node_cnt[k];
-  return uniform(nb);
+__device__ __forceinline__ uint32_t& es_ctr(uint32_t* c, uint32_t g, uint32_t r) {
+  return c[((size_t)g * kEsRegions + r) * kEsCtrStride];
+}
+
+// A generation's items (tasks or nodes) lie in kEsRegions regions with
+// counts cnt(g, r).  Lane r of the calling wave holds region r's count
+// (clamped to the region's capacity) and their inclusive prefix; item i of
+// the generation is in the region whose prefix first exceeds i.
+struct EsRegionScan {
+  uint32_t count, incl, total;
+};
+__device__ __forceinline__ EsRegionScan es_region_scan(uint32_t* ctr, uint32_t g, uint32_t cap_r) {
+  const int lane = threadIdx.x & 63;
+  static_assert(kEsRegions == 64, "one region per lane");
+  EsRegionScan rs;
+  rs.count = min(es_ctr(ctr, g, (uint32_t)lane), cap_r);
+  uint32_t incl = rs.count;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  rs.incl = incl;
+  rs.total = uniform(__shfl(incl, 63, 64));
+  return rs;
+}
+// Region of item i (< total) and its position within the region.
+__device__ __forceinline__ uint32_t es_region_of(const EsRegionScan& rs, uint32_t i, uint32_t* pos) {
+  uint32_t lo = 0, hi = 63;  // the first region whose inclusive prefix exceeds i
+#pragma unroll
+  for (int step = 0; step < 6; ++step) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t v = (uint32_t)__shfl((int)rs.incl, (int)mid, 64);
+    if (v <= i) lo = mid + 1;
+    else hi = mid;
+  }
+  const uint32_t before = (uint32_t)__shfl((int)(rs.incl - rs.count), (int)lo, 64);
+  *pos = i - before;
+  return lo;
 }
 
 template <int ROW, bool TRACE>
@@ -2393,7 +2426,9 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
   const DevSeg& Sg = segs[0];
   const DevGame& G = tenants[uniform(Sg.tenant)];
   // (task counts are final: the previous generation wrote them)
-  const uint32_t n_tasks = uniform(g == 0 ? Sg.n : min(eb.task_cnt[g], eb.cap_task));
+  EsRegionScan ts{0u, 0u, 0u};
+  if (g > 0) ts = es_region_scan(eb.task_cnt, g, eb.cap_task_r);
+  const uint32_t n_tasks = g == 0 ? Sg.n : ts.total;
   if (n_tasks == 0 || blockIdx.x * WG >= n_tasks) return;  // workgroup-uniform
   const int tx = threadIdx.x, lane = tx & 63;
   if (tx < MCCFR_MAX_ACTIONS) {
@@ -2432,7 +2467,11 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
   const int up = Sg.update_player;
   const uint32_t allmask = (1u << N) - 1u;
   const size_t L = sc.cap_lanes;
-  const uint32_t nb = es_gen_first(eb, g);
+  // the wave's region for the nodes and tasks it makes, and its first free node there
+  const uint32_t wreg = uniform(((blockIdx.x * WG + (uint32_t)tx) >> 6) % kEsRegions);
+  uint32_t nb = 0;
+  for (uint32_t k = 0; k < g; ++k) nb += es_ctr(eb.node_cnt, k, wreg);
+  nb = uniform(nb);
   const uint32_t* __restrict__ tin = eb.task[g & 1u];
   uint32_t* __restrict__ tout = eb.task[(g + 1u) & 1u];
   uint32_t* sb = s_sb + tx;
@@ -2442,6 +2481,12 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
   for (uint32_t c0 = uniform(blockIdx.x * WG + (uint32_t)(tx & ~63)); c0 < n_tasks; c0 += stride) {
     const uint32_t i = c0 + (uint32_t)lane;
     const bool live = i < n_tasks;
+    uint32_t tref = 0;  // a task of generation >= 1: node << 4 | action, from its region's list
+    if (g > 0) {  // (every lane takes part in the region search's shuffles)
+      uint32_t pos;
+      const uint32_t r = es_region_of(ts, live ? i : 0u, &pos);
+      if (live) tref = tin[(size_t)r * eb.cap_task_r + pos];
+    }
     uint32_t t = 0, link = kEsNoParent, path = 0, folded = 0, all_in = 0, ab = 0, qmask = 0;
     int qpos = 0, pot = 0, current_bet = 0, lfr = rules.bb, street = 0, raises = 0, depth = 0, dealer = 0;
     uint32_t phash = 0;  // trace: sum of the entered (node, action) pairs' FNV-1a
@@ -2488,7 +2533,7 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
       } else {
         // the task's node state, then its action (apply_action_gpu +
         // rebuild_queue, cfr_gpu.cu:543-585, for the update player)
-        link = tin[i];
+        link = tref;
         const uint4* q = eb.node + (size_t)(link >> 4) * 4;
         const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         const uint32_t w[kMaxPlayers] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
@@ -2676,15 +2721,19 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
     // update nodes: an index each, then one task per legal action
     const bool isnode = status == 4;
     const uint64_t nmask = __ballot(isnode);
+    // (indices within the wave's region wreg; node = wreg * cap_node_r + local)
     uint32_t node = 0xFFFFFFFFu, nch = 0, cfirst = 0;
+    bool stored = false;
     if (nmask) {
       uint32_t nbase = 0;
-      if (lane == 0) nbase = atomicAdd(&eb.node_cnt[g], (uint32_t)__popcll(nmask));
+      if (lane == 0) nbase = atomicAdd(&es_ctr(eb.node_cnt, g, wreg), (uint32_t)__popcll(nmask));
       nbase = __shfl(nbase, 0, 64);
       if (isnode) {
-        node = nb + nbase + (uint32_t)__popcll(nmask & lanes_below);
-        nch = node < eb.cap_node ? (uint32_t)__popc(n_vm) : 0u;
-        if (node >= eb.cap_node) sc.cnt->es_overflow = 1u;
+        const uint32_t loc = nb + nbase + (uint32_t)__popcll(nmask & lanes_below);
+        stored = loc < eb.cap_node_r;
+        node = wreg * eb.cap_node_r + loc;
+        nch = stored ? (uint32_t)__popc(n_vm) : 0u;
+        if (!stored) sc.cnt->es_overflow = 1u;
       }
       uint32_t incl = nch;
 #pragma unroll
@@ -2694,15 +2743,15 @@ __global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __re
       }
       const uint32_t total = __shfl(incl, 63, 64);
       uint32_t cbase = 0;
-      if (lane == 0 && total) cbase = atomicAdd(&eb.task_cnt[g + 1], total);
+      if (lane == 0 && total) cbase = atomicAdd(&es_ctr(eb.task_cnt, g + 1, wreg), total);
       cbase = __shfl(cbase, 0, 64);
-      cfirst = cbase + incl - nch;
-      if (total && cbase + total > eb.cap_task) {
+      cfirst = wreg * eb.cap_task_r + cbase + incl - nch;
+      if (total && cbase + total > eb.cap_task_r) {
         if (lane == 0) sc.cnt->es_overflow = 3u;
         nch = 0;  // the next generation does not run
       }
     }
-    if (isnode && node < eb.cap_node) {
+    if (stored) {
       // the node's state before its action, then a task per legal action
       uint32_t w[kMaxPlayers];
 #pragma unroll
@@ -2749,14 +2798,30 @@ template <int ROW, bool TRACE>
 __global__ __launch_bounds__(256) void k_es_back(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                  const DevGame* __restrict__ tenants, DevScratch sc, uint32_t g) {
   const EsGenBufs& eb = sc.gen;
-  if (sc.cnt->es_overflow) return;
-  const uint32_t n = uniform(eb.node_cnt[g]);
+  if (uniform(sc.cnt->es_overflow) != 0u) return;  // (no barrier in this kernel)
+  const EsRegionScan ns = es_region_scan(eb.node_cnt, g, eb.cap_node_r);
+  const uint32_t n = ns.total;
   if (n == 0 || blockIdx.x * blockDim.x >= n) return;
-  const uint32_t nb = es_gen_first(eb, g);
+  // lane r: region r's nodes of earlier generations (where generation g's
+  // start there); their sum is generation g's first record
+  const int lane = threadIdx.x & 63;
+  uint32_t before = 0;
+  for (uint32_t k = 0; k < g; ++k) before += es_ctr(eb.node_cnt, k, (uint32_t)lane);
+  uint32_t rec0 = before;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) rec0 += __shfl_xor(rec0, d, 64);
+  rec0 = uniform(rec0);
   const DevGame& G = tenants[uniform(segs[0].tenant)];
   const int A = G.num_actions;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const uint32_t node = nb + j;  // below cap_node: no overflow was flagged
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t c0 = uniform(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); c0 < n; c0 += stride) {
+    const uint32_t j = c0 + (uint32_t)lane;
+    const bool live = j < n;
+    uint32_t pos;
+    const uint32_t r = es_region_of(ns, live ? j : 0u, &pos);
+    const uint32_t first = (uint32_t)__shfl((int)before, (int)r, 64);
+    if (!live) continue;
+    const uint32_t node = r * eb.cap_node_r + first + pos;  // below the region's capacity: no overflow
     const uint4 q3 = eb.node[(size_t)node * 4 + 3];
     const uint32_t slot = q3.z, vm = q3.w;
     float rr[ROW], v[ROW], rd[ROW];
@@ -2781,8 +2846,10 @@ __global__ __launch_bounds__(256) void k_es_back(LaunchCfg lc, const DevSeg* __r
 #pragma unroll
     for (int a = 0; a < ROW; ++a) rd[a] = (vm >> a & 1u) ? v[a] - vi : 0.0f;
     store_f32_row<ROW>(sc.es_rd + (size_t)node * ROW, rd);
-    sc.key[0][node] = slot;  // one segment: key = slot
-    sc.val[0][node] = make_uint2((vm << 8) | kEsRecord, node);
+    // the node's record, at generation g's records' next place (one segment: key = slot)
+    const uint32_t rec = rec0 + j;
+    sc.key[0][rec] = slot;
+    sc.val[0][rec] = make_uint2((vm << 8) | kEsRecord, node);
     const uint32_t link = q3.x;
     if (link != kEsNoParent)
       sc.es_rd[(size_t)(link >> 4) * ROW + (link & 15u)] = vi;
@@ -2791,20 +2858,17 @@ __global__ __launch_bounds__(256) void k_es_back(LaunchCfg lc, const DevSeg* __r
   }
 }
 
-// The launch's record count (one per node) for the sort; a count beyond the
-// node capacity is flagged.
-__global__ void k_es_gen_end(DevScratch sc) {
-  if (threadIdx.x != 0) return;
-  uint32_t total = 0;
-  bool over = false;
-  for (uint32_t k = 0; k <= kEsGens; ++k) {
-    const uint32_t c = sc.gen.node_cnt[k];
-    over = over || c > sc.gen.cap_node - total;
-    total = over ? total : total + c;
+// The launch's record count (one per node) for the sort.
+__global__ __launch_bounds__(64) void k_es_gen_end(DevScratch sc) {
+  const uint32_t r = threadIdx.x;  // one region per lane
+  uint32_t c = 0;
+  for (uint32_t k = 0; k <= kEsGens; ++k) c += min(es_ctr(sc.gen.node_cnt, k, r), sc.gen.cap_node_r);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (r == 0) {
+    sc.cnt->n_es = c;
+    sc.cnt->n_es_valid = c;
   }
-  if (over && !sc.cnt->es_overflow) sc.cnt->es_overflow = 1u;
-  sc.cnt->n_es = total;
-  sc.cnt->n_es_valid = total;
 }
 
 // Trace records of a one-segment generation launch.
@@ -2839,7 +2903,7 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
     // external sampling by generations: each generation's tasks walk to
     // their first update node or leaf; then the nodes' values, deepest
     // generation first
-    hipLaunchKernelGGL(k_es_gen_begin, dim3(1), dim3(64), 0, s, ws);
+    hipLaunchKernelGGL(k_es_gen_begin, dim3(1), dim3(256), 0, s, ws);
     const uint32_t g0 = (lc.lanes + 255u) / 256u;
     const uint32_t gn = std::max<uint32_t>(1u, std::min<uint32_t>(4096u, (lc.lanes * 2u + 255u) / 256u));
     for (uint32_t g = 0; g < kEsGens; ++g) {
@@ -2907,32 +2971,32 @@ size_t es_stack_bytes(int row, uint32_t lanes) {
 
 size_t es_gen_bytes(uint32_t cap_lanes) {
   const auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t cap_node = (size_t)cap_lanes * kMaxDepth;
-  const size_t cap_task = cap_node;
-  return 2 * al(cap_task * sizeof(uint32_t)) + 2 * al((kEsGens + 1) * sizeof(uint32_t)) +
-         al(cap_node * sizeof(uint4) * 4) + al((size_t)cap_lanes * sizeof(float)) +
+  const size_t cap_r = (size_t)cap_lanes * kMaxDepth / kEsRegions;  // nodes and tasks per region
+  const size_t ctrs = (size_t)(kEsGens + 1) * kEsRegions * kEsCtrStride;
+  return 2 * al(kEsRegions * cap_r * sizeof(uint32_t)) + 2 * al(ctrs * sizeof(uint32_t)) +
+         al(kEsRegions * cap_r * sizeof(uint4) * 4) + al((size_t)cap_lanes * sizeof(float)) +
          al((size_t)cap_lanes * 8 * sizeof(uint32_t));
 }
 
 void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes) {
   const auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t cap_node = (size_t)cap_lanes * kMaxDepth;
-  const size_t cap_task = cap_node;
+  const size_t cap_r = (size_t)cap_lanes * kMaxDepth / kEsRegions;  // nodes and tasks per region
+  const size_t ctrs = (size_t)(kEsGens + 1) * kEsRegions * kEsCtrStride;
   char* p = base;
   const auto take = [&](size_t bytes) {
     char* q = p;
     p += al(bytes);
     return q;
   };
-  b.task[0] = reinterpret_cast<uint32_t*>(take(cap_task * sizeof(uint32_t)));
-  b.task[1] = reinterpret_cast<uint32_t*>(take(cap_task * sizeof(uint32_t)));
-  b.task_cnt = reinterpret_cast<uint32_t*>(take((kEsGens + 1) * sizeof(uint32_t)));
-  b.node_cnt = reinterpret_cast<uint32_t*>(take((kEsGens + 1) * sizeof(uint32_t)));
-  b.node = reinterpret_cast<uint4*>(take(cap_node * sizeof(uint4) * 4));
+  b.task[0] = reinterpret_cast<uint32_t*>(take(kEsRegions * cap_r * sizeof(uint32_t)));
+  b.task[1] = reinterpret_cast<uint32_t*>(take(kEsRegions * cap_r * sizeof(uint32_t)));
+  b.task_cnt = reinterpret_cast<uint32_t*>(take(ctrs * sizeof(uint32_t)));
+  b.node_cnt = reinterpret_cast<uint32_t*>(take(ctrs * sizeof(uint32_t)));
+  b.node = reinterpret_cast<uint4*>(take(kEsRegions * cap_r * sizeof(uint4) * 4));
   b.root_v = reinterpret_cast<float*>(take((size_t)cap_lanes * sizeof(float)));
   b.tr_acc = reinterpret_cast<uint32_t*>(take((size_t)cap_lanes * 8 * sizeof(uint32_t)));
-  b.cap_task = (uint32_t)cap_task;
-  b.cap_node = (uint32_t)cap_node;
+  b.cap_task_r = (uint32_t)cap_r;
+  b.cap_node_r = (uint32_t)cap_r;
 }
 
 
