@@ -2414,8 +2414,11 @@ __device__ __forceinline__ uint32_t es_region_of(const EsRegionScan& rs, uint32_
   return lo;
 }
 
+// 6 waves per SIMD for 8-wide rows (80 registers, a few spilled: 3.49 ->
+// 3.25 ms per launch of 2^20 traversals at 6-max 100bb against the
+// compiler's 124 registers and 4 waves); 16-wide rows keep 4 waves.
 template <int ROW, bool TRACE>
-__global__ __launch_bounds__(256) void k_es_gen(LaunchCfg lc, const DevSeg* __restrict__ segs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ROW <= 8 ? 6 : 4))) void k_es_gen(LaunchCfg lc, const DevSeg* __restrict__ segs,
                                                 const DevGame* __restrict__ tenants, DevScratch sc, uint32_t g) {
   constexpr int WG = 256;
   __shared__ uint32_t s_sb[kMaxPlayers * WG];
