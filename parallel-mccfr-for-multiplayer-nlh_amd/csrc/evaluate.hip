@@ -15,7 +15,8 @@
 // lanes of a wave touch 64 consecutive words.  Only seat 0 looks the strategy
 // up, so only its buckets are computed; the showdown walks the rank table for
 // the seats still in.  Net chips of seat 0 are summed with one 64-bit atomic
-// per wave (integer sums: the total does not depend on order).
+// per wave into one of 64 counters (integer sums: the total does not depend
+// on order).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -50,8 +51,10 @@ struct EvalArgs {
   const uint8_t* flop5;  // clamped harness buckets of every 5-card set
   const uint8_t* turn6;  // ... of every 6-card set
   int32_t* per_hand;     // [count] or null
-  unsigned long long* sum;
+  unsigned long long* sum;  // [kEvalSums][kEvalSumStride]: partial sums of seat 0's net chips
 };
+constexpr uint32_t kEvalSums = 64;
+constexpr uint32_t kEvalSumStride = 16;
 
 // {fold, call, raise} of the four scripted archetypes (eval.h:5-13).
 __constant__ float c_arch[4][3] = {{0.08f, 0.72f, 0.20f}, {0.60f, 0.30f, 0.10f}, {0.10f, 0.20f, 0.70f},
@@ -326,10 +329,12 @@ __global__ __launch_bounds__(kEvalWG) void k_eval_hands(const DevGame g, const E
     }
     if (e.per_hand) e.per_hand[i] = (int32_t)net0;
   }
-  // one 64-bit atomic per wave
+  // one 64-bit atomic per wave, on one of kEvalSums counters (128 bytes apart:
+  // atomics on one address serialize across the device); the host adds them
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) net0 += __shfl_xor(net0, off, 64);
-  if ((t & 63) == 0 && net0 != 0) atomicAdd(e.sum, (unsigned long long)net0);
+  if ((t & 63) == 0 && net0 != 0)
+    atomicAdd(e.sum + (size_t)((t >> 6) % kEvalSums) * kEvalSumStride, (unsigned long long)net0);
 }
 
 __global__ __launch_bounds__(256) void k_scatter_rows(const uint32_t* __restrict__ keys,
@@ -359,11 +364,12 @@ int run_eval(mccfr_ctx* c, const DevGame& g, const float* P, int n_players, int 
   if (rc) return rc;
   unsigned long long* d_sum = nullptr;
   int32_t* d_hand = nullptr;
-  HIPX(c, hipMalloc(&d_sum, sizeof(unsigned long long)));
+  constexpr size_t kSumBytes = (size_t)kEvalSums * kEvalSumStride * sizeof(unsigned long long);
+  HIPX(c, hipMalloc(&d_sum, kSumBytes));
   hipError_t err = per_hand && n_hands > 0
                        ? hipMalloc(&d_hand, (size_t)std::min<uint64_t>(kEvalChunk, (uint64_t)n_hands) * 4)
                        : hipSuccess;
-  if (err == hipSuccess) err = hipMemsetAsync(d_sum, 0, sizeof(unsigned long long), c->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(d_sum, 0, kSumBytes, c->stream);
   EvalArgs e{};
   e.P = P;
   e.slot_mask = (uint32_t)((1ull << g.table_log2) - 1ull);
@@ -391,9 +397,11 @@ int run_eval(mccfr_ctx* c, const DevGame& g, const float* P, int n_players, int 
     if (err == hipSuccess && per_hand)
       err = hipMemcpyAsync(per_hand + done, d_hand, (size_t)cnt * 4, hipMemcpyDeviceToHost, c->stream);
   }
-  unsigned long long sum = 0;
-  if (err == hipSuccess) err = hipMemcpyAsync(&sum, d_sum, 8, hipMemcpyDeviceToHost, c->stream);
+  unsigned long long parts[kEvalSums * kEvalSumStride] = {};
+  if (err == hipSuccess) err = hipMemcpyAsync(parts, d_sum, kSumBytes, hipMemcpyDeviceToHost, c->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  unsigned long long sum = 0;  // (mod 2^64: the two's-complement total)
+  for (uint32_t k = 0; k < kEvalSums; ++k) sum += parts[k * kEvalSumStride];
   (void)hipFree(d_sum);
   if (d_hand) (void)hipFree(d_hand);
   if (err != hipSuccess) return fail(c, MCCFR_E_HIP, hipGetErrorString(err));
