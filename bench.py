@@ -479,7 +479,8 @@ def main():
     ap.add_argument("--profile-hands", type=int, default=0,
                     help="with --profile: also run the hand-evaluation microbenchmark on 2^N hands")
     ap.add_argument("--profile-es", type=int, default=0,
-                    help="with --profile: also run this many external-sampling launches of 2^20 traversals")
+                    help="with --profile: also run this many external-sampling launches of 2^20 traversals "
+                         "(after 6 warm-up launches, as the bench leg)")
     ap.add_argument("--walk-shape", default="",
                     help="with --profile: the walk's launch shape as WG:TPL (workgroup lanes : traversals "
                          "per lane, 0 = persistent waves); default the library's 256:1")
@@ -582,7 +583,7 @@ def main():
         if args.profile_hands:
             result["hand_eval"] = hand_bench(ctx, args.profile_hands)
         if args.profile_es:
-            result["external_sampling"] = es_bench(mccfr, warmup=2, steps=args.profile_es,
+            result["external_sampling"] = es_bench(mccfr, warmup=6, steps=args.profile_es,
                                                    table_seed=args.table_seed)
         print(json.dumps(result), flush=True)
         return
