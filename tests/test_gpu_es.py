@@ -149,3 +149,33 @@ def test_es_launch_beyond_the_record_scratch_is_refused(hand_table):
         ctx.run_launch(0, 1, 0, 1)  # one tree fits now
         R2, _, _ = ctx.download()
         assert not np.array_equal(R0, R2)
+
+
+def test_es_depth_first_and_generation_walks_agree(hand_table, golden_lut):
+    """A launch of several external-sampling segments takes the depth-first
+    walk (k_walk_es), a launch of one segment the generation walk (k_es_gen /
+    k_es_back).  Both explore the same tasks with the same streams: every
+    traversal's record is bit-exact between the two and against the oracle."""
+    cfgs = [es(mccfr.cfg_3p_20bb(table_log2=12, seed=40)), es(mccfr.cfg_6p_100bb(table_log2=12, seed=41)),
+            es(mccfr.reference_cfg(num_players=2, starting_stack=200, table_log2=12, seed=42))]
+    n = 512
+    with mccfr.Context(mccfr.cfg_3p_20bb(table_log2=4), 64 * n) as ctx:
+        ctx.set_hand_table(hand_table)
+        ids = [ctx.tenant_create(c) for c in cfgs]
+        for t in ids:
+            ctx.tenant_randomize(t, 50 + t)
+        pre = [ctx.tenant_download(t) for t in ids]
+        segs = [(t, n, 1, 2, 1000 * (k + 1)) for k, t in enumerate(ids)]
+        mixed = ctx.run_segments(segs, trace=True)  # depth first
+        for k, (t, c, (R0, S0, sig0)) in enumerate(zip(ids, cfgs, pre)):
+            part = mixed[k * n:(k + 1) * n]
+            _, _, _, _, otr = oracle.replay_launch(c, hand_table, golden_lut, R0, sig0, 1, 2, 1000 * (k + 1), n)
+            for f in FIELDS:
+                assert np.array_equal(part[f], otr[f]), (k, f)
+            assert np.array_equal(part["utility"].view(np.uint32), otr["utility"].view(np.uint32))
+            ctx.tenant_upload(t, R0, S0)
+            alone = ctx.run_segments([segs[k]], trace=True)  # by generations
+            for f in FIELDS + ("utility",):
+                assert np.array_equal(alone[f].view(np.uint32) if f == "utility" else alone[f],
+                                      part[f].view(np.uint32) if f == "utility" else part[f]), (k, f)
+            assert alone["updates"].astype(np.int64).sum() > n
