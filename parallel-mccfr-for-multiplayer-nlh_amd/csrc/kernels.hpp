@@ -24,10 +24,12 @@ constexpr uint32_t kSortBins = 1u << kDigitBits;
 // External sampling by generations (one-segment launches): generation 0's
 // tasks are the traversals' roots; a task walks (sampling the other seats'
 // actions) to a leaf or to an update-player node, and such a node makes one
-// task of the next generation per legal action.  A task starts at least one
-// level below its node, so the depth guard (20 levels) bounds the
-// generations.
-constexpr uint32_t kEsGens = 21;
+// task of the next generation per legal action.  Two update-player decisions
+// on a path are at least two levels apart (another seat acts between them,
+// or a street change takes a level), so under the depth guard (decisions at
+// levels 0..19) a path holds at most 10 update nodes: tasks of generations
+// 0..10.  The last generation flags a node it would make.
+constexpr uint32_t kEsGens = 11;
 // Nodes and tasks are allocated in kEsRegions regions, each with its own
 // counters on a line of its own (a wave uses region wave % kEsRegions), since
 // atomics on one address serialize across the device.

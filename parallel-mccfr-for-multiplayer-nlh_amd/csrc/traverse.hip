@@ -2648,6 +2648,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ROW <= 8 ? 
       const uint32_t slot = info_hash((uint32_t)player, Bp & 0xFFu, bbk, (uint32_t)street, ab, slot_mask);
       ++dec;
       if (player == up) {
+        if (g + 1u >= kEsGens) {
+          // cannot happen (see kEsGens); flagged rather than left unwalked
+          sc.cnt->es_overflow = 2u;
+          status = 3;
+          continue;
+        }
         status = 4;
         n_slot = slot;
         n_vm = vm;
