@@ -249,7 +249,11 @@ int ensure_es_stack(mccfr_ctx* c, int row, uint32_t lanes) {
   if (!rc && c->es_gen_cap != c->sc.cap_lanes) {
     // nodes and tasks per generation for the sort input's records
     rc = grow(c->es_gen, c->es_gen_size, es_gen_bytes(c->sc.cap_lanes), "external-sampling generations: ");
-    if (!rc) {
+    if (rc) {  // the old carving may point into freed memory: forget it
+      c->es_gen_bufs = EsGenBufs{};
+      c->sc.gen = EsGenBufs{};
+      c->es_gen_cap = 0;
+    } else {
       HIPX(c, hipStreamSynchronize(c->stream));
       drop_graph(c);  // a graph holds the old carving
       carve_es_gen(c->es_gen_bufs, c->es_gen, c->sc.cap_lanes);
