@@ -11,8 +11,8 @@ namespace mccfr {
 // written to the lane's node list in HBM when they are reached.
 constexpr int kRegPending = 4;
 
-// Radix sort of the launch's records: ceil(key_bits / 10) passes with the key
-// bits spread evenly over them (26-bit keys: three 9-bit digits), tiles of
+// Radix sort of the launch's records: ceil(sort_bits / 10) passes with the
+// sorted bits spread evenly over them (26 bits: three 9-bit digits), tiles of
 // kSortWG threads x kSortItems keys.
 constexpr int kSortWG = 256;
 constexpr int kSortItems = 8;
@@ -134,14 +134,15 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 
 // One launch (traverse.hip): deal and walk of every segment's traversals
 // (launch_walk), then records -> sort -> owner-computes table update
-// (launch_reduce).  `trace` (device, indexed by lane) may be null.
+// (launch_reduce, whose stable radix passes order the records by the low
+// `sort_bits` bits of their keys).  `trace` (device, indexed by lane) may be null.
 // after_deal / after_walk (may be null): recorded after k_deal and after k_walk
 // (before k_showdown), for per-kernel timing.
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
                  hipEvent_t after_deal = nullptr, hipEvent_t after_walk = nullptr);
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                   const DevScratch& sc, uint32_t key_bits);
+                   const DevScratch& sc, uint32_t sort_bits);
 
 // Bytes of external-sampling walk frames for a launch of `lanes` lanes.
 size_t es_stack_bytes(int row, uint32_t lanes);

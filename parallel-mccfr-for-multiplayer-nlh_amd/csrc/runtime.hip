@@ -298,6 +298,7 @@ struct Prepared {
   std::vector<DevSeg> segs;
   LaunchCfg lc{};
   uint32_t key_bits = 0;
+  uint32_t sort_bits = 0;  // the low key bits the radix passes order by
   int row = 8;
 };
 
@@ -347,6 +348,13 @@ int prepare(mccfr_ctx* c, const HostSeg* hs, uint32_t n, const DevScratch& sc, P
   p.lc.cap_lanes = sc.cap_lanes;
   p.key_bits = slot_bits + seg_bits + es_bit;
   p.lc.null_key = p.key_bits == 32 ? 0xFFFFFFFFu : (1u << p.key_bits) - 1u;
+  // An outcome-sampling launch emits its records in lane order, so in segment
+  // order, and has no null records: stable passes over the slot bits alone
+  // leave equal slots of different segments adjacent but apart, in segment
+  // order, so each run of equal keys is still one slot of one tenant with its
+  // records in emission order.  External-sampling launches of several
+  // segments fill reservation tails with null keys, which must sort last.
+  p.sort_bits = p.lc.es ? p.key_bits : slot_bits;
   return MCCFR_OK;
 }
 
@@ -598,7 +606,7 @@ int issue_launch(mccfr_ctx* c, const HostSeg* segs, uint32_t n, mccfr_trace* hos
   if ((rc = record_event(c, 0))) return rc;
   launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), sc, c->shape,
               host_trace ? c->dtrace : nullptr, timing_event(c, 1), timing_event(c, 2));
-  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, sc, p.key_bits);
+  launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, sc, p.sort_bits);
   if ((rc = record_event(c, 3))) return rc;
   uint64_t total = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -957,7 +965,7 @@ int mccfr_train_tenants(mccfr_ctx* c, const uint32_t* tenants, const uint32_t* c
       hipGraph_t g = nullptr;
       HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape, nullptr);
-      launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.key_bits);
+      launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.sort_bits);
       launch_advance_segments(c->stream, c->d_segs, c->d_tenants, n);
       HIPX(c, hipStreamEndCapture(c->stream, &g));
       hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);

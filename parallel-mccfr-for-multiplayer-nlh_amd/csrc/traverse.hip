@@ -17,8 +17,9 @@
 //                 carries the record's action word and utility, so the table
 //                 update reads records in sorted order without gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
-//                 (key = segment << slot_bits | slot), the key bits spread
-//                 evenly over ceil(key_bits / 10) passes.
+//                 (key = segment << slot_bits | slot; records arrive in
+//                 segment order, so the passes need only the slot bits), the
+//                 sorted bits spread evenly over ceil(sort_bits / 10) passes.
 //   k_runs        every run of equal keys is one table slot of one tenant;
 //                 runs are listed (reduce_cand, in k_reduce_rows' leading
 //                 workgroups, finds the ends of the long ones).
@@ -3010,7 +3011,7 @@ void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes) {
 
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                   const DevScratch& sc, uint32_t key_bits) {
+                   const DevScratch& sc, uint32_t sort_bits) {
   const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
   hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
   if (!lc.es) hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
@@ -3020,10 +3021,10 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   const uint32_t max_rec_tiles = (uint32_t)((max_rec + kSortTile - 1) / kSortTile);
   // a multiple of the XCD count (the tile order assumes it)
   const uint32_t sgrid = std::min<uint32_t>((std::max<uint32_t>(max_rec_tiles, 1u) + 7u) / 8u * 8u, 2048u);
-  // as few passes of at most kDigitBits as the key needs, the bits spread
-  // evenly over them (26-bit keys: 3 passes of 9 bits, 512 digits each)
-  const uint32_t passes = key_bits == 0 ? 1u : (key_bits + kDigitBits - 1u) / kDigitBits;
-  const uint32_t bits = std::max(kMinDigitBits, std::min(kDigitBits, (key_bits + passes - 1u) / passes));
+  // as few passes of at most kDigitBits as the sorted bits need, spread
+  // evenly over them (26 bits: 3 passes of 9 bits, 512 digits each)
+  const uint32_t passes = sort_bits == 0 ? 1u : (sort_bits + kDigitBits - 1u) / kDigitBits;
+  const uint32_t bits = std::max(kMinDigitBits, std::min(kDigitBits, (sort_bits + passes - 1u) / passes));
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = bits * p;
     const uint32_t* kin = sc.key[p & 1];
