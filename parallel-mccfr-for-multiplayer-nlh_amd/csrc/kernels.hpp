@@ -137,7 +137,7 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 // (launch_reduce, whose stable radix passes order the records by the low
 // `sort_bits` bits of their keys).  `trace` (device, indexed by lane) may be null.
 // after_deal / after_walk (may be null): recorded after k_deal and after k_walk
-// (before k_showdown), for per-kernel timing.
+// (before the emission), for per-kernel timing.
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
                  hipEvent_t after_deal = nullptr, hipEvent_t after_walk = nullptr);

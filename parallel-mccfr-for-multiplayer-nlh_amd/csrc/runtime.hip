@@ -167,6 +167,10 @@ hipError_t alloc_scratch(DevScratch& s, uint32_t cap) {
   hipError_t e = hipMalloc(&base, scratch_bytes(cap));
   if (e != hipSuccess) return e;
   carve_scratch(s, base, cap);
+  // the walk adds each emission block's record count to blk_cnt, and the
+  // emission clears what it read: zero at the start
+  e = hipMemset(s.blk_cnt, 0, ((size_t)(cap + kEmitLanes - 1) / kEmitLanes) * 4);
+  if (e != hipSuccess) return e;
   return hipMemset(s.cnt, 0, sizeof(LaunchCounters));
 }
 
@@ -886,6 +890,15 @@ static int noalloc_scratch(mccfr_ctx* c, char** out) {
     if (cur != c->device) HIPX(c, hipSetDevice(c->device));
     void* ptr = nullptr;
     const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && ptr) {
+      // the emission block counts start at zero (the walk adds to them, the
+      // emission clears what it read): a fill kernel on the context's stream
+      // ahead of its first launch here
+      static_assert((kNoAllocLanes / kEmitLanes * 4) % 16 == 0, "whole 16-byte chunks");
+      DevScratch z;
+      carve_scratch(z, static_cast<char*>(ptr), kNoAllocLanes);
+      launch_zero(c->stream, z.blk_cnt, kNoAllocLanes / kEmitLanes * 4);
+    }
     if (cur != c->device) (void)hipSetDevice(cur);
     if (e != hipSuccess || !ptr)
       return fail(c, MCCFR_E_HIP, std::string("static launch scratch: ") + hipGetErrorString(e));

@@ -10,12 +10,13 @@
 //                 exactly one child (cfr_gpu.cu:800, :844), so a traversal is
 //                 one path.  Lanes keep the game state in LDS and record the
 //                 update-player nodes (key, sampled action, legal mask) and
-//                 the terminal utility.
-//   k_showdown    side-pot payoffs of the traversals that reached a showdown;
-//                 counts each emission block's records.
-//   k_emit_*      records in lane order -> (key, payload) arrays; the payload
-//                 carries the record's action word and utility, so the table
-//                 update reads records in sorted order without gathers.
+//                 the terminal utility, and counts each emission block's
+//                 records.
+//   k_emit_*      records in lane order -> (key, payload) arrays, with the
+//                 side-pot payoffs of the traversals that reached a showdown;
+//                 the payload carries the record's action word and utility,
+//                 so the table update reads records in sorted order without
+//                 gathers.
 //   k_rs_*        stable LSD radix sort of the records by key
 //                 (key = segment << slot_bits | slot; records arrive in
 //                 segment order, so the passes need only the slot bits), the
@@ -395,6 +396,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
   pool_n = uniform(pool_n);
   uint32_t pool_next = 0;  // wave-uniform
   const uint64_t lanes_below = (1ull << lane) - 1ull;
+  uint32_t rec_sum = 0;  // records of the lane's finished traversals
 
   while (true) {
     const uint64_t walking = __ballot(status == 0);
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
       // rather than kept in scalar registers through the walk (the compiler
       // barrier stops their loads from being hoisted out of the loop).
       asm volatile("" ::: "memory");
-      // a showdown's payoff is settled by k_showdown after the walk: keep
+      // a showdown's payoff is settled by k_emit after the walk: keep
       // every seat's investment and the folded seats
       if (status == 2) {
 #pragma unroll
@@ -429,6 +431,12 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
           trace[t] = r;
         }
         sc.node_count[t] = (uint8_t)npend;
+        // the traversal's records, counted for its emission block: summed per
+        // lane and added once per wave after the walk (a wave's traversals lie
+        // in one 256-lane block for 1, 2 or 4 per lane), or added at once in
+        // the persistent shape
+        if (jstride == 64u) rec_sum += (uint32_t)npend;
+        else if (npend != 0) atomicAdd(&sc.blk_cnt[t / kEmitLanes], (uint32_t)npend);
         sc.sd_state[t] = status == 2 ? (uint8_t)(kShowdownFlag | folded) : (uint8_t)0;
         if (status != 2) sc.node_u[t] = u;
         const int kr = npend < kRegPending ? npend : kRegPending;
@@ -629,6 +637,11 @@ __global__ __launch_bounds__(WG) void k_walk(LaunchCfg lc, const DevSeg* __restr
       }
     }
     ++depth;
+  }
+  if (jstride == 64u) {  // the wave's records, for its emission block
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) rec_sum += (uint32_t)__shfl_xor((int)rec_sum, d, 64);
+    if (lane == 0 && rec_sum != 0) atomicAdd(&sc.blk_cnt[base / kEmitLanes], rec_sum);
   }
 }
 
@@ -1231,59 +1244,12 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
   return before + incl - v;
 }
 
-
-// ---------------------------------------------------------------------------
-// Showdown payoffs (cfr_gpu.cu:694-752): side pots in ascending levels over
-// every seat's investment, each level shared by the best live ranks that
-// reach it.  One lane per traversal that ended in a showdown, after the walk;
-// the walk leaves the investments (sd_inv) and the folded seats (sd_state).
-// ---------------------------------------------------------------------------
-template <bool TRACE>
-__global__ __launch_bounds__(256) void k_showdown(LaunchCfg lc, const DevSeg* __restrict__ segs,
-                                                  const DevGame* __restrict__ tenants, DevScratch sc,
-                                                  mccfr_trace* __restrict__ trace) {
-  static_assert(kEmitLanes == 256, "a showdown workgroup is one emission block");
-  __shared__ uint32_t s_w[4];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = t < lc.lanes;
-  // every load is issued before the block scan (its barriers), so their
-  // latencies overlap it; lanes return only after the scan
-  const uint32_t cnt = in ? (uint32_t)sc.node_count[t] : 0u;
-  const uint32_t st = in ? (uint32_t)sc.sd_state[t] : 0u;
-  const bool showdown = (st & kShowdownFlag) != 0;
-  int N = 0, up = 0;
-  if (uniform(t & ~63u) < lc.lanes) {  // wave-uniform: the wave's segment
-    const DevSeg& Sg = segs[find_seg(segs, lc.n_segs, uniform(t & ~63u))];
-    N = tenants[uniform(Sg.tenant)].N;
-    up = Sg.update_player;
-  }
-  const size_t L = sc.cap_lanes;
-  int inv[kMaxPlayers];
-  uint32_t rkv[kMaxPlayers];
-#pragma unroll
-  for (int p = 0; p < kMaxPlayers; ++p) {
-    inv[p] = showdown && p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
-    rkv[p] = showdown && p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
-  }
-  {
-    // the emission block's record count (the walk has set every lane's count)
-    uint32_t total = 0;
-    (void)block_excl_scan_256(cnt, s_w, &total);
-    if (threadIdx.x == 0) sc.blk_cnt[blockIdx.x] = total;
-  }
-  if (!showdown) return;
-  const uint32_t folded = st & ~kShowdownFlag;
-  const float pay = showdown_payoff(up, folded, inv, rkv);
-  sc.node_u[t] = pay;
-  if (TRACE) trace[t].utility = pay;
-}
-
 // ---------------------------------------------------------------------------
 // Record emission: lane-ordered (key, (action | mask << 8, utility)) pairs.
 // Blocks of kEmitLanes lanes, one lane per thread.
 // ---------------------------------------------------------------------------
 
-// Exclusive scan of the block counts in place (one workgroup; k_showdown wrote
+// Exclusive scan of the block counts in place (one workgroup; the walk added
 // the counts); resets the launch counters and fills the per-segment update
 // context of the table update.
 __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* __restrict__ segs,
@@ -1394,17 +1360,31 @@ __global__ __launch_bounds__(1024) void k_emit_scan(LaunchCfg lc, const DevSeg* 
 
 // A block's records are staged in LDS (when they fit) and written out
 // contiguously, instead of each lane storing its own records.  One lane per
-// thread, so every read of the node lists ([node][lane]) is coalesced.
+// thread, so every read of the node lists ([node][lane]) is coalesced.  The
+// walk added every block's record count to blk_cnt and k_emit_scan turned the
+// counts into offsets; a block takes its offset and clears the entry for the
+// next launch's counts.  A traversal that ended in a showdown gets its payoff
+// here (the walk left every seat's investment and the folded seats); the
+// others carry the walk's utility.
 constexpr uint32_t kEmitStage = 2048;
-__global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
+template <bool TRACE>
+__global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, const DevSeg* __restrict__ segs,
+                                              const DevGame* __restrict__ tenants, DevScratch sc,
+                                              mccfr_trace* __restrict__ trace) {
   __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_base;
   __shared__ uint32_t s_key[kEmitStage];
   __shared__ uint2 s_val[kEmitStage];
   const uint32_t t = blockIdx.x * kEmitLanes + threadIdx.x;
-  const uint32_t c = t < lc.lanes ? (uint32_t)sc.node_count[t] : 0u;
+  const bool in = t < lc.lanes;
+  const uint32_t c = in ? (uint32_t)sc.node_count[t] : 0u;
+  const uint32_t st = in ? (uint32_t)sc.sd_state[t] : 0u;
+  // payoffs of the showdowns that emit records (traced launches: of all)
+  const bool showdown = (st & kShowdownFlag) != 0 && (TRACE || c != 0);
   const size_t L = sc.cap_lanes;
-  // a lane's first kAhead records and its utility are loaded before the scan,
-  // so their latency overlaps it (most lanes hold at most kAhead records)
+  // every load is issued before the block scan (its barriers), so their
+  // latencies overlap it: a lane's first kAhead records (most lanes hold at
+  // most kAhead), its utility or the showdown's investments and ranks
   constexpr uint32_t kAhead = 4;
   uint32_t ak[kAhead], aa[kAhead];
 #pragma unroll
@@ -1412,10 +1392,32 @@ __global__ __launch_bounds__(256) void k_emit(LaunchCfg lc, DevScratch sc) {
     ak[k] = k < c ? sc.node_slot[(size_t)k * L + t] : 0u;
     aa[k] = k < c ? sc.node_av[(size_t)k * L + t] : 0u;
   }
-  const uint32_t ub = c ? __float_as_uint(sc.node_u[t]) : 0u;
+  int N = 0, up = 0;
+  if (uniform(t & ~63u) < lc.lanes) {  // wave-uniform: the wave's segment
+    const DevSeg& Sg = segs[find_seg(segs, lc.n_segs, uniform(t & ~63u))];
+    N = tenants[uniform(Sg.tenant)].N;
+    up = Sg.update_player;
+  }
+  int inv[kMaxPlayers];
+  uint32_t rkv[kMaxPlayers];
+#pragma unroll
+  for (int p = 0; p < kMaxPlayers; ++p) {
+    inv[p] = showdown && p < N ? (int)sc.sd_inv[(size_t)p * L + t] : 0;
+    rkv[p] = showdown && p < N ? (uint32_t)sc.root_rank[(size_t)p * L + t] : 0u;
+  }
+  uint32_t ub = c && !showdown ? __float_as_uint(sc.node_u[t]) : 0u;
+  if (threadIdx.x == 0) {
+    s_base = sc.blk_cnt[blockIdx.x];
+    sc.blk_cnt[blockIdx.x] = 0u;  // (this thread's own load of it comes first)
+  }
   uint32_t total = 0;
   uint32_t pos = block_excl_scan_256(c, s_w, &total);
-  const uint32_t gbase = sc.blk_cnt[blockIdx.x];
+  const uint32_t gbase = s_base;
+  if (showdown) {
+    const float pay = showdown_payoff(up, st & ~kShowdownFlag, inv, rkv);
+    ub = __float_as_uint(pay);
+    if (TRACE) trace[t].utility = pay;
+  }
   const bool staged = total <= kEmitStage;  // block-uniform
   auto put = [&](uint32_t key, uint2 v) {
     if (staged) {
@@ -2967,11 +2969,13 @@ void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs
   else
     walk_dispatch<16>(s, lc, segs, tenants, ws, shape, trace);
   if (after_walk) (void)hipEventRecord(after_walk, s);
-  const dim3 grid((lc.lanes + 255) / 256);
+  // the records, in lane order, with the showdowns' payoffs
+  hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
+  const dim3 grid((lc.lanes + kEmitLanes - 1) / kEmitLanes);
   if (trace)
-    hipLaunchKernelGGL(k_showdown<true>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
+    hipLaunchKernelGGL(k_emit<true>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
   else
-    hipLaunchKernelGGL(k_showdown<false>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
+    hipLaunchKernelGGL(k_emit<false>, grid, dim3(256), 0, s, lc, segs, tenants, sc, trace);
 }
 
 size_t es_stack_bytes(int row, uint32_t lanes) {
@@ -3012,9 +3016,9 @@ void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes) {
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                    const DevScratch& sc, uint32_t sort_bits) {
-  const uint32_t nblk = (lc.lanes + kEmitLanes - 1) / kEmitLanes;
-  hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
-  if (!lc.es) hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(256), 0, s, lc, sc);
+  // (outcome-sampling records were emitted after the walk; external-sampling
+  // walks append theirs, and the scan only sets up the update)
+  if (lc.es) hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
   // at most lanes * kMaxDepth records (external sampling: up to the sort
   // input's capacity); the sort kernels loop over the tiles
   const uint64_t max_rec = (uint64_t)(lc.es ? sc.cap_lanes : lc.lanes) * kMaxDepth;

@@ -1,4 +1,4 @@
-"""The identity k_showdown (traverse.hip) uses for the update player's
+"""The identity the record emission (k_emit, traverse.hip) uses for the update player's
 showdown payoff, checked bit-for-bit against the reference's side-pot loop
 (cfr_gpu.cu:694-752: ascending investment levels, each layer shared by the
 best live ranks that reach it, FP32 shares added in level order).
