@@ -425,8 +425,9 @@ def large_table_run(mccfr, table_log2, batch_log2, traversals, table_seed):
                       "table_update": float(upd_ms[N:].mean())},
         "launch": {"records_sorted": recs, "long_runs": long_runs, "pieces": pieces,
                    "table_atomics": 0,
-                   # one 64-bit counter atomic per k_reduce_cand wave (4,096 records) holding a long run
-                   "counter_atomics_max": min(long_runs, -(-recs // 4096))},
+                   # one atomic per walk wave on its emission block's record count, and one
+                   # 64-bit counter atomic per long-run search wave (4,096 records) holding a long run
+                   "counter_atomics_max": B // 64 + min(long_runs, -(-recs // 4096))},
     }
     for name in ("large_table_l2.json", "large_table_traffic.json"):
         path = os.path.join(ROOT, "profiles", name)
