@@ -1680,6 +1680,10 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // most kShortRun records, is listed in the wave's region for k_reduce_rows;
 // the longer ones (at most 4 per wave) in the wave's region for
 // reduce_cand.  Plain stores, no atomics: every region has one writer.
+// The heads of a wave's deferred runs lie more than kShortRun apart (all but
+// the last are closed and long), so a wave defers at most 63 / (kShortRun + 1)
+// + 1 of them, and its list holds 4; the look-ahead is one ballot.
+static_assert(63 / (kShortRun + 1) + 1 <= 4 && kShortRun < 64, "a wave's deferred runs fit its list of 4");
 __global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys) {
   const uint32_t n = sc.cnt->n_rec;
   const int lane = threadIdx.x & 63;
