@@ -3053,7 +3053,10 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   }
   const uint32_t* keys = sc.key[passes & 1];
   const uint2* vals = sc.val[passes & 1];
-  const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 8192);
+  // (at most 2,048 workgroups: a wave's 64 records are little work, and a grid
+  // sized for the record bound dispatches mostly empty waves; 8,192 took
+  // 9.0 us per launch at 6-max 100bb against 7.6)
+  const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 2048);
   hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
   // the long-run search's workgroups lead the short runs' grid.  The reduce
   // grids (4 waves per workgroup, all grid-stride) are sized from the record
