@@ -93,6 +93,34 @@ def test_edge_shapes_parity(case, hand_table, golden_lut):
     assert np.count_nonzero(S) > 0
 
 
+def test_min_raise_war_fills_emission_blocks_beyond_their_stage(hand_table, golden_lut):
+    """Heads-up, every slot's regret row one-hot on the min-raise: both seats
+    min-raise until the depth guard, so every traversal holds 10 update-player
+    nodes and each 256-lane emission block 2,560 records, more than the
+    emission's LDS stage (k_emit writes them directly).  Checked against the
+    oracle like every other launch."""
+    cfg = mccfr.menu_cfg(2, 4000, [(mccfr.FOLD, 0, 1), (mccfr.CHECK_CALL, 0, 1), (mccfr.RAISE_MIN, 0, 1),
+                                   (mccfr.ALL_IN, 0, 1)], max_raises=0, table_log2=12, seed=34)
+    n, row = 2048, 8
+    with mccfr.Context(cfg, n) as ctx:
+        ctx.set_hand_table(hand_table)
+        R = np.zeros((1 << cfg.table_log2, row), np.float32)
+        R[:, 2] = 1.0
+        ctx.upload(R.reshape(-1), np.zeros(R.size, np.float32))
+        for L in range(2):
+            R0, S0, sig0 = ctx.download()
+            tr = ctx.run_launch(L, 1, L * n, n, trace=True)
+            assert (tr["updates"] == 10).all(), np.bincount(tr["updates"])
+            dR, dS, aR, aS, otr = oracle.replay_launch(cfg, hand_table, golden_lut, R0, sig0, L, 1, L * n, n)
+            for f in ("path_hash", "decisions", "updates", "terminal"):
+                assert np.array_equal(tr[f], otr[f]), (f, L)
+            assert np.array_equal(tr["utility"].view(np.uint32), otr["utility"].view(np.uint32)), L
+            R1, S1, sig1 = ctx.download()
+            eR, eS = helpers.expected_post(R0, S0, dR, dS, bool(cfg.use_cfr_plus))
+            assert helpers.within_tol(R1, eR, aR).all() and helpers.within_tol(S1, eS, aS).all(), L
+            assert np.array_equal(sig1, oracle.regret_matching(R1, row)), L
+
+
 def test_train_graph_equals_launch_sequence(hand_table, golden_lut):
     """mccfr_train (captured hipGraph, device-advanced launch state) gives the
     tables of the same launches issued one by one and checked against the oracle."""
