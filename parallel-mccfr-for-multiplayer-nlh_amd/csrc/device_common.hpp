@@ -87,6 +87,19 @@ struct DevSeg {
 };
 constexpr uint32_t kSegAlign = 256;
 
+// A segment's next launch in the training loop (cfr_gpu.cu:1303-1318): the
+// next traversal ids, the next seat of the tenant's update schedule (the first
+// seat of the next iteration after the last) and its strategy-sum weight.
+__host__ __device__ inline void advance_segment(DevSeg& s, const DevGame& G) {
+  s.first_tid += s.n;
+  if (++s.upd_idx >= (uint32_t)G.n_upd) {
+    s.upd_idx = 0;
+    s.iteration += 1;
+  }
+  s.update_player = G.upd_seats[s.upd_idx];
+  s.t_weight = G.use_linear_cfr ? s.iteration : 1;
+}
+
 // Per-launch constants passed by value to the kernels.
 struct LaunchCfg {
   uint32_t n_segs;

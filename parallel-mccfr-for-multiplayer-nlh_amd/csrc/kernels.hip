@@ -68,24 +68,6 @@ __global__ void k_set_tenant(DevGame* dst, DevGame g) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = g;
 }
 
-// Advances every segment to its tenant's next launch of the training loop
-// (cfr_gpu.cu:1303-1318: the next seat of the schedule, or the first seat of
-// the next iteration).
-__global__ void k_advance_segments(DevSeg* segs, const DevGame* __restrict__ tenants, uint32_t n_segs) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_segs; i += gridDim.x * blockDim.x) {
-    DevSeg s = segs[i];
-    const DevGame& G = tenants[s.tenant];
-    s.first_tid += s.n;
-    if (++s.upd_idx >= (uint32_t)G.n_upd) {
-      s.upd_idx = 0;
-      s.iteration += 1;
-    }
-    s.update_player = G.upd_seats[s.upd_idx];
-    s.t_weight = G.use_linear_cfr ? s.iteration : 1;
-    segs[i] = s;
-  }
-}
-
 // kernel_clamp_regrets (cfr_gpu.cu:1018-1022) over a whole table.
 template <int ROW>
 __global__ __launch_bounds__(256) void k_clamp(float* __restrict__ R, uint64_t slots) {
@@ -313,9 +295,6 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 void launch_set_tenant(hipStream_t s, DevGame* dst, const DevGame& g) {
   hipLaunchKernelGGL(k_set_tenant, dim3(1), dim3(64), 0, s, dst, g);
-}
-void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants, uint32_t n_segs) {
-  hipLaunchKernelGGL(k_advance_segments, dim3(grid_for(n_segs, 256, 64)), dim3(256), 0, s, segs, tenants, n_segs);
 }
 void launch_zero(hipStream_t s, void* p, size_t bytes) {  // bytes: a multiple of 16
   hipLaunchKernelGGL(k_zero, dim3(grid_for(bytes / 16, 256, 1u << 16)), dim3(256), 0, s, (uint4*)p, bytes / 16);

@@ -130,7 +130,6 @@ struct WalkShape {
 
 void launch_build_buckets(hipStream_t s, const int32_t* hr, BucketTables b, bool clamp = false);
 void launch_set_tenant(hipStream_t s, DevGame* dst, const DevGame& g);
-void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants, uint32_t n_segs);
 
 // One launch (traverse.hip): deal and walk of every segment's traversals
 // (launch_walk), then records -> sort -> owner-computes table update
@@ -141,8 +140,11 @@ void launch_advance_segments(hipStream_t s, DevSeg* segs, const DevGame* tenants
 void launch_walk(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
                  const DealInputs& in, const DevScratch& sc, WalkShape shape, mccfr_trace* trace,
                  hipEvent_t after_deal = nullptr, hipEvent_t after_walk = nullptr);
+// advance (may be null): the launch's segment table, moved on to the next
+// launch's segments by run discovery (k_runs) once the emission has read it
+// (a captured training launch: one kernel fewer per replay).
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                   const DevScratch& sc, uint32_t sort_bits);
+                   const DevScratch& sc, uint32_t sort_bits, DevSeg* advance = nullptr);
 
 // Bytes of external-sampling walk frames for a launch of `lanes` lanes.
 size_t es_stack_bytes(int row, uint32_t lanes);

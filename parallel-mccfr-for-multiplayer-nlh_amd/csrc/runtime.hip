@@ -391,7 +391,7 @@ DealInputs deal_inputs(const mccfr_ctx* c) {
   return in;
 }
 
-// The host mirror of k_advance_segments.
+// The host mirror of advance_segment (device_common.hpp).
 void advance(const Tenant& t, HostSeg& s) {
   s.first_tid += s.n;
   if (++s.upd_idx >= (uint32_t)t.hg.n_upd) {
@@ -974,12 +974,12 @@ int mccfr_train_tenants(mccfr_ctx* c, const uint32_t* tenants, const uint32_t* c
     if ((rc = stage_segments(c, p.segs))) return rc;
     if (!c->gexec || c->graph_key != key) {
       drop_graph(c);
-      // one batched launch: [deal + walk -> records -> sort -> table update -> advance segments]
+      // one batched launch: deal + walk -> records -> sort -> table update, run
+      // discovery moving the segment table on to the next launch
       hipGraph_t g = nullptr;
       HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       launch_walk(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, deal_inputs(c), c->sc, c->shape, nullptr);
-      launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.sort_bits);
-      launch_advance_segments(c->stream, c->d_segs, c->d_tenants, n);
+      launch_reduce(c->stream, p.row, p.lc, c->d_segs, c->d_tenants, c->sc, p.sort_bits, c->d_segs);
       HIPX(c, hipStreamEndCapture(c->stream, &g));
       hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);

@@ -1686,10 +1686,19 @@ __global__ __launch_bounds__(kSortWG) void k_rs_scatter(const uint32_t* __restri
 // the last are closed and long), so a wave defers at most 63 / (kShortRun + 1)
 // + 1 of them, and its list holds 4; the look-ahead is one ballot.
 static_assert(63 / (kShortRun + 1) + 1 <= 4 && kShortRun < 64, "a wave's deferred runs fit its list of 4");
-__global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys) {
+__global__ __launch_bounds__(256) void k_runs(DevScratch sc, const uint32_t* __restrict__ keys, DevSeg* advance,
+                                             const DevGame* __restrict__ tenants, uint32_t n_segs) {
   const uint32_t n = sc.cnt->n_rec;
   const int lane = threadIdx.x & 63;
   const uint32_t stride = gridDim.x * blockDim.x;
+  // a captured training launch moves its segments on to the next launch here
+  // (every kernel that reads them has run)
+  if (advance)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_segs; i += stride) {
+      DevSeg sg = advance[i];
+      advance_segment(sg, tenants[sg.tenant]);
+      advance[i] = sg;
+    }
   const uint64_t lt = (1ull << lane) - 1ull;
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
     const uint32_t w = base >> 6;
@@ -3021,7 +3030,7 @@ void carve_es_gen(EsGenBufs& b, char* base, uint32_t cap_lanes) {
 
 
 void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* segs, const DevGame* tenants,
-                   const DevScratch& sc, uint32_t sort_bits) {
+                   const DevScratch& sc, uint32_t sort_bits, DevSeg* advance) {
   // (outcome-sampling records were emitted after the walk; external-sampling
   // walks append theirs, and the scan only sets up the update)
   if (lc.es) hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, s, lc, segs, tenants, sc);
@@ -3059,7 +3068,7 @@ void launch_reduce(hipStream_t s, int row, const LaunchCfg& lc, const DevSeg* se
   // sized for the record bound dispatches mostly empty waves; 8,192 took
   // 9.0 us per launch at 6-max 100bb against 7.6)
   const uint32_t rgrid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((max_rec / 4 + 255) / 256, 1), 2048);
-  hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys);
+  hipLaunchKernelGGL(k_runs, dim3(rgrid), dim3(256), 0, s, sc, keys, advance, tenants, lc.n_segs);
   // the long-run search's workgroups lead the short runs' grid.  The reduce
   // grids (4 waves per workgroup, all grid-stride) are sized from the record
   // bound, so a small launch does not dispatch thousands of empty workgroups
